@@ -1,0 +1,169 @@
+/*
+ * tn_abi.h -- C ABI of the MI355X (gfx950) DMRG hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns
+ * an int status: 0 = ok, < 0 = error (HIP failure, bad argument, bad state),
+ * > 0 = numerical warning (e.g. eigensolver not converged).  tn_last_error()
+ * gives the message of the last failure on a context.  One host thread per
+ * context; host buffers stay owned by the caller; calls are synchronous.
+ *
+ * Layouts at this boundary are the reference's (0-based):
+ *   site tensor   psi[m + d*(i1 + D1*i2)]         (Containers/Matrix6.H:110-111)
+ *   environment   F[w + Dw*(i + D*j)], i bra, j ket (Containers/Vector3.H:99-102)
+ *   MPO block     W[((w1*Dw2 + w2)*d + m)*d + n], m bra, n ket
+ *                                                  (Operators/OperatorValuedMatrix.H:11)
+ *   dense matrix  column-major (LAPACK convention of the reference solvers)
+ */
+#ifndef TN_ABI_H
+#define TN_ABI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tn_ctx tn_ctx;
+typedef struct tn_chain tn_chain;
+
+enum tn_status {
+    TN_OK = 0,
+    TN_WARN_NOT_CONVERGED = 1,
+    TN_ERR_HIP = -1,
+    TN_ERR_ARG = -2,
+    TN_ERR_STATE = -3,
+    TN_ERR_NOMEM = -4
+};
+
+/* arithmetic of the contractions (Heff apply / environment updates) */
+enum tn_precision { TN_FP64 = 0, TN_FP32 = 1, TN_BF16 = 2 };
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces nothing in the reference (it has no device); owns the HIP stream
+ * and scratch memory.  One context per chain or per batch, no globals. */
+int tn_ctx_create(int device, tn_ctx** ctx);
+int tn_ctx_destroy(tn_ctx* ctx);
+const char* tn_last_error(const tn_ctx* ctx);
+int tn_version(void);
+/* number of visible HIP devices (0 when none or when the runtime fails) */
+int tn_device_count(void);
+
+/* ---- reference solver seam (host pointers, reference layouts) ---------- */
+/* Binds the operands of one site's effective Hamiltonian: W (Dw1 x Dw2 blocks
+ * of d x d), left environment L (Dw1, D1, D1) and right environment R
+ * (Dw2, D2, D2).  Replaces the inputs of MPSSite::GetHeff
+ * (TensorNetworksImp/MPS/MPSSite_Contractions.C:155-193). */
+int tn_site_bind(tn_ctx* ctx, int d, int D1, int D2, int Dw1, int Dw2, const double* W,
+                 const double* L, const double* R, int precision);
+/* y = Heff x, N = d*D1*D2: the matrix-free product behind
+ * SparseEigenSolverClient<T>::DoMatVecContraction(int N, const T* x, T* y)
+ * (NumericalMethods/SparseEigenSolver.H:20-21), replacing the dense Heff of
+ * MPSSite::GetHeff. */
+int tn_heff_apply(tn_ctx* ctx, const double* x, double* y, int64_t N);
+/* Environment update through the bound site operator: dir 0 = IterateLeft_F
+ * (uses the bound L, writes F of shape (Dw2, D2, D2)), dir 1 = IterateRightF
+ * (uses the bound R, writes (Dw1, D1, D1)).  M is the site tensor.
+ * TensorNetworksImp/MPS/MPSSite_Contractions.C:292-400. */
+int tn_env_update(tn_ctx* ctx, int dir, const double* M, double* Fout);
+/* Lowest eigenpair(s) of the bound Heff by Lanczos with full FP64
+ * re-orthogonalisation; replaces LapackEigenSolver<dcmplx>::Solve(Heff, eps, 2)
+ * (NumericalMethods/LapackEigenSolver.C:62-109).  evec: in = start vector
+ * (zero -> seeded random), out = Ritz vector (unit norm).  evals[0..nev-1]
+ * ascending (nev <= 2).  Returns TN_WARN_NOT_CONVERGED if maxit is reached. */
+int tn_eig_lowest(tn_ctx* ctx, int64_t N, int nev, double tol, int maxit, double* evals,
+                  double* evec, int* n_matvec);
+/* Thin SVD A = U diag(s) Vt of a column-major m x n matrix, k = min(m,n):
+ * U (m x k), s (k, descending), Vt (k x n), column-major; U and Vt have
+ * orthonormal columns/rows even for zero singular values.  Replaces
+ * LapackSVDSolver<T>::Solve (NumericalMethods/LapackSVDSolver.C:42-90). */
+int tn_svd(tn_ctx* ctx, int m, int n, const double* A, double* U, double* s, double* Vt);
+/* Truncation rule of SVCompressorImp<T>::Compress
+ * (TensorNetworksImp/SVCompressorImp.C:19-73): returns the kept dimension,
+ * the discarded weight sum(s^2) and the squared-norm ratio rescale factor. */
+int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded,
+                       double* rescale);
+
+/* ---- device-resident chain (the sweep) ---------------------------------- */
+/* Per-site MPO: dw1[i], dw2[i] and W blocks concatenated in site order.
+ * Sites start as in MPSImp::InitSitesAndBonds (TensorNetworksImp/MPS/MPSImp.C:69-90):
+ * site 0 is 1 x D, bulk D x D, site L-1 is D x 1; bonds hold 1/D.
+ * Dcap bounds every bond dimension the chain will ever hold (device memory
+ * is sized from it). */
+int tn_chain_create(tn_ctx* ctx, int L, int d, const int* dw1, const int* dw2,
+                    const double* wcat, int D, int Dcap, tn_chain** chain);
+int tn_chain_destroy(tn_chain* chain);
+/* MPSSite::InitializeWith (TensorNetworksImp/MPS/MPSSite.C:84-140); state
+ * 0 = Product, 1 = Random (seeded U(-1,1) / sqrt(d D1 D2)), 2 = Neel. */
+int tn_chain_initialize(tn_chain* chain, int state, uint64_t seed);
+int tn_chain_get_dims(const tn_chain* chain, int* D1, int* D2);
+int tn_chain_get_site(tn_chain* chain, int site, double* M); /* reference layout */
+int tn_chain_set_site(tn_chain* chain, int site, int D1, int D2, const double* M);
+
+/* Options of the local update.  precision: Heff arithmetic; env_precision:
+ * environment-update arithmetic; refine: FP64 Lanczos polish of the Ritz
+ * vector after a reduced-precision solve (0/1). */
+typedef struct tn_sweep_opts {
+    double eig_tol;      /* Lanczos residual tolerance |H x - theta x| */
+    int max_krylov;      /* Krylov dimension cap (restarted beyond) */
+    int max_restarts;
+    int precision;       /* enum tn_precision */
+    int env_precision;   /* enum tn_precision */
+    int refine;          /* FP64 refinement after reduced-precision Lanczos */
+    double refine_tol;
+    int gemm_tile;       /* 0 = automatic tile choice */
+} tn_sweep_opts;
+
+typedef struct tn_sweep_stats {
+    int64_t n_matvec;          /* Heff applications (all precisions) */
+    int64_t n_matvec_fp64;     /* of which in FP64 */
+    int64_t n_env;             /* environment updates */
+    int64_t n_svd;             /* bond SVDs */
+    double heff_flops;         /* algorithmic flops of all Heff applies */
+    double heff_ms;            /* device time of all Heff applies (HIP events) */
+    double heff_gemm_ms;       /* device time of the two GEMM stages only */
+    double env_flops, env_ms;
+    double svd_ms, lanczos_ms, wall_ms;
+    double energy;             /* last site eigenvalue */
+} tn_sweep_stats;
+
+void tn_sweep_opts_default(tn_sweep_opts* o);
+/* MPS::Normalize(Direction) via SVD gauge moves, no compression
+ * (TensorNetworks/MPS.C:8, TensorNetworksImp/MPS/MPSSite_Normalization.C:45-78).
+ * dir 0 = DLeft, 1 = DRight. */
+int tn_chain_normalize(tn_chain* chain, int dir);
+/* MPSImp::LoadHeffCaches (TensorNetworksImp/MPS/MPSImp_Variational.C:120-124) */
+int tn_chain_load_caches(tn_chain* chain, const tn_sweep_opts* opts);
+/* MPSImp::Sweep (TensorNetworksImp/MPS/MPSImp_Variational.C:64-76): Refine every
+ * site in direction dir (0 = sites 0..L-1, 1 = L-1..0). */
+int tn_chain_sweep(tn_chain* chain, int dir, const tn_sweep_opts* opts, tn_sweep_stats* st);
+/* MPSImp::Refine (MPSImp_Variational.C:79-99) for one site. */
+int tn_chain_refine_site(tn_chain* chain, int site, int dir, const tn_sweep_opts* opts,
+                         tn_sweep_stats* st);
+/* MPSImp::IncreaseBondDimensions (TensorNetworksImp/MPS/MPSImp.C:123-147) */
+int tn_chain_increase_bond_dims(tn_chain* chain, int D);
+/* MPSImp::GetMaxDeltaE / GetMaxDeltal (MPSImp_Variational.C:152-173) */
+int tn_chain_deltas(const tn_chain* chain, double* max_de, double* max_dl);
+/* Bond b (between sites b and b+1): normalised singular values (n <= cap). */
+int tn_chain_bond_sv(const tn_chain* chain, int b, double* sv, int cap);
+/* MPSImp::GetExpectation(const MPO*) (TensorNetworksImp/MPS/MPSImp_ExpectationsAndDMs.C:33-45)
+ * for any MPO given like tn_chain_create's; FP64 left zipper on the device. */
+int tn_chain_expectation(tn_chain* chain, const int* dw1, const int* dw2, const double* wcat,
+                         double* value);
+/* Two-site update of bond (site, site+1) with the SVCompressor rule (Dmax, eps):
+ * an extension (the reference sweep is one-site only). */
+int tn_chain_sweep2(tn_chain* chain, int dir, int Dmax, double eps_sv,
+                    const tn_sweep_opts* opts, tn_sweep_stats* st, double* max_trunc);
+
+/* ---- diagnostics (no counterpart in the reference) ---------------------- */
+/* One batched GEMM through the device kernel, for numerics tests of the
+ * matrix-core path: C[b] = A[b] B[b] with operands rounded to tin
+ * (enum tn_precision values 0 fp64, 1 fp32, 2 bf16) and the result stored as
+ * tout before conversion back to double.  a_kmajor: A[b] is M x K row-major
+ * (else K x M); b_kmajor: B[b] is N x K row-major (else K x N).  M, N multiples
+ * of 32, K of 32.  tile: 0 automatic, else a forced tile code. */
+int tn_gemm_test(tn_ctx* ctx, int tin, int tout, int a_kmajor, int b_kmajor, int M, int N, int K,
+                 int batch, const double* A, const double* B, double* C, int tile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

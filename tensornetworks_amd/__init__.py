@@ -1,0 +1,27 @@
+"""tensornetworks_amd -- MI355X (gfx950) variational MPS ground-state sweep.
+
+Python view of the C ABI in include/tn_abi.h (libtn_hip.so, built in-tree by
+``python tensornetworks_amd/build.py``).  There is no CPU fallback: importing
+the package without the built library raises.
+"""
+from ._abi import (  # noqa: F401
+    LIB_PATH,
+    TN_BF16,
+    TN_FP32,
+    TN_FP64,
+    Chain,
+    Context,
+    SweepOpts,
+    SweepStats,
+    TnError,
+    lib,
+    svcompress_rank,
+)
+from .models import xxz_mpo, mpo_square, identity_mpo  # noqa: F401
+from .dmrg import FindVariationalGroundState, ScheduleLine  # noqa: F401
+
+__all__ = [
+    "Context", "Chain", "SweepOpts", "SweepStats", "TnError", "lib", "LIB_PATH",
+    "TN_FP64", "TN_FP32", "TN_BF16", "svcompress_rank", "xxz_mpo", "mpo_square",
+    "identity_mpo", "FindVariationalGroundState", "ScheduleLine",
+]

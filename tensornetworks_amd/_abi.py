@@ -1,0 +1,301 @@
+"""ctypes binding of include/tn_abi.h (libtn_hip.so)."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtn_hip.so")
+
+TN_FP64, TN_FP32, TN_BF16 = 0, 1, 2
+TN_OK, TN_WARN_NOT_CONVERGED = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class TnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tn error {code}: {msg}")
+        self.code = code
+
+
+class SweepOpts(ctypes.Structure):
+    _fields_ = [
+        ("eig_tol", ctypes.c_double),
+        ("max_krylov", ctypes.c_int),
+        ("max_restarts", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("env_precision", ctypes.c_int),
+        ("refine", ctypes.c_int),
+        ("refine_tol", ctypes.c_double),
+        ("gemm_tile", ctypes.c_int),
+    ]
+
+    @classmethod
+    def default(cls, **kw):
+        o = cls()
+        lib.tn_sweep_opts_default(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+
+class SweepStats(ctypes.Structure):
+    _fields_ = [
+        ("n_matvec", ctypes.c_int64),
+        ("n_matvec_fp64", ctypes.c_int64),
+        ("n_env", ctypes.c_int64),
+        ("n_svd", ctypes.c_int64),
+        ("heff_flops", ctypes.c_double),
+        ("heff_ms", ctypes.c_double),
+        ("heff_gemm_ms", ctypes.c_double),
+        ("env_flops", ctypes.c_double),
+        ("env_ms", ctypes.c_double),
+        ("svd_ms", ctypes.c_double),
+        ("lanczos_ms", ctypes.c_double),
+        ("wall_ms", ctypes.c_double),
+        ("energy", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python tensornetworks_amd/build.py` "
+            "(the HIP library is required: there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    i, d, v, i64, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64
+    vp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "tn_version": (i, []),
+        "tn_device_count": (i, []),
+        "tn_ctx_create": (i, [i, vp]),
+        "tn_ctx_destroy": (i, [v]),
+        "tn_last_error": (ctypes.c_char_p, [v]),
+        "tn_site_bind": (i, [v, i, i, i, i, i, _dp, _dp, _dp, i]),
+        "tn_heff_apply": (i, [v, _dp, _dp, i64]),
+        "tn_env_update": (i, [v, i, _dp, _dp]),
+        "tn_eig_lowest": (i, [v, i64, i, d, i, _dp, _dp, _ip]),
+        "tn_svd": (i, [v, i, i, _dp, _dp, _dp, _dp]),
+        "tn_svcompress_rank": (i, [i, _dp, i, d, _dp, _dp]),
+        "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i]),
+        "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
+        "tn_chain_destroy": (i, [v]),
+        "tn_chain_initialize": (i, [v, i, u64]),
+        "tn_chain_get_dims": (i, [v, _ip, _ip]),
+        "tn_chain_get_site": (i, [v, i, _dp]),
+        "tn_chain_set_site": (i, [v, i, i, i, _dp]),
+        "tn_sweep_opts_default": (None, [ctypes.POINTER(SweepOpts)]),
+        "tn_chain_normalize": (i, [v, i]),
+        "tn_chain_load_caches": (i, [v, ctypes.POINTER(SweepOpts)]),
+        "tn_chain_sweep": (i, [v, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats)]),
+        "tn_chain_refine_site": (i, [v, i, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats)]),
+        "tn_chain_increase_bond_dims": (i, [v, i]),
+        "tn_chain_deltas": (i, [v, _dp, _dp]),
+        "tn_chain_bond_sv": (i, [v, i, _dp, i]),
+        "tn_chain_expectation": (i, [v, _ip, _ip, _dp, _dp]),
+        "tn_chain_sweep2": (i, [v, i, i, d, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def P(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def IP(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+def svcompress_rank(s, Dmax, eps):
+    """SVCompressorImp::Compress rule -> (D, discarded, rescale)."""
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    disc, resc = ctypes.c_double(), ctypes.c_double()
+    D = lib.tn_svcompress_rank(len(s), P(s), Dmax, eps, ctypes.byref(disc), ctypes.byref(resc))
+    if D < 0:
+        raise TnError(D, "tn_svcompress_rank")
+    return D, disc.value, resc.value
+
+
+class Context:
+    """One HIP stream + scratch; the owner of every device call."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib.tn_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise TnError(rc, f"tn_ctx_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.tn_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, rc, what):
+        if rc < 0:
+            msg = lib.tn_last_error(self.h)
+            raise TnError(rc, f"{what}: {msg.decode() if msg else ''}")
+        return rc
+
+    # -- reference seam -------------------------------------------------------
+    def site_bind(self, W, L, R, d, D1, D2, precision=TN_FP64):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        Dw1, Dw2 = W.shape[0], W.shape[1]
+        self._bound = (np.ascontiguousarray(L, dtype=np.float64), np.ascontiguousarray(R, dtype=np.float64), W)
+        self.check(lib.tn_site_bind(self.h, d, D1, D2, Dw1, Dw2, P(W), P(self._bound[0]), P(self._bound[1]), precision),
+                   "tn_site_bind")
+
+    def heff_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.check(lib.tn_heff_apply(self.h, P(x), P(y), x.size), "tn_heff_apply")
+        return y
+
+    def env_update(self, direction, M, out_size):
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        F = np.zeros(out_size)
+        self.check(lib.tn_env_update(self.h, direction, P(M), P(F)), "tn_env_update")
+        return F
+
+    def eig_lowest(self, x0, tol=1e-12, maxit=200, nev=2):
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        ev = np.zeros(2)
+        nmv = ctypes.c_int()
+        rc = self.check(lib.tn_eig_lowest(self.h, x.size, nev, tol, maxit, P(ev), P(x), ctypes.byref(nmv)),
+                        "tn_eig_lowest")
+        return ev[:nev], x, nmv.value, rc
+
+    def svd(self, A):
+        A = np.asarray(A, dtype=np.float64)
+        m, n = A.shape
+        k = min(m, n)
+        Af = np.ascontiguousarray(A.T).ravel()  # column-major
+        U, s, Vt = np.zeros(m * k), np.zeros(k), np.zeros(k * n)
+        self.check(lib.tn_svd(self.h, m, n, P(Af), P(U), P(s), P(Vt)), "tn_svd")
+        return U.reshape(k, m).T.copy(), s, Vt.reshape(n, k).T.copy()
+
+    def gemm_test(self, A, B, tin, tout, a_kmajor, b_kmajor, M, N, K, batch=1, tile=0):
+        A = np.ascontiguousarray(A, dtype=np.float64).ravel()
+        B = np.ascontiguousarray(B, dtype=np.float64).ravel()
+        C = np.zeros(batch * M * N)
+        self.check(lib.tn_gemm_test(self.h, tin, tout, int(a_kmajor), int(b_kmajor), M, N, K, batch, P(A), P(B),
+                                    P(C), tile), "tn_gemm_test")
+        return C.reshape(batch, M, N)
+
+
+def _mpo_arrays(Ws):
+    dw1 = np.array([W.shape[0] for W in Ws], dtype=np.int32)
+    dw2 = np.array([W.shape[1] for W in Ws], dtype=np.int32)
+    wcat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(W, dtype=np.float64).ravel() for W in Ws]))
+    return dw1, dw2, wcat
+
+
+class Chain:
+    """Device-resident MPS of L sites with its MPO (list of W[i] arrays
+    shaped (Dw1, Dw2, d, d)); sites start as in MPSImp::InitSitesAndBonds."""
+
+    def __init__(self, ctx, Ws, D, Dcap=None):
+        self.ctx = ctx
+        self.Ws = Ws
+        self.L = len(Ws)
+        self.d = Ws[0].shape[2]
+        self._mpo = _mpo_arrays(Ws)
+        h = ctypes.c_void_p()
+        dw1, dw2, wcat = self._mpo
+        ctx.check(lib.tn_chain_create(ctx.h, self.L, self.d, IP(dw1), IP(dw2), P(wcat), D, Dcap or D, ctypes.byref(h)),
+                  "tn_chain_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.tn_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def initialize(self, state=1, seed=20261202):
+        self.ctx.check(lib.tn_chain_initialize(self.h, state, seed), "tn_chain_initialize")
+
+    def dims(self):
+        D1 = np.zeros(self.L, dtype=np.int32)
+        D2 = np.zeros(self.L, dtype=np.int32)
+        lib.tn_chain_get_dims(self.h, IP(D1), IP(D2))
+        return D1.tolist(), D2.tolist()
+
+    def site(self, i):
+        D1, D2 = self.dims()
+        M = np.zeros(self.d * D1[i] * D2[i])
+        self.ctx.check(lib.tn_chain_get_site(self.h, i, P(M)), "tn_chain_get_site")
+        return M
+
+    def set_site(self, i, D1, D2, M):
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        self.ctx.check(lib.tn_chain_set_site(self.h, i, D1, D2, P(M)), "tn_chain_set_site")
+
+    def normalize(self, direction):
+        self.ctx.check(lib.tn_chain_normalize(self.h, direction), "tn_chain_normalize")
+
+    def load_caches(self, opts=None):
+        opts = opts or SweepOpts.default()
+        self.ctx.check(lib.tn_chain_load_caches(self.h, ctypes.byref(opts)), "tn_chain_load_caches")
+
+    def sweep(self, direction, opts=None, stats=None):
+        opts = opts or SweepOpts.default()
+        st = stats if stats is not None else SweepStats()
+        self.ctx.check(lib.tn_chain_sweep(self.h, direction, ctypes.byref(opts), ctypes.byref(st)), "tn_chain_sweep")
+        return st
+
+    def sweep2(self, direction, Dmax, eps_sv=0.0, opts=None, stats=None):
+        opts = opts or SweepOpts.default()
+        st = stats if stats is not None else SweepStats()
+        tr = ctypes.c_double()
+        self.ctx.check(lib.tn_chain_sweep2(self.h, direction, Dmax, eps_sv, ctypes.byref(opts), ctypes.byref(st),
+                                           ctypes.byref(tr)), "tn_chain_sweep2")
+        return st, tr.value
+
+    def refine_site(self, i, direction, opts=None, stats=None):
+        opts = opts or SweepOpts.default()
+        st = stats if stats is not None else SweepStats()
+        self.ctx.check(lib.tn_chain_refine_site(self.h, i, direction, ctypes.byref(opts), ctypes.byref(st)),
+                       "tn_chain_refine_site")
+        return st
+
+    def increase_bond_dims(self, D):
+        self.ctx.check(lib.tn_chain_increase_bond_dims(self.h, D), "tn_chain_increase_bond_dims")
+
+    def deltas(self):
+        de, dl = ctypes.c_double(), ctypes.c_double()
+        lib.tn_chain_deltas(self.h, ctypes.byref(de), ctypes.byref(dl))
+        return de.value, dl.value
+
+    def bond_sv(self, b):
+        buf = np.zeros(65536)
+        n = lib.tn_chain_bond_sv(self.h, b, P(buf), buf.size)
+        if n < 0:
+            raise TnError(n, "tn_chain_bond_sv")
+        return buf[:n].copy()
+
+    def expectation(self, Ws=None):
+        dw1, dw2, wcat = self._mpo if Ws is None else _mpo_arrays(Ws)
+        v = ctypes.c_double()
+        self.ctx.check(lib.tn_chain_expectation(self.h, IP(dw1), IP(dw2), P(wcat), ctypes.byref(v)),
+                       "tn_chain_expectation")
+        return v.value

@@ -1,0 +1,1176 @@
+// tn_chain.hip -- device-resident MPS and the variational sweep behind the C ABI.
+//
+// Mirrors the reference's sweep bookkeeping (TensorNetworksImp/MPS/MPSImp_Variational.C,
+// MPSSite_Normalization.C, MPSSite_Contractions.C, Bond.C, MPSImp.C): thin SVD
+// gauge moves with bond dimensions min(rows, cols), edge rescaling, bond
+// singular values for the convergence test, zero-padded bond growth.
+// Device layouts (bond dims padded to multiples of 32, padding kept at zero):
+//   site  A[a][m][b]   [D1p][d][D2p]
+//   left  env after site i   [D2p][Dw2][D2p]  (bra, w, ket)
+//   right env from site i on [Dw1][D1p][D1p]  (w, bra, ket)
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "tn_engine.h"
+
+using namespace tn;
+
+struct tn_chain {
+    tn_ctx* ctx = nullptr;
+    int L = 0, d = 0, Dcap = 0, Dcapp = 0, Dwmax = 0;
+    std::vector<int> dw1, dw2;
+    std::vector<std::vector<double>> W;
+    std::vector<int> D1, D2;
+    size_t site_cap = 0, env_cap = 0;
+    DBuf sites_buf, spare_buf, env_buf, lo_buf, bnd_buf;
+    std::vector<double*> A, Lenv, Renv;
+    std::vector<void*> Llo, Rlo;
+    int lo_prec = -1;
+    double *bL = nullptr, *bR = nullptr;  // boundary environments, fp64
+    void *bLlo = nullptr, *bRlo = nullptr;
+    std::vector<DevMix> mixH, mixL, mixR;
+    // bonds b = 0..L-2
+    std::vector<std::vector<double>> lam;
+    std::vector<double> maxDelta, compErr;
+    std::vector<double> Emin, IterDE, GapE;
+    double* spare = nullptr;
+    // two-site scratch
+    DBuf theta, w12buf;
+    std::vector<std::vector<double>> W12;
+    std::vector<DevMix> mix2H;
+};
+
+namespace {
+
+int fail(tn_ctx* c, const TnError& e)
+{
+    if (c) c->err = e.msg;
+    return e.code;
+}
+
+#define ABI_BEGIN try {
+#define ABI_END(ctx)                                                  \
+    }                                                                 \
+    catch (const TnError& e) { return fail(ctx, e); }                 \
+    catch (const std::exception& e) { return fail(ctx, TnError{TN_ERR_STATE, e.what()}); }
+
+inline long psi_ref(int d, int D1, int m, int i1, int i2) { return m + (long)d * (i1 + (long)D1 * i2); }
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+// --- host <-> device site transfers (reference layout on the host) -------
+void upload_site(tn_chain* ch, int i, int D1, int D2, const double* Mref)
+{
+    const int d = ch->d, D1p = pad32(D1), D2p = pad32(D2);
+    TN_REQUIRE(D1 <= ch->Dcap && D2 <= ch->Dcap, TN_ERR_ARG, "site dims exceed Dcap");
+    std::vector<double> buf((size_t)D1p * d * D2p, 0.0);
+    for (int a = 0; a < D1; a++)
+        for (int m = 0; m < d; m++)
+            for (int b = 0; b < D2; b++) buf[((size_t)a * d + m) * D2p + b] = Mref[psi_ref(d, D1, m, a, b)];
+    TN_HIP(hipMemcpyAsync(ch->A[i], buf.data(), buf.size() * 8, hipMemcpyHostToDevice, ch->ctx->stream));
+    sync(ch->ctx);
+    ch->D1[i] = D1;
+    ch->D2[i] = D2;
+}
+
+void download_site(tn_chain* ch, int i, std::vector<double>& Mref)
+{
+    const int d = ch->d, D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+    std::vector<double> buf((size_t)D1p * d * D2p);
+    TN_HIP(hipMemcpyAsync(buf.data(), ch->A[i], buf.size() * 8, hipMemcpyDeviceToHost, ch->ctx->stream));
+    sync(ch->ctx);
+    Mref.assign((size_t)d * D1 * D2, 0.0);
+    for (int a = 0; a < D1; a++)
+        for (int m = 0; m < d; m++)
+            for (int b = 0; b < D2; b++) Mref[psi_ref(d, D1, m, a, b)] = buf[((size_t)a * d + m) * D2p + b];
+}
+
+// --- low precision environment copies -----------------------------------
+void ensure_lo(tn_chain* ch, int prec)
+{
+    if (prec == TN_FP64) return;
+    if (ch->lo_prec == prec) return;
+    tn_ctx* c = ch->ctx;
+    ch->lo_buf.ensure((size_t)2 * ch->L * ch->env_cap * 4 + 2 * 32 * 32 * ch->Dwmax * 4 + 1024);
+    char* base = (char*)ch->lo_buf.p;
+    const size_t es = prec == TN_FP32 ? 4 : 2;
+    for (int i = 0; i < ch->L; i++) {
+        ch->Llo[i] = base + (size_t)(2 * i) * ch->env_cap * es;
+        ch->Rlo[i] = base + (size_t)(2 * i + 1) * ch->env_cap * es;
+    }
+    char* bb = base + (size_t)2 * ch->L * ch->env_cap * es;
+    ch->bLlo = bb;
+    ch->bRlo = bb + (size_t)32 * 32 * ch->Dwmax * es + 256;
+    ch->lo_prec = prec;
+    // refresh every valid environment
+    make_lowprec(c, prec, ch->bL, ch->bLlo, (long)32 * 32 * ch->Dwmax);
+    make_lowprec(c, prec, ch->bR, ch->bRlo, (long)32 * 32 * ch->Dwmax);
+    for (int i = 0; i < ch->L; i++) {
+        make_lowprec(c, prec, ch->Lenv[i], ch->Llo[i], (long)ch->env_cap);
+        make_lowprec(c, prec, ch->Renv[i], ch->Rlo[i], (long)ch->env_cap);
+    }
+}
+
+// --- environment update of site i (dir 0: left env, dir 1: right env) ---
+void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_stats* st)
+{
+    tn_ctx* c = ch->ctx;
+    const int d = ch->d, D1p = pad32(ch->D1[i]), D2p = pad32(ch->D2[i]);
+    const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+    int eprec = o.env_precision;
+    if (eprec != TN_FP64 && eprec != o.precision) eprec = TN_FP64;
+    const int lo = (o.precision != TN_FP64) ? o.precision : -1;
+    if (lo >= 0) ensure_lo(ch, lo);
+    double t0 = now_ms();
+    if (dir == 0) {
+        mix_build(ch->mixL[i], MIX_ENVL, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+        const double* Lin = i > 0 ? ch->Lenv[i - 1] : ch->bL;
+        const void* Llo = i > 0 ? ch->Llo[i - 1] : ch->bLlo;
+        env_left(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixL[i], Lin, Llo, ch->A[i], ch->Lenv[i], o.gemm_tile);
+        if (lo >= 0) make_lowprec(c, lo, ch->Lenv[i], ch->Llo[i], (long)D2p * Dw2 * D2p);
+    } else {
+        mix_build(ch->mixR[i], MIX_ENVR, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+        const double* Rin = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
+        const void* Rlo = i < ch->L - 1 ? ch->Rlo[i + 1] : ch->bRlo;
+        env_right(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixR[i], Rin, Rlo, ch->A[i], ch->Renv[i], o.gemm_tile);
+        if (lo >= 0) make_lowprec(c, lo, ch->Renv[i], ch->Rlo[i], (long)Dw1 * D1p * D1p);
+    }
+    if (st) {
+        sync(c);
+        st->env_ms += now_ms() - t0;
+        st->n_env++;
+        const double D1 = ch->D1[i], D2 = ch->D2[i];
+        if (dir == 0)
+            st->env_flops += 2.0 * D1 * Dw1 * D1 * d * D2 + 2.0 * Dw1 * Dw2 * d * d * D1 * D2 + 2.0 * D2 * Dw2 * D2 * D1 * d;
+        else
+            st->env_flops += 2.0 * D1 * d * Dw2 * D2 * D2 + 2.0 * Dw1 * Dw2 * d * d * D1 * D2 + 2.0 * Dw1 * D1 * D1 * d * D2;
+    }
+}
+
+// --- Heff apply callback for the Lanczos driver ---------------------------
+struct HeffCall {
+    HeffOp op;
+    tn_sweep_stats* st;
+    int D1, D2;  // logical dims for the algorithmic flop count
+};
+
+void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y)
+{
+    const HeffCall* hc = (const HeffCall*)vop;
+    const bool timed = hc->st != nullptr;
+    heff_apply(c, hc->op, x, y, timed);
+    if (timed) {
+        heff_timing_read(c, &hc->st->heff_gemm_ms, &hc->st->heff_ms);
+        hc->st->heff_flops += hc->op.flops_alg(hc->D1, hc->D2);
+        hc->st->n_matvec++;
+        if (hc->op.precision == TN_FP64) hc->st->n_matvec_fp64++;
+    }
+}
+
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
+{
+    HeffOp op;
+    op.p = ch->d;
+    op.D1p = pad32(ch->D1[i]);
+    op.D2p = pad32(ch->D2[i]);
+    op.Dw1 = ch->dw1[i];
+    op.Dw2 = ch->dw2[i];
+    op.L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
+    op.R = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
+    op.precision = prec;
+    op.Llo = prec == TN_FP64 ? nullptr : (i > 0 ? ch->Llo[i - 1] : ch->bLlo);
+    op.Rlo = prec == TN_FP64 ? nullptr : (i < ch->L - 1 ? ch->Rlo[i + 1] : ch->bRlo);
+    mix_build(ch->mixH[i], MIX_HEFF, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream);
+    op.mix = &ch->mixH[i];
+    op.tile = tile;
+    return op;
+}
+
+// local eigensolve of site i; returns (e0, e1)
+void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1)
+{
+    tn_ctx* c = ch->ctx;
+    if (o.precision != TN_FP64) ensure_lo(ch, o.precision);
+    HeffCall hc;
+    hc.op = make_heff(ch, i, o.precision, o.gemm_tile);
+    hc.st = st;
+    hc.D1 = ch->D1[i];
+    hc.D2 = ch->D2[i];
+    const long N = hc.op.n();
+    double t0 = now_ms();
+    LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts);
+    if (o.precision != TN_FP64 && o.refine) {
+        HeffCall h64 = hc;
+        h64.op = make_heff(ch, i, TN_FP64, o.gemm_tile);
+        LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts);
+        r.e0 = r2.e0;
+        r.e1 = r2.e1;
+    }
+    if (st) st->lanczos_ms += now_ms() - t0;
+    *e0 = r.e0;
+    *e1 = r.e1;
+}
+
+// Bond.C:86-106 SetSingularValues
+void bond_set_sv(tn_chain* ch, int b, const std::vector<double>& s, double comp)
+{
+    const int n = (int)s.size();
+    double nr = 0;
+    for (double v : s) nr += v * v;
+    nr = std::sqrt(nr);
+    std::vector<double> sn(n), old(n, 0.0);
+    for (int k = 0; k < n; k++) sn[k] = nr > 0 ? s[k] / nr : 0.0;
+    for (int k = 0; k < n && k < (int)ch->lam[b].size(); k++) old[k] = ch->lam[b][k];
+    double md = 0;
+    for (int k = 0; k < n; k++) md = std::max(md, std::fabs(sn[k] - old[k]));
+    ch->maxDelta[b] = md;
+    ch->lam[b] = sn;
+    ch->compErr[b] += comp;
+}
+
+// resize a site keeping the overlapping block (MPSSite.C:156-171), host path
+void site_resize(tn_chain* ch, int i, int nD1, int nD2, bool corner_rule)
+{
+    const int d = ch->d, oD1 = ch->D1[i], oD2 = ch->D2[i];
+    if (oD1 == nD1 && oD2 == nD2) return;
+    std::vector<double> M;
+    download_site(ch, i, M);
+    std::vector<double> N((size_t)d * nD1 * nD2, 0.0);
+    for (int m = 0; m < d; m++)
+        for (int a = 0; a < std::min(oD1, nD1); a++)
+            for (int b = 0; b < std::min(oD2, nD2); b++) N[psi_ref(d, nD1, m, a, b)] = M[psi_ref(d, oD1, m, a, b)];
+    if (corner_rule)
+        for (int m = 0; m < d; m++)
+            for (int a = oD1 - 1; a < nD1; a++)
+                for (int b = oD2 - 1; b < nD2; b++)
+                    if (a >= 0 && b >= 0) N[psi_ref(d, nD1, m, a, b)] = 0.0;
+    upload_site(ch, i, nD1, nD2, N.data());
+}
+
+void rescale_site(tn_chain* ch, int i)
+{
+    tn_ctx* c = ch->ctx;
+    const long n = (long)pad32(ch->D1[i]) * ch->d * pad32(ch->D2[i]);
+    c->lanczos_small.ensure(sizeof(double) * (4 * 512 + 64));
+    double* h = c->lanczos_small.as<double>() + 4 * 512 + 16;
+    long pcap = mdot_part_size(n, 1);
+    c->part.ensure(pcap * 8);
+    TN_HIP(mdot(ch->A[i], n, 1, ch->A[i], n, h, c->part.as<double>(), (int)pcap, c->stream));
+    TN_HIP(dsqrt_scalar(h, h, c->stream));
+    TN_HIP(scale_by_inv(ch->A[i], h, n, c->stream));
+}
+
+// MPSSite_Normalization.C:45-78 SVDNormalize(lr, nullptr) incl. HandleEdges :20-43
+void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
+{
+    tn_ctx* c = ch->ctx;
+    hipStream_t s = c->stream;
+    const int d = ch->d, L = ch->L;
+    const int D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+    if (dir == 1 && i == 0) {
+        const int newD2 = L > 1 ? std::max((int)ch->lam[0].size(), d) : D2;
+        if (newD2 < D2) site_resize(ch, i, D1, newD2, false);
+        rescale_site(ch, i);
+        return;
+    }
+    if (dir == 0 && i == L - 1) {
+        const int newD1 = L > 1 ? std::max((int)ch->lam[L - 2].size(), d) : D1;
+        if (newD1 < D1) site_resize(ch, i, newD1, D2, false);
+        rescale_site(ch, i);
+        return;
+    }
+    double t0 = now_ms();
+    std::vector<double> sv;
+    if (dir == 0) {
+        // A as (D1*d) x D2: row (a,m) at (a*d+m)*D2p, column b
+        const int r = D1 * d, cc = D2, k = std::min(r, cc), kp = pad32(k);
+        c->svd_U.ensure((size_t)r * k * 8 + 64);
+        c->svd_Vt.ensure((size_t)kp * D2p * 8 + 64);
+        svd_device(c, ch->A[i], D2p, 1, r, cc, c->svd_U.as<double>(), k, r, c->svd_Vt.as<double>(), D2p, kp, sv);
+        bond_set_sv(ch, i, sv, 0.0);
+        // neighbour: M_{i+1} <- diag(s) Vt M_{i+1}
+        const int j = i + 1, D3p = pad32(ch->D2[j]);
+        std::vector<double> sv_pad(kp, 0.0);
+        std::copy(sv.begin(), sv.end(), sv_pad.begin());
+        c->pack.ensure(kp * 8 + 64);
+        TN_HIP(hipMemcpyAsync(c->pack.p, sv_pad.data(), kp * 8, hipMemcpyHostToDevice, s));
+        TN_HIP(scale_rows(c->svd_Vt.as<double>(), D2p, kp, D2p, c->pack.as<double>(), s));
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+        g.M = kp; g.N = d * D3p; g.K = D2p; g.batch = 1;
+        g.A = c->svd_Vt.p; g.lda = D2p;
+        g.B = ch->A[j]; g.ldb = (long)d * D3p;
+        g.C = ch->spare; g.ldc = (long)d * D3p; g.alpha = 1.0f;
+        TN_HIP(gemm(g, s));
+        std::swap(ch->A[j], ch->spare);
+        ch->D1[j] = k;
+        // site i <- U  (rows D1p*d, cols kp, zero padded)
+        TN_HIP(copy2(c->svd_U.as<double>(), k, 1, r, k, ch->A[i], kp, 1, D1p * d, kp, s));
+        ch->D2[i] = k;
+    } else {
+        // A as D1 x (d*D2): pack the logical matrix first
+        const int r = D1, cc = d * D2, k = std::min(r, cc), kp = pad32(k);
+        c->svd_A.ensure((size_t)r * cc * 8 + 64);
+        TN_HIP(copy3(ch->A[i], (long)d * D2p, D2p, 1, c->svd_A.as<double>(), r, d, D2, D2, s));
+        c->svd_U.ensure((size_t)D1p * kp * 8 + 64);
+        c->svd_Vt.ensure((size_t)kp * cc * 8 + 64);
+        svd_device(c, c->svd_A.as<double>(), cc, 1, r, cc, c->svd_U.as<double>(), kp, D1p,
+                   c->svd_Vt.as<double>(), cc, kp, sv);
+        bond_set_sv(ch, i - 1, sv, 0.0);
+        // neighbour: M_{i-1} <- M_{i-1} U diag(s)
+        const int j = i - 1, D0p = pad32(ch->D1[j]);
+        std::vector<double> sv_pad(kp, 0.0);
+        std::copy(sv.begin(), sv.end(), sv_pad.begin());
+        c->pack.ensure(kp * 8 + 64);
+        TN_HIP(hipMemcpyAsync(c->pack.p, sv_pad.data(), kp * 8, hipMemcpyHostToDevice, s));
+        TN_HIP(scale_cols(c->svd_U.as<double>(), kp, D1p, kp, c->pack.as<double>(), s));
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+        g.M = D0p * d; g.N = kp; g.K = D1p; g.batch = 1;
+        g.A = ch->A[j]; g.lda = D1p;
+        g.B = c->svd_U.p; g.ldb = kp;
+        g.C = ch->spare; g.ldc = kp; g.alpha = 1.0f;
+        TN_HIP(gemm(g, s));
+        std::swap(ch->A[j], ch->spare);
+        ch->D2[j] = k;
+        // site i <- Vt (k x d*D2) into [kp][d][D2p]
+        TN_HIP(copy3(c->svd_Vt.as<double>(), cc, D2, 1, ch->A[i], kp, d, D2, D2p, s));
+        ch->D1[i] = k;
+    }
+    if (st) {
+        sync(c);
+        st->svd_ms += now_ms() - t0;
+        st->n_svd++;
+    }
+}
+
+void refine_site(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_stats* st)
+{
+    double e0 = 0, e1 = 0;
+    eigensolve(ch, i, o, st, &e0, &e1);
+    ch->IterDE[i] = e0 - ch->Emin[i];
+    ch->Emin[i] = e0;
+    ch->GapE[i] = e1 - 1.0;  // MPSSite_Variational.C:29 as written
+    if (st) st->energy = e0;
+    svd_normalize(ch, i, dir, st);
+    env_update(ch, i, dir, o, st);
+}
+
+// mixing-table and Lanczos helpers for arbitrary physical dimension p
+void two_site_W(int d, int Dw1, int Dwm, int Dw3, const double* Wa, const double* Wb, std::vector<double>& W12)
+{
+    const int p = d * d;
+    W12.assign((size_t)Dw1 * Dw3 * p * p, 0.0);
+    for (int w = 0; w < Dw1; w++)
+        for (int w2 = 0; w2 < Dw3; w2++)
+            for (int wp = 0; wp < Dwm; wp++)
+                for (int m1 = 0; m1 < d; m1++)
+                    for (int n1 = 0; n1 < d; n1++) {
+                        double a = Wa[((w * Dwm + wp) * d + m1) * d + n1];
+                        if (a == 0.0) continue;
+                        for (int m2 = 0; m2 < d; m2++)
+                            for (int n2 = 0; n2 < d; n2++)
+                                W12[(((size_t)w * Dw3 + w2) * p + (m1 * d + m2)) * p + (n1 * d + n2)] +=
+                                    a * Wb[((wp * Dw3 + w2) * d + m2) * d + n2];
+                    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int tn_version(void) { return 1; }
+
+int tn_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int tn_ctx_create(int device, tn_ctx** out)
+{
+    if (!out) return TN_ERR_ARG;
+    *out = nullptr;
+    tn_ctx* c = new tn_ctx();
+    ABI_BEGIN
+    int n = 0;
+    TN_HIP(hipGetDeviceCount(&n));
+    TN_REQUIRE(device >= 0 && device < n, TN_ERR_ARG, "no such HIP device");
+    c->device = device;
+    TN_HIP(hipSetDevice(device));
+    TN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) TN_HIP(hipEventCreate(&e));
+    TN_HIP(hipHostMalloc((void**)&c->sc.h, 64 * sizeof(double)));
+    *out = c;
+    return TN_OK;
+    }
+    catch (const TnError& e) {
+        int code = e.code;
+        delete c;
+        return code;
+    }
+}
+
+int tn_ctx_destroy(tn_ctx* c)
+{
+    if (!c) return TN_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    tn::DBuf* bufs[] = {&c->T1, &c->T2, &c->xlo, &c->part, &c->krylov, &c->lanczos_small, &c->svd_X,
+                        &c->svd_V, &c->svd_U, &c->svd_Vt, &c->svd_s, &c->svd_perm, &c->svd_pairs,
+                        &c->svd_flag, &c->svd_A, &c->spare, &c->pack, &c->b_L, &c->b_R, &c->b_Llo,
+                        &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M};
+    for (auto* b : bufs) b->release();
+    c->b_mix_heff.buf.release();
+    c->b_mix_envl.buf.release();
+    c->b_mix_envr.buf.release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->sc.h) (void)hipHostFree(c->sc.h);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TN_OK;
+}
+
+const char* tn_last_error(const tn_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void tn_sweep_opts_default(tn_sweep_opts* o)
+{
+    if (!o) return;
+    o->eig_tol = 1e-10;
+    o->max_krylov = 64;
+    o->max_restarts = 8;
+    o->precision = TN_FP64;
+    o->env_precision = TN_FP64;
+    o->refine = 1;
+    o->refine_tol = 1e-10;
+    o->gemm_tile = 0;
+}
+
+int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
+{
+    // SVCompressorImp.C:19-73
+    if (N < 0 || (N > 0 && !s)) return TN_ERR_ARG;
+    int D = N;
+    double integrated = 0.0;
+    if (eps == 0.0) {
+        D = Dmax;
+        for (int id = D; id < N; id++) integrated += s[id] * s[id];
+    } else {
+        for (; D >= 1; D--) {
+            double s2 = s[D - 1] * s[D - 1];
+            if (integrated + s2 >= eps) break;
+            integrated += s2;
+        }
+        if (Dmax > 0 && D > Dmax) D = Dmax;
+    }
+    double r = 1.0;
+    if (D < N) {
+        double before = 0, after = 0;
+        for (int i = 0; i < N; i++) before += s[i] * s[i];
+        for (int i = 0; i < D; i++) after += s[i] * s[i];
+        if (after > 0.0) r = before / after;
+    }
+    if (D > N) D = N;
+    if (discarded) *discarded = integrated;
+    if (rescale) *rescale = r;
+    return D;
+}
+
+// ---- seam primitives ------------------------------------------------------
+int tn_site_bind(tn_ctx* c, int d, int D1, int D2, int Dw1, int Dw2, const double* W, const double* L,
+                 const double* R, int precision)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(d > 0 && D1 > 0 && D2 > 0 && Dw1 > 0 && Dw2 > 0 && W && L && R, TN_ERR_ARG, "tn_site_bind: bad arguments");
+    TN_REQUIRE(precision >= TN_FP64 && precision <= TN_BF16, TN_ERR_ARG, "tn_site_bind: bad precision");
+    const int D1p = pad32(D1), D2p = pad32(D2);
+    c->b_d = d; c->b_D1 = D1; c->b_D2 = D2; c->b_Dw1 = Dw1; c->b_Dw2 = Dw2; c->b_prec = precision;
+    c->b_W.assign(W, W + (size_t)Dw1 * Dw2 * d * d);
+    // L(w,i,j) = L[w + Dw1*(i + D1*j)] -> Lint[i][w][j] padded
+    std::vector<double> Li((size_t)D1p * Dw1 * D1p, 0.0), Ri((size_t)Dw2 * D2p * D2p, 0.0);
+    for (int i = 0; i < D1; i++)
+        for (int w = 0; w < Dw1; w++)
+            for (int j = 0; j < D1; j++) Li[((size_t)i * Dw1 + w) * D1p + j] = L[w + (size_t)Dw1 * (i + (size_t)D1 * j)];
+    for (int w = 0; w < Dw2; w++)
+        for (int i = 0; i < D2; i++)
+            for (int j = 0; j < D2; j++) Ri[((size_t)w * D2p + i) * D2p + j] = R[w + (size_t)Dw2 * (i + (size_t)D2 * j)];
+    c->b_L.ensure(Li.size() * 8);
+    c->b_R.ensure(Ri.size() * 8);
+    h2d(c, c->b_L.p, Li.data(), Li.size() * 8);
+    h2d(c, c->b_R.p, Ri.data(), Ri.size() * 8);
+    if (precision != TN_FP64) {
+        c->b_Llo.ensure(Li.size() * 4);
+        c->b_Rlo.ensure(Ri.size() * 4);
+        make_lowprec(c, precision, c->b_L.as<double>(), c->b_Llo.p, (long)Li.size());
+        make_lowprec(c, precision, c->b_R.as<double>(), c->b_Rlo.p, (long)Ri.size());
+    }
+    mix_build(c->b_mix_heff, MIX_HEFF, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+    const long n = (long)D1p * d * D2p;
+    c->b_x.ensure(n * 8);
+    c->b_y.ensure(n * 8);
+    sync(c);
+    c->bound = true;
+    return TN_OK;
+    ABI_END(c)
+}
+
+static HeffOp bound_op(tn_ctx* c)
+{
+    HeffOp op;
+    op.p = c->b_d;
+    op.D1p = pad32(c->b_D1);
+    op.D2p = pad32(c->b_D2);
+    op.Dw1 = c->b_Dw1;
+    op.Dw2 = c->b_Dw2;
+    op.L = c->b_L.as<double>();
+    op.R = c->b_R.as<double>();
+    op.Llo = c->b_prec == TN_FP64 ? nullptr : c->b_Llo.p;
+    op.Rlo = c->b_prec == TN_FP64 ? nullptr : c->b_Rlo.p;
+    op.precision = c->b_prec;
+    op.mix = &c->b_mix_heff;
+    op.tile = 0;
+    return op;
+}
+
+static void psi_to_dev(tn_ctx* c, const double* x, double* dst)
+{
+    const int d = c->b_d, D1 = c->b_D1, D2 = c->b_D2, D2p = pad32(D2), D1p = pad32(D1);
+    std::vector<double> buf((size_t)D1p * d * D2p, 0.0);
+    for (int a = 0; a < D1; a++)
+        for (int m = 0; m < d; m++)
+            for (int b = 0; b < D2; b++) buf[((size_t)a * d + m) * D2p + b] = x[psi_ref(d, D1, m, a, b)];
+    h2d(c, dst, buf.data(), buf.size() * 8);
+}
+static void psi_from_dev(tn_ctx* c, const double* src, double* y)
+{
+    const int d = c->b_d, D1 = c->b_D1, D2 = c->b_D2, D2p = pad32(D2), D1p = pad32(D1);
+    std::vector<double> buf((size_t)D1p * d * D2p);
+    d2h(c, buf.data(), src, buf.size() * 8);
+    for (int a = 0; a < D1; a++)
+        for (int m = 0; m < d; m++)
+            for (int b = 0; b < D2; b++) y[psi_ref(d, D1, m, a, b)] = buf[((size_t)a * d + m) * D2p + b];
+}
+
+int tn_heff_apply(tn_ctx* c, const double* x, double* y, int64_t N)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_heff_apply: no site bound");
+    TN_REQUIRE(x && y && N == (int64_t)c->b_d * c->b_D1 * c->b_D2, TN_ERR_ARG, "tn_heff_apply: N != d*D1*D2");
+    psi_to_dev(c, x, c->b_x.as<double>());
+    HeffOp op = bound_op(c);
+    heff_apply(c, op, c->b_x.as<double>(), c->b_y.as<double>(), false);
+    sync(c);
+    psi_from_dev(c, c->b_y.as<double>(), y);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_env_update(tn_ctx* c, int dir, const double* M, double* Fout)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_env_update: no site bound");
+    TN_REQUIRE(M && Fout && (dir == 0 || dir == 1), TN_ERR_ARG, "tn_env_update: bad arguments");
+    const int d = c->b_d, D1 = c->b_D1, D2 = c->b_D2, D1p = pad32(D1), D2p = pad32(D2);
+    const int Dw1 = c->b_Dw1, Dw2 = c->b_Dw2;
+    c->b_M.ensure((size_t)D1p * d * D2p * 8);
+    psi_to_dev(c, M, c->b_M.as<double>());
+    const int prec = c->b_prec;
+    if (dir == 0) {
+        mix_build(c->b_mix_envl, MIX_ENVL, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+        c->b_y.ensure((size_t)D2p * Dw2 * D2p * 8);
+        env_left(c, prec, d, D1p, D2p, Dw1, Dw2, c->b_mix_envl, c->b_L.as<double>(), c->b_Llo.p,
+                 c->b_M.as<double>(), c->b_y.as<double>(), 0);
+        sync(c);
+        std::vector<double> buf((size_t)D2p * Dw2 * D2p);
+        d2h(c, buf.data(), c->b_y.p, buf.size() * 8);
+        for (int i = 0; i < D2; i++)
+            for (int w = 0; w < Dw2; w++)
+                for (int j = 0; j < D2; j++) Fout[w + (size_t)Dw2 * (i + (size_t)D2 * j)] = buf[((size_t)i * Dw2 + w) * D2p + j];
+    } else {
+        mix_build(c->b_mix_envr, MIX_ENVR, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+        c->b_y.ensure((size_t)Dw1 * D1p * D1p * 8);
+        env_right(c, prec, d, D1p, D2p, Dw1, Dw2, c->b_mix_envr, c->b_R.as<double>(), c->b_Rlo.p,
+                  c->b_M.as<double>(), c->b_y.as<double>(), 0);
+        sync(c);
+        std::vector<double> buf((size_t)Dw1 * D1p * D1p);
+        d2h(c, buf.data(), c->b_y.p, buf.size() * 8);
+        for (int w = 0; w < Dw1; w++)
+            for (int i = 0; i < D1; i++)
+                for (int j = 0; j < D1; j++) Fout[w + (size_t)Dw1 * (i + (size_t)D1 * j)] = buf[((size_t)w * D1p + i) * D1p + j];
+    }
+    // restore the y buffer size for later applies
+    c->b_y.ensure((size_t)D1p * d * D2p * 8);
+    return TN_OK;
+    ABI_END(c)
+}
+
+static void bound_cb(tn_ctx* c, const void* vop, const double* x, double* y)
+{
+    heff_apply(c, *(const HeffOp*)vop, x, y, false);
+}
+
+int tn_eig_lowest(tn_ctx* c, int64_t N, int nev, double tol, int maxit, double* evals, double* evec,
+                  int* n_matvec)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_eig_lowest: no site bound");
+    TN_REQUIRE(N == (int64_t)c->b_d * c->b_D1 * c->b_D2 && evals && evec && nev >= 1 && nev <= 2 && maxit >= 1,
+               TN_ERR_ARG, "tn_eig_lowest: bad arguments");
+    HeffOp op = bound_op(c);
+    psi_to_dev(c, evec, c->b_x.as<double>());
+    LanczosResult r = lanczos(c, bound_cb, &op, op.n(), c->b_x.as<double>(), tol, maxit, 4);
+    sync(c);
+    psi_from_dev(c, c->b_x.as<double>(), evec);
+    evals[0] = r.e0;
+    if (nev > 1) evals[1] = r.e1;
+    if (n_matvec) *n_matvec = r.n_matvec;
+    return r.converged ? TN_OK : TN_WARN_NOT_CONVERGED;
+    ABI_END(c)
+}
+
+int tn_svd(tn_ctx* c, int m, int n, const double* A, double* U, double* s, double* Vt)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(m > 0 && n > 0 && A && U && s && Vt, TN_ERR_ARG, "tn_svd: bad arguments");
+    const int k = std::min(m, n);
+    c->svd_A.ensure((size_t)m * n * 8);
+    h2d(c, c->svd_A.p, A, (size_t)m * n * 8);
+    // A column-major: A(i,j) = A[i + j*m]
+    tn::DBuf Ub, Vb;
+    Ub.ensure((size_t)m * k * 8);
+    Vb.ensure((size_t)k * n * 8);
+    std::vector<double> sv;
+    svd_device(c, c->svd_A.as<double>(), 1, m, m, n, Ub.as<double>(), k, m, Vb.as<double>(), n, k, sv);
+    std::vector<double> hu((size_t)m * k), hv((size_t)k * n);
+    d2h(c, hu.data(), Ub.p, hu.size() * 8);
+    d2h(c, hv.data(), Vb.p, hv.size() * 8);
+    Ub.release();
+    Vb.release();
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < k; j++) U[i + (size_t)j * m] = hu[(size_t)i * k + j];
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < n; j++) Vt[i + (size_t)j * k] = hv[(size_t)i * n + j];
+    for (int j = 0; j < k; j++) s[j] = sv[j];
+    return TN_OK;
+    ABI_END(c)
+}
+
+// ---- chain ------------------------------------------------------------------
+int tn_chain_create(tn_ctx* c, int L, int d, const int* dw1, const int* dw2, const double* wcat, int D,
+                    int Dcap, tn_chain** out)
+{
+    if (!c || !out) return TN_ERR_ARG;
+    *out = nullptr;
+    tn_chain* ch = new tn_chain();
+    ABI_BEGIN
+    TN_REQUIRE(L >= 1 && d >= 2 && D >= 1 && dw1 && dw2 && wcat, TN_ERR_ARG, "tn_chain_create: bad arguments");
+    if (Dcap < D) Dcap = D;
+    ch->ctx = c;
+    ch->L = L;
+    ch->d = d;
+    ch->Dcap = Dcap;
+    ch->Dcapp = pad32(Dcap);
+    ch->dw1.assign(dw1, dw1 + L);
+    ch->dw2.assign(dw2, dw2 + L);
+    size_t off = 0;
+    ch->W.resize(L);
+    ch->Dwmax = 1;
+    for (int i = 0; i < L; i++) {
+        TN_REQUIRE(dw1[i] >= 1 && dw2[i] >= 1, TN_ERR_ARG, "bad MPO dims");
+        if (i > 0) TN_REQUIRE(dw1[i] == dw2[i - 1], TN_ERR_ARG, "MPO bond dims do not match");
+        size_t n = (size_t)dw1[i] * dw2[i] * d * d;
+        ch->W[i].assign(wcat + off, wcat + off + n);
+        off += n;
+        ch->Dwmax = std::max(ch->Dwmax, std::max(dw1[i], dw2[i]));
+    }
+    TN_REQUIRE(dw1[0] == 1 && dw2[L - 1] == 1, TN_ERR_ARG, "edge MPO blocks must be 1 x Dw and Dw x 1");
+    ch->site_cap = (size_t)ch->Dcapp * d * ch->Dcapp;
+    ch->env_cap = (size_t)ch->Dcapp * ch->Dwmax * ch->Dcapp;
+    ch->sites_buf.ensure(ch->site_cap * 8 * L);
+    ch->spare_buf.ensure(ch->site_cap * 8);
+    ch->env_buf.ensure(ch->env_cap * 8 * 2 * L);
+    ch->bnd_buf.ensure((size_t)2 * 32 * 32 * ch->Dwmax * 8 + 512);
+    ch->A.resize(L);
+    ch->Lenv.resize(L);
+    ch->Renv.resize(L);
+    ch->Llo.assign(L, nullptr);
+    ch->Rlo.assign(L, nullptr);
+    for (int i = 0; i < L; i++) {
+        ch->A[i] = ch->sites_buf.as<double>() + (size_t)i * ch->site_cap;
+        ch->Lenv[i] = ch->env_buf.as<double>() + (size_t)(2 * i) * ch->env_cap;
+        ch->Renv[i] = ch->env_buf.as<double>() + (size_t)(2 * i + 1) * ch->env_cap;
+    }
+    ch->spare = ch->spare_buf.as<double>();
+    TN_HIP(fill_zero(ch->sites_buf.p, (long)ch->sites_buf.bytes, c->stream));
+    TN_HIP(fill_zero(ch->env_buf.p, (long)ch->env_buf.bytes, c->stream));
+    TN_HIP(fill_zero(ch->bnd_buf.p, (long)ch->bnd_buf.bytes, c->stream));
+    ch->bL = ch->bnd_buf.as<double>();
+    ch->bR = ch->bL + 32 * 32 * ch->Dwmax + 32;
+    const double one = 1.0;
+    h2d(c, ch->bL, &one, 8);
+    h2d(c, ch->bR, &one, 8);
+    ch->mixH.resize(L);
+    ch->mixL.resize(L);
+    ch->mixR.resize(L);
+    ch->mix2H.resize(L);
+    ch->W12.resize(L);
+    // MPSImp.C:69-90 shapes; Bond.C:11-25 lambda = 1/D
+    ch->D1.resize(L);
+    ch->D2.resize(L);
+    for (int i = 0; i < L; i++) {
+        ch->D1[i] = (i == 0) ? 1 : D;
+        ch->D2[i] = (i == L - 1) ? 1 : D;
+    }
+    ch->lam.assign(std::max(L - 1, 1), std::vector<double>(D, 1.0 / D));
+    ch->maxDelta.assign(std::max(L - 1, 1), 0.0);
+    ch->compErr.assign(std::max(L - 1, 1), -99.0);
+    ch->Emin.assign(L, 0.0);
+    ch->IterDE.assign(L, 1.0);
+    ch->GapE.assign(L, 0.0);
+    sync(c);
+    *out = ch;
+    return TN_OK;
+    }
+    catch (const TnError& e) {
+        int code = fail(c, e);
+        delete ch;
+        return code;
+    }
+}
+
+int tn_chain_destroy(tn_chain* ch)
+{
+    if (!ch) return TN_OK;
+    if (ch->ctx && ch->ctx->stream) (void)hipStreamSynchronize(ch->ctx->stream);
+    ch->sites_buf.release();
+    ch->spare_buf.release();
+    ch->env_buf.release();
+    ch->lo_buf.release();
+    ch->bnd_buf.release();
+    ch->theta.release();
+    ch->w12buf.release();
+    for (auto* v : {&ch->mixH, &ch->mixL, &ch->mixR, &ch->mix2H})
+        for (auto& m : *v) m.buf.release();
+    delete ch;
+    return TN_OK;
+}
+
+int tn_chain_initialize(tn_chain* ch, int state, uint64_t seed)
+{
+    if (!ch) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    // MPSSite.C:84-140 per site, sign alternating
+    uint64_t st = seed;
+    int sgn = 1;
+    const int d = ch->d, L = ch->L;
+    for (int i = 0; i < L; i++) {
+        const int D1 = ch->D1[i], D2 = ch->D2[i];
+        std::vector<double> M((size_t)d * D1 * D2, 0.0);
+        if (state == 1) {
+            const double sc = 1.0 / std::sqrt((double)d * D1 * D2);
+            for (int m = 0; m < d; m++)
+                for (int i2 = 0; i2 < D2; i2++)
+                    for (int i1 = 0; i1 < D1; i1++) {
+                        st += 0x9e3779b97f4a7c15ULL;
+                        uint64_t z = st;
+                        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+                        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+                        z ^= z >> 31;
+                        M[psi_ref(d, D1, m, i1, i2)] = ((double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * sc;
+                    }
+        } else if (state == 2) {
+            if (sgn == 1) M[psi_ref(d, D1, 0, 0, 0)] = 1.0;
+            else M[psi_ref(d, D1, d - 1, 0, 0)] = 1.0;
+        } else if (state == 0) {
+            if (i == 0 && L > 1) {
+                for (int m = 0; m < d && m < D2; m++) M[psi_ref(d, D1, m, 0, m)] = sgn;
+            } else if (i == L - 1 && L > 1) {
+                for (int m = 0; m < d && m < D1; m++) M[psi_ref(d, D1, m, m, 0)] = sgn;
+            } else {
+                for (int m = 0; m < d; m++)
+                    for (int k = 0; k < std::min(D1, D2); k++) M[psi_ref(d, D1, m, k, k)] = sgn / std::sqrt((double)d);
+            }
+        } else {
+            TN_REQUIRE(false, TN_ERR_ARG, "unknown initial state");
+        }
+        upload_site(ch, i, D1, D2, M.data());
+        sgn = -sgn;
+    }
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_get_dims(const tn_chain* ch, int* D1, int* D2)
+{
+    if (!ch || !D1 || !D2) return TN_ERR_ARG;
+    for (int i = 0; i < ch->L; i++) {
+        D1[i] = ch->D1[i];
+        D2[i] = ch->D2[i];
+    }
+    return TN_OK;
+}
+
+int tn_chain_get_site(tn_chain* ch, int site, double* M)
+{
+    if (!ch || !M || site < 0 || site >= ch->L) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    std::vector<double> v;
+    download_site(ch, site, v);
+    std::memcpy(M, v.data(), v.size() * 8);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_set_site(tn_chain* ch, int site, int D1, int D2, const double* M)
+{
+    if (!ch || !M || site < 0 || site >= ch->L || D1 < 1 || D2 < 1) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    upload_site(ch, site, D1, D2, M);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_normalize(tn_chain* ch, int dir)
+{
+    if (!ch || (dir != 0 && dir != 1)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    if (dir == 0)
+        for (int i = 0; i < ch->L; i++) svd_normalize(ch, i, 0, nullptr);
+    else
+        for (int i = ch->L - 1; i >= 0; i--) svd_normalize(ch, i, 1, nullptr);
+    sync(c);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_load_caches(tn_chain* ch, const tn_sweep_opts* opts)
+{
+    if (!ch) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    tn_sweep_opts o;
+    tn_sweep_opts_default(&o);
+    if (opts) o = *opts;
+    for (int i = 0; i < ch->L - 1; i++) env_update(ch, i, 0, o, nullptr);
+    for (int i = ch->L - 1; i >= 1; i--) env_update(ch, i, 1, o, nullptr);
+    sync(c);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_refine_site(tn_chain* ch, int site, int dir, const tn_sweep_opts* opts, tn_sweep_stats* st)
+{
+    if (!ch || site < 0 || site >= ch->L || (dir != 0 && dir != 1)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    tn_sweep_opts o;
+    tn_sweep_opts_default(&o);
+    if (opts) o = *opts;
+    refine_site(ch, site, dir, o, st);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_sweep(tn_chain* ch, int dir, const tn_sweep_opts* opts, tn_sweep_stats* st)
+{
+    if (!ch || (dir != 0 && dir != 1)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    tn_sweep_opts o;
+    tn_sweep_opts_default(&o);
+    if (opts) o = *opts;
+    double t0 = now_ms();
+    if (dir == 0)
+        for (int i = 0; i < ch->L; i++) refine_site(ch, i, 0, o, st);
+    else
+        for (int i = ch->L - 1; i >= 0; i--) refine_site(ch, i, 1, o, st);
+    sync(c);
+    if (st) st->wall_ms += now_ms() - t0;
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_increase_bond_dims(tn_chain* ch, int D)
+{
+    if (!ch || D < 1) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    TN_REQUIRE(D <= ch->Dcap, TN_ERR_ARG, "D exceeds Dcap");
+    // MPSImp.C:123-147
+    const int L = ch->L, d = ch->d;
+    long Dideal = 1;
+    int ia = 0;
+    auto grow_bond = [&](int b, int nD) {  // Bond.C:60-84
+        std::vector<double>& l = ch->lam[b];
+        const int oD = (int)l.size();
+        std::vector<double> nl(nD);
+        if (nD > oD) {
+            double mn = l.empty() ? 0.0 : *std::min_element(l.begin(), l.end());
+            for (int k = 0; k < oD; k++) nl[k] = l[k];
+            for (int k = oD; k < nD; k++) nl[k] = mn / 10.0;
+        } else {
+            for (int k = 0; k < nD; k++) nl[k] = l[k];
+        }
+        double nr = 0;
+        for (double v : nl) nr += v * v;
+        nr = std::sqrt(nr);
+        for (double& v : nl) v = nr > 0 ? v / nr : v;
+        l = nl;
+    };
+    for (; ia < L / 2; ia++) {
+        int D1 = (int)std::min<long>(Dideal, D), D2 = (int)std::min<long>(Dideal * d, D);
+        site_resize(ch, ia, D1, D2, true);
+        grow_bond(ia, D2);
+        Dideal *= d;
+    }
+    Dideal = 1;
+    for (int ib = L - 1; ib >= ia; ib--) {
+        int D2 = (int)std::min<long>(Dideal, D), D1 = (int)std::min<long>(Dideal * d, D);
+        site_resize(ch, ib, D1, D2, true);
+        if (ib >= 1) grow_bond(ib - 1, D1);
+        Dideal *= d;
+    }
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_deltas(const tn_chain* ch, double* max_de, double* max_dl)
+{
+    if (!ch) return TN_ERR_ARG;
+    double de = 0, dl = 0;
+    for (int i = 0; i < ch->L; i++) de = std::max(de, std::fabs(ch->IterDE[i]));
+    for (int b = 0; b < ch->L - 1; b++) dl = std::max(dl, ch->maxDelta[b]);
+    if (max_de) *max_de = de;
+    if (max_dl) *max_dl = dl;
+    return TN_OK;
+}
+
+int tn_chain_bond_sv(const tn_chain* ch, int b, double* sv, int cap)
+{
+    if (!ch || b < 0 || b >= ch->L - 1) return TN_ERR_ARG;
+    const std::vector<double>& l = ch->lam[b];
+    int n = (int)l.size();
+    for (int k = 0; k < n && k < cap; k++) sv[k] = l[k];
+    return n;
+}
+
+int tn_chain_expectation(tn_chain* ch, const int* dw1, const int* dw2, const double* wcat, double* value)
+{
+    if (!ch || !dw1 || !dw2 || !wcat || !value) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    const int L = ch->L, d = ch->d;
+    TN_REQUIRE(dw1[0] == 1 && dw2[L - 1] == 1, TN_ERR_ARG, "edge MPO blocks must be 1 x Dw and Dw x 1");
+    int Dwm = 1;
+    for (int i = 0; i < L; i++) Dwm = std::max(Dwm, std::max(dw1[i], dw2[i]));
+    const size_t cap = (size_t)ch->Dcapp * Dwm * ch->Dcapp;
+    tn::DBuf F0, F1;
+    F0.ensure(std::max(cap, (size_t)32 * 32) * 8);
+    F1.ensure(std::max(cap, (size_t)32 * 32) * 8);
+    TN_HIP(fill_zero(F0.p, (long)F0.bytes, c->stream));
+    const double one = 1.0;
+    h2d(c, F0.p, &one, 8);
+    double* Fin = F0.as<double>();
+    double* Fout = F1.as<double>();
+    size_t off = 0;
+    DevMix mix;
+    for (int i = 0; i < L; i++) {
+        std::vector<double> Wi(wcat + off, wcat + off + (size_t)dw1[i] * dw2[i] * d * d);
+        off += Wi.size();
+        const int D1p = pad32(ch->D1[i]), D2p = pad32(ch->D2[i]);
+        mix_build(mix, MIX_ENVL, Wi.data(), dw1[i], dw2[i], d, D1p, D2p, c->stream);
+        env_left(c, TN_FP64, d, D1p, D2p, dw1[i], dw2[i], mix, Fin, nullptr, ch->A[i], Fout, 0);
+        sync(c);
+        mix.wsrc = nullptr;  // W buffer is local: never reuse by address
+        std::swap(Fin, Fout);
+    }
+    d2h(c, value, Fin, 8);
+    mix.buf.release();
+    F0.release();
+    F1.release();
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_sweep_opts* opts,
+                    tn_sweep_stats* st, double* max_trunc)
+{
+    if (!ch || (dir != 0 && dir != 1) || ch->L < 2) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    tn_sweep_opts o;
+    tn_sweep_opts_default(&o);
+    if (opts) o = *opts;
+    TN_REQUIRE(Dmax <= ch->Dcap, TN_ERR_ARG, "Dmax exceeds Dcap");
+    hipStream_t s = c->stream;
+    const int d = ch->d, L = ch->L, p2 = d * d;
+    double mtr = 0.0;
+    double t0w = now_ms();
+    if (o.precision != TN_FP64) ensure_lo(ch, o.precision);
+    for (int step = 0; step < L - 1; step++) {
+        const int i = (dir == 0) ? step : L - 2 - step, j = i + 1;
+        const int D1 = ch->D1[i], Dm = ch->D2[i], D3 = ch->D2[j];
+        const int D1p = pad32(D1), Dmp = pad32(Dm), D3p = pad32(D3);
+        const int Dw1 = ch->dw1[i], Dwm = ch->dw2[i], Dw3 = ch->dw2[j];
+        // Theta[a][m1][m2][c] = A_i[(a,m1)][b] A_j[b][(m2,c)]
+        const long N = (long)D1p * p2 * D3p;
+        ch->theta.ensure((size_t)N * 8);
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+        g.M = D1p * d; g.N = d * D3p; g.K = Dmp; g.batch = 1;
+        g.A = ch->A[i]; g.lda = Dmp; g.B = ch->A[j]; g.ldb = (long)d * D3p;
+        g.C = ch->theta.p; g.ldc = (long)d * D3p; g.alpha = 1.0f;
+        TN_HIP(gemm(g, s));
+        two_site_W(d, Dw1, Dwm, Dw3, ch->W[i].data(), ch->W[j].data(), ch->W12[i]);
+        HeffCall hc;
+        hc.op.p = p2;
+        hc.op.D1p = D1p;
+        hc.op.D2p = D3p;
+        hc.op.Dw1 = Dw1;
+        hc.op.Dw2 = Dw3;
+        hc.op.L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
+        hc.op.R = j < L - 1 ? ch->Renv[j + 1] : ch->bR;
+        hc.op.precision = o.precision;
+        hc.op.Llo = o.precision == TN_FP64 ? nullptr : (i > 0 ? ch->Llo[i - 1] : ch->bLlo);
+        hc.op.Rlo = o.precision == TN_FP64 ? nullptr : (j < L - 1 ? ch->Rlo[j + 1] : ch->bRlo);
+        ch->mix2H[i].wsrc = nullptr;  // W12 rebuilt each visit
+        mix_build(ch->mix2H[i], MIX_HEFF, ch->W12[i].data(), Dw1, Dw3, p2, D1p, D3p, s);
+        hc.op.mix = &ch->mix2H[i];
+        hc.op.tile = o.gemm_tile;
+        hc.st = st;
+        hc.D1 = D1;
+        hc.D2 = D3;
+        double tl = now_ms();
+        LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->theta.as<double>(), o.eig_tol, o.max_krylov, o.max_restarts);
+        if (o.precision != TN_FP64 && o.refine) {
+            HeffCall h64 = hc;
+            h64.op.precision = TN_FP64;
+            LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->theta.as<double>(), o.refine_tol, o.max_krylov, o.max_restarts);
+            r.e0 = r2.e0;
+            r.e1 = r2.e1;
+        }
+        if (st) {
+            st->lanczos_ms += now_ms() - tl;
+            st->energy = r.e0;
+        }
+        ch->IterDE[i] = r.e0 - ch->Emin[i];
+        ch->Emin[i] = r.e0;
+        // SVD of Theta as (D1*d) x (d*D3): pack logical matrix
+        double ts = now_ms();
+        const int rr = D1 * d, cc = d * D3, k = std::min(rr, cc);
+        c->svd_A.ensure((size_t)rr * cc * 8 + 64);
+        TN_HIP(copy3(ch->theta.as<double>(), (long)d * D3p, D3p, 1, c->svd_A.as<double>(), rr, d, D3, D3, s));
+        const int kp = pad32(k);
+        c->svd_U.ensure((size_t)rr * kp * 8 + 64);
+        c->svd_Vt.ensure((size_t)kp * cc * 8 + 64);
+        std::vector<double> sv;
+        svd_device(c, c->svd_A.as<double>(), cc, 1, rr, cc, c->svd_U.as<double>(), kp, rr,
+                   c->svd_Vt.as<double>(), cc, kp, sv);
+        double disc = 0, resc = 1;
+        int Dk = tn_svcompress_rank(k, sv.data(), Dmax, eps_sv, &disc, &resc);
+        if (Dk < 1) Dk = 1;
+        std::vector<double> sk(sv.begin(), sv.begin() + Dk);
+        if (Dk < k)
+            for (double& v : sk) v *= resc;
+        mtr = std::max(mtr, disc);
+        bond_set_sv(ch, i, sk, disc);
+        double nr = 0;
+        for (double v : sk) nr += v * v;
+        nr = std::sqrt(nr);
+        const int Dkp = pad32(Dk);
+        std::vector<double> scl(std::max(Dkp, 32), 0.0);
+        for (int t = 0; t < Dk; t++) scl[t] = sk[t] / nr;
+        c->pack.ensure((size_t)scl.size() * 8 + 64);
+        TN_HIP(hipMemcpyAsync(c->pack.p, scl.data(), scl.size() * 8, hipMemcpyHostToDevice, s));
+        if (dir == 0) {
+            // A_i <- U[:, :Dk] ; A_j <- diag(s) Vt[:Dk]
+            TN_HIP(copy2(c->svd_U.as<double>(), kp, 1, rr, Dk, ch->A[i], Dkp, 1, D1p * d, Dkp, s));
+            TN_HIP(scale_rows(c->svd_Vt.as<double>(), cc, Dk, cc, c->pack.as<double>(), s));
+            TN_HIP(copy3(c->svd_Vt.as<double>(), cc, D3, 1, ch->A[j], Dk, d, D3, D3p, s));
+            if (Dkp > Dk) TN_HIP(fill_zero(ch->A[j] + (size_t)Dk * d * D3p, (long)(Dkp - Dk) * d * D3p * 8, s));
+        } else {
+            // A_j <- Vt[:Dk] ; A_i <- U diag(s)
+            TN_HIP(copy3(c->svd_Vt.as<double>(), cc, D3, 1, ch->A[j], Dk, d, D3, D3p, s));
+            if (Dkp > Dk) TN_HIP(fill_zero(ch->A[j] + (size_t)Dk * d * D3p, (long)(Dkp - Dk) * d * D3p * 8, s));
+            TN_HIP(scale_cols(c->svd_U.as<double>(), kp, rr, Dk, c->pack.as<double>(), s));
+            TN_HIP(copy2(c->svd_U.as<double>(), kp, 1, rr, Dk, ch->A[i], Dkp, 1, D1p * d, Dkp, s));
+        }
+        ch->D2[i] = Dk;
+        ch->D1[j] = Dk;
+        if (st) {
+            sync(c);
+            st->svd_ms += now_ms() - ts;
+            st->n_svd++;
+        }
+        if (dir == 0) env_update(ch, i, 0, o, st);
+        else env_update(ch, j, 1, o, st);
+    }
+    sync(c);
+    if (st) st->wall_ms += now_ms() - t0w;
+    if (max_trunc) *max_trunc = mtr;
+    return TN_OK;
+    ABI_END(c)
+}
+
+// ---- diagnostics --------------------------------------------------------------
+int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, int K, int batch,
+                 const double* A, const double* B, double* C, int tile)
+{
+    if (!c || !A || !B || !C) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(M > 0 && N > 0 && K > 0 && batch > 0, TN_ERR_ARG, "tn_gemm_test: bad dims");
+    const long nA = (long)batch * M * K, nB = (long)batch * K * N, nC = (long)batch * M * N;
+    tn::DBuf a64, b64, alo, blo, cout, c64;
+    a64.ensure(nA * 8);
+    b64.ensure(nB * 8);
+    h2d(c, a64.p, A, nA * 8);
+    h2d(c, b64.p, B, nB * 8);
+    const void* Ad = a64.p;
+    const void* Bd = b64.p;
+    if (tin != T_F64) {
+        alo.ensure(nA * 4);
+        blo.ensure(nB * 4);
+        TN_HIP(convert(T_F64, a64.p, tin, alo.p, nA, c->stream));
+        TN_HIP(convert(T_F64, b64.p, tin, blo.p, nB, c->stream));
+        Ad = alo.p;
+        Bd = blo.p;
+    }
+    cout.ensure(nC * 8);
+    c64.ensure(nC * 8);
+    GemmDesc g{};
+    g.tin = tin; g.tout = tout; g.a_kmajor = akm != 0; g.b_kmajor = bkm != 0;
+    g.M = M; g.N = N; g.K = K; g.batch = batch;
+    g.A = Ad; g.lda = akm ? K : M; g.sA = (long)M * K;
+    g.B = Bd; g.ldb = bkm ? K : N; g.sB = (long)K * N;
+    g.C = cout.p; g.ldc = N; g.sC = (long)M * N;
+    g.alpha = 1.0f; g.tile = tile;
+    TN_HIP(gemm(g, c->stream));
+    TN_HIP(convert(tout, cout.p, T_F64, c64.p, nC, c->stream));
+    d2h(c, C, c64.p, nC * 8);
+    for (auto* b : {&a64, &b64, &alo, &blo, &cout, &c64}) b->release();
+    return TN_OK;
+    ABI_END(c)
+}
+
+}  // extern "C"
+

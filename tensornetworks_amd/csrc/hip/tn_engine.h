@@ -1,0 +1,139 @@
+// tn_engine.h -- device-side building blocks behind the C ABI (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../../include/tn_abi.h"
+#include "tn_kernels.h"
+
+namespace tn {
+
+inline int pad32(int n) { return n <= 0 ? 32 : ((n + 31) / 32) * 32; }
+
+struct TnError {
+    int code;
+    std::string msg;
+};
+
+#define TN_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            throw ::tn::TnError{TN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e) + \
+                                " at " + __FILE__ + ":" + std::to_string(__LINE__)};   \
+    } while (0)
+#define TN_REQUIRE(cond, code, msg)                                                    \
+    do {                                                                               \
+        if (!(cond)) throw ::tn::TnError{(code), std::string(msg)};                    \
+    } while (0)
+
+// grow-only device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+// a few device scalars + pinned host mirror for the per-iteration readback
+struct Scalars {
+    double* d = nullptr;  // device, 64 doubles
+    double* h = nullptr;  // pinned host, 64 doubles
+};
+
+// Sparse MPO mixing table on the device (see wmix in tn_kernels.h)
+struct DevMix {
+    DBuf buf;
+    MixTable t{};
+    // key
+    const double* wsrc = nullptr;
+    int op = -1, D1p = -1, D2p = -1, p = -1, dw1 = -1, dw2 = -1;
+};
+enum MixOp { MIX_HEFF = 0, MIX_ENVL = 1, MIX_ENVR = 2 };
+// W host array [dw1][dw2][p][p]; builds (or reuses) the table for the op/dims
+void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
+               hipStream_t s);
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace tn
+
+struct tn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // scratch
+    tn::DBuf T1, T2, xlo, part, krylov, lanczos_small, svd_X, svd_V, svd_U, svd_Vt, svd_s,
+        svd_perm, svd_pairs, svd_flag, svd_A, spare, pack;
+    tn::Scalars sc;
+    hipEvent_t ev[8] = {};
+    // seam state (tn_site_bind)
+    int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
+    std::vector<double> b_W;
+    tn::DBuf b_L, b_R, b_Llo, b_Rlo, b_x, b_y, b_M;
+    tn::DevMix b_mix_heff, b_mix_envl, b_mix_envr;
+    bool bound = false;
+};
+
+namespace tn {
+
+// --- effective Hamiltonian operator (internal padded layout) -------------
+// x, y: [D1p][p][D2p]   L: [D1p][Dw1][D1p]   R: [Dw2][D2p][D2p]
+struct HeffOp {
+    int p, D1p, D2p, Dw1, Dw2;
+    const double* L;
+    const double* R;
+    const void* Llo;  // low precision copies (nullptr when precision == FP64)
+    const void* Rlo;
+    int precision;
+    const DevMix* mix;  // MIX_HEFF table for (D1p, D2p)
+    int tile;
+    long n() const { return (long)D1p * p * D2p; }
+    double flops_alg(int D1, int D2) const;  // reference formula on logical dims
+};
+// y = Heff x (fp64 in/out).  gemm_ms/total_ms accumulate device times when
+// timing != nullptr (events recorded; read after the next sync).
+void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed);
+void heff_timing_read(tn_ctx* c, double* gemm_ms, double* total_ms);
+
+// --- environment updates ---------------------------------------------------
+// left : Lout[D2p][Dw2][D2p]  from Lin[D1p][Dw1][D1p] and A[D1p][p][D2p]
+// right: Rout[Dw1][D1p][D1p]  from Rin[Dw2][D2p][D2p] and A
+void env_left(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, const DevMix& mix,
+              const double* Lin, const void* Lin_lo, const double* A, double* Lout, int tile);
+void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, const DevMix& mix,
+               const double* Rin, const void* Rin_lo, const double* A, double* Rout, int tile);
+// store a low precision copy of an fp64 buffer
+void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n);
+
+// --- Lanczos ---------------------------------------------------------------
+struct LanczosResult {
+    double e0 = 0, e1 = 0, residual = 0;
+    int n_matvec = 0;
+    bool converged = false;
+};
+// generic operator callback: y = A x (device fp64)
+typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
+LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
+                      int max_krylov, int max_restarts);
+
+// --- SVD -----------------------------------------------------------------
+// Thin SVD of a device matrix view A(i,j) = A[i*rs + j*cs], r x c.
+// Writes U (row-major r x k, ldu), Vt (row-major k x c, ldvt), s (host, k).
+// U/Vt buffers must hold ru x ldu / ku x ldvt and are zero filled first over
+// [0, zr) x [0, ldu) and [0, zk) x [0, ldvt).
+int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int c_, double* U, long ldu,
+               int zr, double* Vt, long ldvt, int zk, std::vector<double>& s);
+
+void sync(tn_ctx* c);
+// stream-ordered copies that return when the data has arrived
+void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes);
+void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes);
+
+}  // namespace tn

@@ -1,0 +1,526 @@
+// tn_engine.hip -- host orchestration of the device kernels: the effective
+// Hamiltonian operator, environment updates, Lanczos and the bond SVD.
+#include "tn_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace tn {
+
+void DBuf::ensure(size_t n)
+{
+    if (n <= bytes && p) return;
+    // kernels queued on any stream may still read the old buffer
+    if (p) {
+        TN_HIP(hipDeviceSynchronize());
+        TN_HIP(hipFree(p));
+    }
+    p = nullptr;
+    bytes = 0;
+    size_t want = n < 256 ? 256 : n;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw TnError{TN_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(want) +
+                                        " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = want;
+}
+void DBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void sync(tn_ctx* c) { TN_HIP(hipStreamSynchronize(c->stream)); }
+void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    sync(c);
+}
+void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+}
+
+// ---------------------------------------------------------------------------
+// MPO mixing tables.  Orientation per op (W[w][w'][m][n], m bra, n ket):
+//  HEFF : T1[a'][n][w'][b] -> T2[w][a'][m][b]      (sum over w', n)
+//  ENVR : X [a'][n][w'][b] -> Y [a'][w][m][b]      (sum over w', n)
+//  ENVL : X [a][w][n][b']  -> Y [a][m][w'][b']     (sum over w , n)
+void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
+               hipStream_t s)
+{
+    if (m.wsrc == W && m.op == op && m.D1p == D1p && m.D2p == D2p && m.p == p && m.dw1 == dw1 &&
+        m.dw2 == dw2)
+        return;
+    std::vector<int> optr;
+    std::vector<long> ooff, eoff;
+    std::vector<double> ev;
+    auto Wv = [&](int w, int wp, int mm, int n) { return W[(((long)w * dw2 + wp) * p + mm) * p + n]; };
+    optr.push_back(0);
+    if (op == MIX_HEFF || op == MIX_ENVR) {
+        for (int w = 0; w < dw1; w++)
+            for (int mm = 0; mm < p; mm++) {
+                ooff.push_back(op == MIX_HEFF ? (long)w * D1p * p * D2p + (long)mm * D2p
+                                              : (long)w * p * D2p + (long)mm * D2p);
+                for (int wp = 0; wp < dw2; wp++)
+                    for (int n = 0; n < p; n++) {
+                        double v = Wv(w, wp, mm, n);
+                        if (v != 0.0) {
+                            eoff.push_back((long)n * dw2 * D2p + (long)wp * D2p);
+                            ev.push_back(v);
+                        }
+                    }
+                optr.push_back((int)ev.size());
+            }
+    } else {
+        for (int wp = 0; wp < dw2; wp++)
+            for (int mm = 0; mm < p; mm++) {
+                ooff.push_back((long)mm * dw2 * D2p + (long)wp * D2p);
+                for (int w = 0; w < dw1; w++)
+                    for (int n = 0; n < p; n++) {
+                        double v = Wv(w, wp, mm, n);
+                        if (v != 0.0) {
+                            eoff.push_back((long)w * p * D2p + (long)n * D2p);
+                            ev.push_back(v);
+                        }
+                    }
+                optr.push_back((int)ev.size());
+            }
+    }
+    const int nout = (int)ooff.size(), nnz = (int)ev.size();
+    std::vector<float> evf(ev.begin(), ev.end());
+    // layout: optr | ooff | eoff | eval | evalf  (each 256-byte aligned)
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o1 = 0, o2 = o1 + al((nout + 1) * sizeof(int)), o3 = o2 + al(nout * sizeof(long) + 8),
+           o4 = o3 + al(nnz * sizeof(long) + 8), o5 = o4 + al(nnz * sizeof(double) + 8),
+           tot = o5 + al(nnz * sizeof(float) + 8);
+    std::vector<char> host(tot, 0);
+    std::memcpy(host.data() + o1, optr.data(), (nout + 1) * sizeof(int));
+    if (nout) std::memcpy(host.data() + o2, ooff.data(), nout * sizeof(long));
+    if (nnz) {
+        std::memcpy(host.data() + o3, eoff.data(), nnz * sizeof(long));
+        std::memcpy(host.data() + o4, ev.data(), nnz * sizeof(double));
+        std::memcpy(host.data() + o5, evf.data(), nnz * sizeof(float));
+    }
+    m.buf.ensure(tot);
+    TN_HIP(hipMemcpyAsync(m.buf.p, host.data(), tot, hipMemcpyHostToDevice, s));
+    TN_HIP(hipStreamSynchronize(s));
+    char* b = (char*)m.buf.p;
+    m.t.nout = nout;
+    m.t.nnz = nnz;
+    m.t.optr = (const int*)(b + o1);
+    m.t.ooff = (const long*)(b + o2);
+    m.t.eoff = (const long*)(b + o3);
+    m.t.eval = (const double*)(b + o4);
+    m.t.evalf = (const float*)(b + o5);
+    m.wsrc = W;
+    m.op = op;
+    m.D1p = D1p;
+    m.D2p = D2p;
+    m.p = p;
+    m.dw1 = dw1;
+    m.dw2 = dw2;
+}
+
+// ---------------------------------------------------------------------------
+double HeffOp::flops_alg(int D1, int D2) const
+{
+    // 2 GEMM stages + W stage on logical dims: for D1 = D2 = D this is
+    // 4 d Dw D^3 + 2 d^2 Dw^2 D^2 (one-site) with d -> p.
+    double s1 = 2.0 * D1 * p * (double)Dw2 * D2 * D2;
+    double s2 = 2.0 * (double)Dw1 * Dw2 * p * p * D1 * D2;
+    double s3 = 2.0 * D1 * (double)p * D2 * Dw1 * D1;
+    return s1 + s2 + s3;
+}
+
+static inline int lo_type(int prec) { return prec == TN_FP32 ? T_F32 : T_BF16; }
+
+void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed)
+{
+    hipStream_t s = c->stream;
+    const int p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
+    const long n = op.n();
+    const long n1 = (long)D1p * p * Dw2 * D2p;
+    const long n2 = (long)Dw1 * D1p * p * D2p;
+    if (timed) TN_HIP(hipEventRecord(c->ev[0], s));
+    GemmDesc g1{}, g3{};
+    g1.a_kmajor = true;
+    g1.b_kmajor = true;
+    g1.M = D1p * p;
+    g1.N = Dw2 * D2p;
+    g1.K = D2p;
+    g1.batch = 1;
+    g1.lda = D2p;
+    g1.ldb = D2p;
+    g1.ldc = (long)Dw2 * D2p;
+    g1.alpha = 1.0f;
+    g1.tile = op.tile;
+    g3.a_kmajor = true;
+    g3.b_kmajor = false;
+    g3.M = D1p;
+    g3.N = p * D2p;
+    g3.K = Dw1 * D1p;
+    g3.batch = 1;
+    g3.lda = (long)Dw1 * D1p;
+    g3.ldb = (long)p * D2p;
+    g3.ldc = (long)p * D2p;
+    g3.alpha = 1.0f;
+    g3.tile = op.tile;
+    if (op.precision == TN_FP64) {
+        c->T1.ensure(n1 * 8);
+        c->T2.ensure(n2 * 8);
+        if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
+        g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R; g1.C = c->T1.p;
+        TN_HIP(gemm(g1, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
+        TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
+                    op.mix->t, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
+        g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L; g3.B = c->T2.p; g3.C = y;
+        TN_HIP(gemm(g3, s));
+    } else {
+        const int tl = lo_type(op.precision);
+        const size_t es = dtype_size(tl);
+        c->xlo.ensure(n * es);
+        c->T1.ensure(n1 * 4);
+        c->T2.ensure(n2 * es);
+        TN_HIP(convert(T_F64, x, tl, c->xlo.p, n, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
+        g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = op.Rlo; g1.C = c->T1.p;
+        TN_HIP(gemm(g1, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
+        TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
+                    op.mix->t, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
+        g3.tin = tl; g3.tout = T_F64; g3.A = op.Llo; g3.B = c->T2.p; g3.C = y;
+        TN_HIP(gemm(g3, s));
+    }
+    if (timed) TN_HIP(hipEventRecord(c->ev[4], s));
+}
+
+void heff_timing_read(tn_ctx* c, double* gemm_ms, double* total_ms)
+{
+    float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
+    TN_HIP(hipEventSynchronize(c->ev[4]));
+    TN_HIP(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]));
+    TN_HIP(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+    TN_HIP(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+    TN_HIP(hipEventElapsedTime(&t34, c->ev[3], c->ev[4]));
+    if (gemm_ms) *gemm_ms += (double)t12 + (double)t34;
+    if (total_ms) *total_ms += (double)t01 + t12 + t23 + t34;
+}
+
+void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n)
+{
+    if (prec == TN_FP64) return;
+    TN_HIP(convert(T_F64, src, lo_type(prec), dst, n, c->stream));
+}
+
+// ---------------------------------------------------------------------------
+void env_left(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, const DevMix& mix,
+              const double* Lin, const void* Lin_lo, const double* A, double* Lout, int tile)
+{
+    hipStream_t s = c->stream;
+    const long nA = (long)D1p * p * D2p;
+    const long nX = (long)D1p * Dw1 * p * D2p;
+    const long nY = (long)D1p * p * Dw2 * D2p;
+    GemmDesc e1{}, e3{};
+    e1.a_kmajor = true; e1.b_kmajor = false;
+    e1.M = D1p * Dw1; e1.N = p * D2p; e1.K = D1p; e1.batch = 1;
+    e1.lda = D1p; e1.ldb = (long)p * D2p; e1.ldc = (long)p * D2p; e1.alpha = 1.0f; e1.tile = tile;
+    e3.a_kmajor = false; e3.b_kmajor = false;
+    e3.M = D2p; e3.N = Dw2 * D2p; e3.K = D1p * p; e3.batch = 1;
+    e3.lda = D2p; e3.ldb = (long)Dw2 * D2p; e3.ldc = (long)Dw2 * D2p; e3.alpha = 1.0f; e3.tile = tile;
+    if (prec == TN_FP64) {
+        c->T1.ensure(nX * 8);
+        c->T2.ensure(nY * 8);
+        e1.tin = T_F64; e1.tout = T_F64; e1.A = Lin; e1.B = A; e1.C = c->T1.p;
+        TN_HIP(gemm(e1, s));
+        TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)Dw1 * p * D2p, c->T2.p, (long)p * Dw2 * D2p, D1p,
+                    D2p, mix.t, s));
+        e3.tin = T_F64; e3.tout = T_F64; e3.A = A; e3.B = c->T2.p; e3.C = Lout;
+        TN_HIP(gemm(e3, s));
+    } else {
+        const int tl = lo_type(prec);
+        const size_t es = dtype_size(tl);
+        c->xlo.ensure(nA * es);
+        c->T1.ensure(nX * 4);
+        c->T2.ensure(nY * es);
+        TN_HIP(convert(T_F64, A, tl, c->xlo.p, nA, s));
+        e1.tin = tl; e1.tout = T_F32; e1.A = Lin_lo; e1.B = c->xlo.p; e1.C = c->T1.p;
+        TN_HIP(gemm(e1, s));
+        TN_HIP(wmix(T_F32, tl, c->T1.p, (long)Dw1 * p * D2p, c->T2.p, (long)p * Dw2 * D2p, D1p,
+                    D2p, mix.t, s));
+        e3.tin = tl; e3.tout = T_F64; e3.A = c->xlo.p; e3.B = c->T2.p; e3.C = Lout;
+        TN_HIP(gemm(e3, s));
+    }
+}
+
+void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, const DevMix& mix,
+               const double* Rin, const void* Rin_lo, const double* A, double* Rout, int tile)
+{
+    hipStream_t s = c->stream;
+    const long nA = (long)D1p * p * D2p;
+    const long nX = (long)D1p * p * Dw2 * D2p;
+    const long nY = (long)D1p * Dw1 * p * D2p;
+    GemmDesc f1{}, f3{};
+    f1.a_kmajor = true; f1.b_kmajor = true;
+    f1.M = D1p * p; f1.N = Dw2 * D2p; f1.K = D2p; f1.batch = 1;
+    f1.lda = D2p; f1.ldb = D2p; f1.ldc = (long)Dw2 * D2p; f1.alpha = 1.0f; f1.tile = tile;
+    f3.a_kmajor = true; f3.b_kmajor = true;
+    f3.M = D1p; f3.N = D1p; f3.K = p * D2p; f3.batch = Dw1;
+    f3.lda = (long)p * D2p; f3.sA = 0;
+    f3.ldb = (long)Dw1 * p * D2p; f3.sB = (long)p * D2p;
+    f3.ldc = D1p; f3.sC = (long)D1p * D1p;
+    f3.alpha = 1.0f; f3.tile = tile;
+    if (prec == TN_FP64) {
+        c->T1.ensure(nX * 8);
+        c->T2.ensure(nY * 8);
+        f1.tin = T_F64; f1.tout = T_F64; f1.A = A; f1.B = Rin; f1.C = c->T1.p;
+        TN_HIP(gemm(f1, s));
+        TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)Dw1 * p * D2p, D1p,
+                    D2p, mix.t, s));
+        f3.tin = T_F64; f3.tout = T_F64; f3.A = A; f3.B = c->T2.p; f3.C = Rout;
+        TN_HIP(gemm(f3, s));
+    } else {
+        const int tl = lo_type(prec);
+        const size_t es = dtype_size(tl);
+        c->xlo.ensure(nA * es);
+        c->T1.ensure(nX * 4);
+        c->T2.ensure(nY * es);
+        TN_HIP(convert(T_F64, A, tl, c->xlo.p, nA, s));
+        f1.tin = tl; f1.tout = T_F32; f1.A = c->xlo.p; f1.B = Rin_lo; f1.C = c->T1.p;
+        TN_HIP(gemm(f1, s));
+        TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)Dw1 * p * D2p, D1p,
+                    D2p, mix.t, s));
+        f3.tin = tl; f3.tout = T_F64; f3.A = c->xlo.p; f3.B = c->T2.p; f3.C = Rout;
+        TN_HIP(gemm(f3, s));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos with full (two-pass classical Gram-Schmidt) re-orthogonalisation.
+// Krylov vectors, tridiagonal solve, Ritz vector and residual stay in FP64 on
+// the device; the host reads 4 scalars per iteration for loop control.
+LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
+                      int max_krylov, int max_restarts)
+{
+    hipStream_t s = c->stream;
+    LanczosResult res;
+    int kmax = std::max(2, std::min(max_krylov, 500));
+    c->krylov.ensure((size_t)(kmax + 1) * N * 8);
+    double* V = c->krylov.as<double>();
+    c->lanczos_small.ensure(sizeof(double) * (4 * 512 + 64));
+    double* alpha = c->lanczos_small.as<double>();
+    double* beta = alpha + 512;
+    double* h = beta + 512;
+    double* z = h + 512;
+    double* out = z + 512;
+    long pcap = mdot_part_size(N, kmax + 1);
+    c->part.ensure(pcap * 8);
+    double* part = c->part.as<double>();
+    double* hs = c->sc.h;
+
+    auto norm_to = [&](const double* v, double* dst) {
+        TN_HIP(mdot(v, N, 1, v, N, dst, part, (int)pcap, s));
+        TN_HIP(dsqrt_scalar(dst, dst, s));
+    };
+
+    for (int restart = 0; restart <= max_restarts; restart++) {
+        // v0 = x / |x|
+        norm_to(x, out + 8);
+        TN_HIP(hipMemcpyAsync(hs, out + 8, 8, hipMemcpyDeviceToHost, s));
+        sync(c);
+        if (!(hs[0] > 0.0) || !std::isfinite(hs[0])) {
+            // zero or non-finite start: seeded pseudo-random start vector
+            std::vector<double> r(N);
+            uint64_t st = 0x5eedULL + restart;
+            for (long i = 0; i < N; i++) {
+                st += 0x9e3779b97f4a7c15ULL;
+                uint64_t zz = st;
+                zz = (zz ^ (zz >> 30)) * 0xbf58476d1ce4e5b9ULL;
+                zz = (zz ^ (zz >> 27)) * 0x94d049bb133111ebULL;
+                zz ^= zz >> 31;
+                r[i] = (double)(zz >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+            }
+            h2d(c, x, r.data(), N * 8);
+            norm_to(x, out + 8);
+        }
+        TN_HIP(hipMemcpyAsync(V, x, N * 8, hipMemcpyDeviceToDevice, s));
+        TN_HIP(scale_by_inv(V, out + 8, N, s));
+        int k = 0;
+        bool conv = false;
+        for (int j = 0; j < kmax; j++) {
+            double* vj = V + (long)j * N;
+            double* w = V + (long)(j + 1) * N;
+            fn(c, op, vj, w);
+            res.n_matvec++;
+            TN_HIP(mdot(V, N, j + 1, w, N, h, part, (int)pcap, s));
+            TN_HIP(dcopy_scalar(h + j, alpha + j, s));
+            TN_HIP(gemv_acc(V, N, j + 1, h, -1.0, 1.0, w, N, s));
+            TN_HIP(mdot(V, N, j + 1, w, N, h, part, (int)pcap, s));
+            TN_HIP(gemv_acc(V, N, j + 1, h, -1.0, 1.0, w, N, s));
+            norm_to(w, beta + j);
+            TN_HIP(tridiag_ritz(alpha, beta, j + 1, z, out, s));
+            TN_HIP(hipMemcpyAsync(hs, out, 4 * 8, hipMemcpyDeviceToHost, s));
+            sync(c);
+            k = j + 1;
+            res.e0 = hs[0];
+            res.e1 = hs[1];
+            res.residual = hs[2];
+            const double b = hs[3];
+            if (!std::isfinite(res.e0)) throw TnError{TN_ERR_STATE, "Lanczos: non-finite Ritz value"};
+            if (res.residual < tol || b < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
+                conv = true;
+                break;
+            }
+            if (j + 1 < kmax) TN_HIP(scale_by_inv(w, beta + j, N, s));
+        }
+        // Ritz vector x = sum_i z_i v_i, normalised
+        TN_HIP(gemv_acc(V, N, k, z, 1.0, 0.0, x, N, s));
+        norm_to(x, out + 8);
+        TN_HIP(scale_by_inv(x, out + 8, N, s));
+        if (conv) {
+            res.converged = true;
+            break;
+        }
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Thin SVD by one-sided block Jacobi on the q <= p columns of X (p x q):
+// X = A (r >= c) or A^T (r < c).  X J = Q S, so A = Q S J^T or J S Q^T.
+int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, double* U, long ldu,
+               int zr, double* Vt, long ldvt, int zk, std::vector<double>& s_out)
+{
+    hipStream_t s = c->stream;
+    const int k = std::min(r, cc);
+    const bool tall = r >= cc;
+    const int p = tall ? r : cc, q = tall ? cc : r;
+    const int qp = pad32(q);
+    const long ldx = p;
+    c->svd_X.ensure((size_t)ldx * qp * 8);
+    c->svd_V.ensure((size_t)qp * qp * 8);
+    double* X = c->svd_X.as<double>();
+    double* V = c->svd_V.as<double>();
+    // X[j*ldx + i] = tall ? A(i,j) : A(j,i)
+    if (tall) TN_HIP(copy2(A, cs, rs, q, p, X, ldx, 1, qp, p, s));
+    else TN_HIP(copy2(A, rs, cs, q, p, X, ldx, 1, qp, p, s));
+    TN_HIP(set_identity(V, qp, qp, s));
+    // tournament pairs for nb = qp/16 column blocks
+    const int nb = qp / 16;
+    std::vector<int> pairs((size_t)(nb - 1) * nb);
+    for (int rd = 0; rd < nb - 1; rd++)
+        for (int i = 0; i < nb / 2; i++) {
+            auto elem = [&](int kp) { return kp == 0 ? 0 : ((kp - 1 + rd) % (nb - 1)) + 1; };
+            pairs[(size_t)rd * nb + 2 * i] = elem(i);
+            pairs[(size_t)rd * nb + 2 * i + 1] = elem(nb - 1 - i);
+        }
+    c->svd_pairs.ensure(pairs.size() * sizeof(int) + 64);
+    TN_HIP(hipMemcpyAsync(c->svd_pairs.p, pairs.data(), pairs.size() * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    c->svd_flag.ensure(256);
+    int* flag = c->svd_flag.as<int>();
+    const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)p));
+    int hflag = 1;
+    for (int sweep = 0; sweep < 40 && hflag; sweep++) {
+        TN_HIP(set_int(flag, 0, s));
+        for (int rd = 0; rd < nb - 1; rd++)
+            TN_HIP(jacobi_step(X, ldx, p, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb,
+                               nb / 2, tol, flag, s));
+        TN_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        sync(c);
+    }
+    // singular values = column norms; sort descending
+    c->svd_s.ensure((size_t)(qp + 64) * 8 * 2);
+    double* sd = c->svd_s.as<double>();
+    TN_HIP(col_norms(X, ldx, p, q, sd, s));
+    std::vector<double> sv(q);
+    TN_HIP(hipMemcpyAsync(sv.data(), sd, q * 8, hipMemcpyDeviceToHost, s));
+    sync(c);
+    std::vector<int> perm(q);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return sv[a] > sv[b]; });
+    const double smax = q > 0 ? sv[perm[0]] : 0.0;
+    std::vector<double> inv(qp, 1.0);
+    std::vector<char> good(q, 0);
+    for (int j = 0; j < q; j++) {
+        if (sv[j] > 1e-300 && sv[j] > 1e-140 * smax) {
+            good[j] = 1;
+            inv[j] = 1.0 / sv[j];
+        }
+    }
+    // normalise the good columns of X
+    double* dinv = sd + qp + 32;
+    TN_HIP(hipMemcpyAsync(dinv, inv.data(), qp * 8, hipMemcpyHostToDevice, s));
+    TN_HIP(scale_rows(X, ldx, q, p, dinv, s));
+    // orthonormal completion of (exactly) zero columns, in the sorted order
+    for (int jj = 0; jj < q; jj++) {
+        const int j = perm[jj];
+        if (good[j]) continue;
+        if (p <= 0) break;
+        double* xj = X + (long)j * ldx;
+        long pcap = mdot_part_size(p, q);
+        c->part.ensure(pcap * 8);
+        c->lanczos_small.ensure(sizeof(double) * (4 * 512 + 64));
+        double* h = c->lanczos_small.as<double>();
+        c->pack.ensure((size_t)q * 8 + 64);
+        double* hc = c->pack.as<double>();
+        for (int tries = 0; tries < p && !good[j]; tries++) {
+            TN_HIP(set_unit_col(xj, p, (jj + tries) % p, s));
+            for (int pass = 0; pass < 2; pass++) {
+                std::vector<double> coef(q);
+                if (q > 512) throw TnError{TN_ERR_ARG, "svd completion: q > 512"};
+                TN_HIP(mdot(X, ldx, q, xj, p, h, c->part.as<double>(), (int)pcap, s));
+                TN_HIP(hipMemcpyAsync(coef.data(), h, q * 8, hipMemcpyDeviceToHost, s));
+                sync(c);
+                for (int t = 0; t < q; t++)
+                    if (!good[t] || t == j) coef[t] = 0.0;
+                TN_HIP(hipMemcpyAsync(hc, coef.data(), q * 8, hipMemcpyHostToDevice, s));
+                TN_HIP(gemv_acc(X, ldx, q, hc, -1.0, 1.0, xj, p, s));
+            }
+            double nr2 = 0;
+            TN_HIP(mdot(xj, p, 1, xj, p, h, c->part.as<double>(), (int)pcap, s));
+            TN_HIP(hipMemcpyAsync(&nr2, h, 8, hipMemcpyDeviceToHost, s));
+            sync(c);
+            if (nr2 > 0.25) {
+                double f = 1.0 / std::sqrt(nr2);
+                std::vector<double> one(1, f);
+                TN_HIP(hipMemcpyAsync(hc, one.data(), 8, hipMemcpyHostToDevice, s));
+                TN_HIP(scale_rows(xj, p, 1, p, hc, s));
+                sync(c);
+                good[j] = 1;
+            }
+        }
+    }
+    // gather into U / Vt (zero padded first)
+    if (zr > 0) TN_HIP(fill_zero(U, (long)zr * ldu * 8, s));
+    if (zk > 0) TN_HIP(fill_zero(Vt, (long)zk * ldvt * 8, s));
+    int* dperm = (int*)(sd + 2 * qp + 64);
+    c->svd_perm.ensure((size_t)(q + 64) * 12);
+    dperm = c->svd_perm.as<int>();
+    double* ones = (double*)(dperm + ((q + 63) / 64) * 64);
+    std::vector<double> onev(k, 1.0);
+    TN_HIP(hipMemcpyAsync(dperm, perm.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
+    TN_HIP(hipMemcpyAsync(ones, onev.data(), k * 8, hipMemcpyHostToDevice, s));
+    if (tall) {
+        // U (r x k) from X, Vt (k x c) from V
+        TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, U, ldu, 1, Vt, ldvt, 1, s));
+    } else {
+        // Vt (k x c) from X (columns of length c), U (r x k) from V
+        TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, Vt, 1, ldvt, U, 1, ldu, s));
+    }
+    s_out.resize(k);
+    for (int j = 0; j < k; j++) s_out[j] = sv[perm[j]];
+    sync(c);
+    return k;
+}
+
+}  // namespace tn

@@ -1,0 +1,103 @@
+// tn_kernels.h -- launch wrappers for the gfx950 kernels of the DMRG hot path.
+// Internal header (not part of the C ABI).  All launches are asynchronous on
+// the given stream; pointers are device pointers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tn {
+
+typedef uint16_t bf16;  // storage type of bfloat16 operands
+
+enum Prec { P_FP64 = 0, P_FP32 = 1, P_BF16 = 2 };
+enum DType { T_F64 = 0, T_F32 = 1, T_BF16 = 2 };
+
+inline size_t dtype_size(int t) { return t == T_F64 ? 8 : (t == T_F32 ? 4 : 2); }
+
+// C[b] = alpha * op(A[b]) * op(B[b]) (+ C[b] if accumulate)
+//   a_kmajor: A(m,k) = A[m*lda + k]   else A[k*lda + m]
+//   b_kmajor: B(k,n) = B[n*ldb + k]   else B[k*ldb + n]
+// M, N must be multiples of 32, K a multiple of 32 (operands are zero padded).
+struct GemmDesc {
+    int tin;   // DType of A and B
+    int tout;  // DType of C
+    bool a_kmajor, b_kmajor;
+    int M, N, K, batch;
+    const void* A; long lda, sA;
+    const void* B; long ldb, sB;
+    void* C; long ldc, sC;
+    float alpha;
+    int accumulate;
+    int tile;  // 0 = automatic, else forced tile code (autotuner)
+};
+hipError_t gemm(const GemmDesc& g, hipStream_t s);
+int gemm_num_tiles();
+const char* gemm_tile_name(int code);
+
+// MPO mixing: for every position (x, b), out[x*osx + ooff[o] + b] =
+//   sum_{e in optr[o]..optr[o+1]} eval[e] * in[x*isx + eoff[e] + b]
+struct MixTable {
+    int nout, nnz;
+    const int* optr;     // device, nout+1
+    const long* ooff;    // device, nout
+    const long* eoff;    // device, nnz
+    const double* eval;  // device, nnz
+    const float* evalf;  // device, nnz (same values, fp32)
+};
+hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx, int X,
+                int Bn, const MixTable& t, hipStream_t s);
+
+// elementwise
+hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s);
+hipError_t fill_zero(void* p, long bytes, hipStream_t s);
+hipError_t fill_value(double* p, double v, long n, hipStream_t s);
+
+// copy a 3-index tensor with a strided view: out[(i*n1 + j)*n2o + k] = in[i*si + j*sj + k*sk]
+// for i<n0, j<n1, k<n2; entries k in [n2, n2o) are zeroed.
+hipError_t copy3(const double* in, long si, long sj, long sk, double* out, int n0, int n1,
+                 int n2, int n2o, hipStream_t s);
+// general strided 2-D copy with zero fill of the destination block
+// out[i*so0 + j*so1] = (i<r && j<c) ? in[i*si0 + j*si1] : 0, for i<ro, j<co
+hipError_t copy2(const double* in, long si0, long si1, int r, int c, double* out, long so0,
+                 long so1, int ro, int co, hipStream_t s);
+
+// Lanczos vector kernels (fp64)
+// out[i] = sum_n V[i*ldv + n] * w[n], i < k     (two-stage deterministic)
+hipError_t mdot(const double* V, long ldv, int k, const double* w, long n, double* out,
+                double* part, int part_cap, hipStream_t s);
+long mdot_part_size(long n, int k);
+// w[n] = beta_w * w[n] + sum_i coef[i] * V[i*ldv + n]
+hipError_t gemv_acc(const double* V, long ldv, int k, const double* coef, double coef_scale,
+                    double beta_w, double* w, long n, hipStream_t s);
+// v[n] *= 1/den[0] (den on device)
+hipError_t scale_by_inv(double* v, const double* den, long n, hipStream_t s);
+hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStream_t s);
+
+// tridiagonal Ritz solve on the device:
+// out[0]=theta0, out[1]=theta1, out[2]=residual=|beta[k-1]*z[k-1]|, out[3]=beta[k-1]; z (k)
+hipError_t tridiag_ritz(const double* alpha, const double* beta, int k, double* z,
+                        double* out, hipStream_t s);
+// alpha[j] = h[j]   (copy a device scalar)
+hipError_t dcopy_scalar(const double* src, double* dst, hipStream_t s);
+// dst[0] = sqrt(src[0])
+hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s);
+
+// one-sided block Jacobi SVD pieces (fp64, column-major X with ldx, q%32==0)
+hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv,
+                       const int* pairs, int npairs, double tol, int* flag, hipStream_t s);
+hipError_t col_norms(const double* X, long ldx, int p, int q, double* s, hipStream_t st);
+hipError_t set_int(int* p, int v, hipStream_t s);
+hipError_t set_identity(double* V, long ldv, int n, hipStream_t s);
+// out1[i*s10 + j*s11] = X[perm[j]*ldx + i] * scl[j]   (i < p, j < k)
+// out2[j*s20 + i*s21] = V[perm[j]*ldv + i]            (i < q, j < k)
+hipError_t svd_gather(const double* X, long ldx, int p, const double* V, long ldv, int q,
+                      const int* perm, const double* scl, int k, double* out1, long s10, long s11,
+                      double* out2, long s20, long s21, hipStream_t s);
+// unit vector e_pos in column j of X (length p)
+hipError_t set_unit_col(double* x, int p, int pos, hipStream_t s);
+
+// scale columns / rows: A[i*lda + j] *= sv[j] (cols) or sv[i] (rows), i<r, j<c
+hipError_t scale_cols(double* A, long lda, int r, int c, const double* sv, hipStream_t s);
+hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipStream_t s);
+
+}  // namespace tn

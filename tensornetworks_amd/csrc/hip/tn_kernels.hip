@@ -1,0 +1,1053 @@
+// tn_kernels.hip -- CDNA4 (gfx950) kernels for the variational MPS sweep.
+//
+// * gemm_k     : LDS-tiled GEMM on the matrix cores (v_mfma 16x16, fp64/fp32/bf16
+//                operands, fp32/fp64 accumulate), register-prefetch pipelined,
+//                XCD-aware tile order.  The three stages of Heff*psi and of the
+//                environment updates (MPSSite_Contractions.C:155-400) are GEMMs
+//                once the MPO index is mixed by wmix_k.
+// * wmix_k     : applies the sparse MPO tensor W[w][w'][m][n] between the GEMMs
+//                (coalesced along the trailing bond index).
+// * mdot/gemv  : Lanczos re-orthogonalisation (fp64, deterministic reductions).
+// * tridiag_k  : Sturm multisection + inverse iteration for the Ritz pair.
+// * jacobi_k   : one block pair of a one-sided block-Jacobi SVD (fp64).
+#include "tn_kernels.h"
+
+#include <type_traits>
+
+namespace tn {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// scalar conversions
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+template <typename T> __device__ __forceinline__ T cvt_from_d(double v);
+template <> __device__ __forceinline__ double cvt_from_d<double>(double v) { return v; }
+template <> __device__ __forceinline__ float cvt_from_d<float>(double v) { return (float)v; }
+template <> __device__ __forceinline__ bf16 cvt_from_d<bf16>(double v) { return f2bf((float)v); }
+template <typename T> __device__ __forceinline__ T cvt_from_f(float v);
+template <> __device__ __forceinline__ double cvt_from_f<double>(float v) { return (double)v; }
+template <> __device__ __forceinline__ float cvt_from_f<float>(float v) { return v; }
+template <> __device__ __forceinline__ bf16 cvt_from_f<bf16>(float v) { return f2bf(v); }
+__device__ __forceinline__ double to_d(double v) { return v; }
+__device__ __forceinline__ double to_d(float v) { return (double)v; }
+__device__ __forceinline__ double to_d(bf16 v) { return (double)bf2f(v); }
+__device__ __forceinline__ float to_f(double v) { return (float)v; }
+__device__ __forceinline__ float to_f(float v) { return v; }
+__device__ __forceinline__ float to_f(bf16 v) { return bf2f(v); }
+
+// ---------------------------------------------------------------------------
+// MFMA traits: 16x16 output block per wave instruction
+//   A frag: lane l holds A[l%16][kbase + (l/16)*KPL + 0..KPL-1]
+//   B frag: lane l holds B[kbase + (l/16)*KPL + ..][l%16]
+//   C    : lane l holds C[row(l, r)][l%16], r = 0..3, with
+//          row = (l/16)*4 + r for the f32/bf16 instructions and
+//          row = (l/16) + 4*r for f64 (measured on gfx950 with a one-wave probe)
+template <typename TI> struct MT;
+template <> struct MT<double> {
+    static constexpr int KPL = 1, MK = 4, VEC = 2, PAD = 2;
+    static __device__ __forceinline__ int crow(int lane, int r) { return (lane >> 4) + 4 * r; }
+    typedef double frag_t;
+    typedef v4d acc_t;
+    typedef double acc_e;
+};
+template <> struct MT<float> {
+    static constexpr int KPL = 1, MK = 4, VEC = 4, PAD = 4;
+    static __device__ __forceinline__ int crow(int lane, int r) { return (lane >> 4) * 4 + r; }
+    typedef float frag_t;
+    typedef v4f acc_t;
+    typedef float acc_e;
+};
+template <> struct MT<bf16> {
+    static constexpr int KPL = 8, MK = 32, VEC = 8, PAD = 8;
+    static __device__ __forceinline__ int crow(int lane, int r) { return (lane >> 4) * 4 + r; }
+    typedef v8bf frag_t;
+    typedef v4f acc_t;
+    typedef float acc_e;
+};
+
+__device__ __forceinline__ v4d mma(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mma(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mma(v8bf a, v8bf b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct GemmParams {
+    const void* A;
+    const void* B;
+    void* C;
+    int M, N, K;
+    long lda, ldb, ldc, sA, sB, sC;
+    float alpha;
+    int accumulate;
+};
+
+// element j of a 16-byte vector holding VEC values of type TI
+template <typename TI> __device__ __forceinline__ TI vec_elem(const u32x4& v, int j);
+template <> __device__ __forceinline__ double vec_elem<double>(const u32x4& v, int j)
+{
+    return j == 0 ? __hiloint2double((int)v.y, (int)v.x) : __hiloint2double((int)v.w, (int)v.z);
+}
+template <> __device__ __forceinline__ float vec_elem<float>(const u32x4& v, int j)
+{
+    const uint32_t w = j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+    return __uint_as_float(w);
+}
+template <> __device__ __forceinline__ bf16 vec_elem<bf16>(const u32x4& v, int j)
+{
+    const uint32_t w = (j >> 1) == 0 ? v.x : ((j >> 1) == 1 ? v.y : ((j >> 1) == 2 ? v.z : v.w));
+    return (bf16)((j & 1) ? (w >> 16) : (w & 0xffffu));
+}
+
+// global -> registers for one operand tile (ROWS x BK, K-major in LDS)
+//   KM  : operand(row, k) = G[row*ld + k]   (vector loads along k)
+//   !KM : operand(row, k) = G[k*ld + row]   (vector loads along row)
+template <typename TI, int ROWS, int BK, bool KM, int R>
+__device__ __forceinline__ void tile_gload(u32x4 (&rg)[R], const TI* __restrict__ G, long ld,
+                                           int r0, int k0, int tid)
+{
+    constexpr int VEC = MT<TI>::VEC;
+    constexpr int TOT = ROWS * BK / VEC;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int v = tid + r * 256;
+        if (TOT % 256 == 0 || v < TOT) {
+            if constexpr (KM) {
+                const int row = v / (BK / VEC), kv = v % (BK / VEC);
+                rg[r] = *(const u32x4*)(G + (long)(r0 + row) * ld + k0 + kv * VEC);
+            } else {
+                const int kk = v / (ROWS / VEC), rv = v % (ROWS / VEC);
+                rg[r] = *(const u32x4*)(G + (long)(k0 + kk) * ld + r0 + rv * VEC);
+            }
+        }
+    }
+}
+template <typename TI, int ROWS, int BK, bool KM, int R>
+__device__ __forceinline__ void tile_sstore(const u32x4 (&rg)[R], TI* __restrict__ S, int tid)
+{
+    constexpr int VEC = MT<TI>::VEC;
+    constexpr int TOT = ROWS * BK / VEC;
+    constexpr int LDK = BK + MT<TI>::PAD;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int v = tid + r * 256;
+        if (TOT % 256 == 0 || v < TOT) {
+            if constexpr (KM) {
+                const int row = v / (BK / VEC), kv = v % (BK / VEC);
+                *(u32x4*)(S + row * LDK + kv * VEC) = rg[r];
+            } else {
+                const int kk = v / (ROWS / VEC), rv = v % (ROWS / VEC);
+#pragma unroll
+                for (int j = 0; j < VEC; j++) S[(rv * VEC + j) * LDK + kk] = vec_elem<TI>(rg[r], j);
+            }
+        }
+    }
+}
+
+// 256 threads = 4 waves in a 2x2 arrangement; wave tile (BM/2) x (BN/2).
+template <typename TI, typename TO, int BM, int BN, int BK, bool AKM, bool BKM>
+__global__ void __launch_bounds__(256) gemm_k(GemmParams p)
+{
+    typedef MT<TI> T;
+    typedef typename T::frag_t frag_t;
+    typedef typename T::acc_t acc_t;
+    constexpr int LDK = BK + T::PAD;
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int VEC = T::VEC;
+    constexpr int TA = BM * BK / VEC, TB = BN * BK / VEC;
+    constexpr int RA = (TA + 255) / 256, RB = (TB + 255) / 256;
+    static_assert(FM >= 1 && FN >= 1, "tile");
+    static_assert(BK % T::MK == 0, "BK");
+
+    __shared__ __attribute__((aligned(16))) TI As[BM * LDK];
+    __shared__ __attribute__((aligned(16))) TI Bs[BN * LDK];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
+    const int ntn = p.N / BN, ntm = p.M / BM;
+    const int nblk = ntn * ntm;
+    int bid = blockIdx.x;
+    // consecutive block ids land on different XCDs; give each XCD a contiguous
+    // range of output tiles so tiles sharing A rows share an L2
+    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    const int tm = bid / ntn, tn = bid - tm * ntn;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const long bz = blockIdx.y;
+    const TI* __restrict__ A = (const TI*)p.A + bz * p.sA;
+    const TI* __restrict__ B = (const TI*)p.B + bz * p.sB;
+    TO* __restrict__ C = (TO*)p.C + bz * p.sC;
+
+    u32x4 ra[RA], rb[RB];
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+
+    const int nk = p.K / BK;
+    tile_gload<TI, BM, BK, AKM>(ra, A, p.lda, m0, 0, tid);
+    tile_gload<TI, BN, BK, BKM>(rb, B, p.ldb, n0, 0, tid);
+    tile_sstore<TI, BM, BK, AKM>(ra, As, tid);
+    tile_sstore<TI, BN, BK, BKM>(rb, Bs, tid);
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        if (t + 1 < nk) {
+            tile_gload<TI, BM, BK, AKM>(ra, A, p.lda, m0, (t + 1) * BK, tid);
+            tile_gload<TI, BN, BK, BKM>(rb, B, p.ldb, n0, (t + 1) * BK, tid);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += T::MK) {
+            frag_t a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+                a[i] = *(const frag_t*)(As + (wm0 + i * 16 + (lane & 15)) * LDK + kk + (lane >> 4) * T::KPL);
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+                b[j] = *(const frag_t*)(Bs + (wn0 + j * 16 + (lane & 15)) * LDK + kk + (lane >> 4) * T::KPL);
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++) acc[i][j] = mma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+        if (t + 1 < nk) {
+            tile_sstore<TI, BM, BK, AKM>(ra, As, tid);
+            tile_sstore<TI, BN, BK, BKM>(rb, Bs, tid);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long row = m0 + wm0 + i * 16 + T::crow(lane, r);
+                const long col = n0 + wn0 + j * 16 + (lane & 15);
+                TO* cp = C + row * p.ldc + col;
+                if constexpr (std::is_same<TO, double>::value) {
+                    double v = (double)acc[i][j][r] * (double)p.alpha;
+                    if (p.accumulate) v += to_d(*cp);
+                    *cp = cvt_from_d<TO>(v);
+                } else {
+                    float v = (float)acc[i][j][r] * p.alpha;
+                    if (p.accumulate) v += to_f(*cp);
+                    *cp = cvt_from_f<TO>(v);
+                }
+            }
+}
+
+// tile menu (code -> BM x BN)
+struct TileCfg { int bm, bn; const char* name; };
+static const TileCfg kTiles[] = {
+    {128, 128, "128x128"}, {128, 64, "128x64"}, {64, 128, "64x128"},
+    {64, 64, "64x64"},     {32, 64, "32x64"},   {64, 32, "64x32"},   {32, 32, "32x32"},
+};
+static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+int gemm_num_tiles() { return kNumTiles; }
+const char* gemm_tile_name(int code) { return (code >= 1 && code <= kNumTiles) ? kTiles[code - 1].name : "auto"; }
+
+template <typename TI, typename TO, int BK, bool AKM, bool BKM>
+static hipError_t launch_tile(int code, const GemmParams& p, int batch, hipStream_t s)
+{
+    const TileCfg& t = kTiles[code];
+    dim3 grid((p.M / t.bm) * (p.N / t.bn), batch);
+    switch (code) {
+    case 0: gemm_k<TI, TO, 128, 128, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 1: gemm_k<TI, TO, 128, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 2: gemm_k<TI, TO, 64, 128, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 3: gemm_k<TI, TO, 64, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 4: gemm_k<TI, TO, 32, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 5: gemm_k<TI, TO, 64, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    default: gemm_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    }
+    return hipGetLastError();
+}
+
+// Pick the largest tile that divides the problem and still gives >= ~1.5
+// workgroups per CU (256 CUs); fall back to the tile with the most blocks.
+static int choose_tile(int M, int N, int batch)
+{
+    int best = -1;
+    long best_blocks = -1;
+    for (int c = 0; c < kNumTiles; c++) {
+        if (M % kTiles[c].bm || N % kTiles[c].bn) continue;
+        long blocks = (long)(M / kTiles[c].bm) * (N / kTiles[c].bn) * batch;
+        if (blocks >= 384) return c;
+        if (blocks > best_blocks) { best_blocks = blocks; best = c; }
+    }
+    return best;
+}
+
+template <typename TI, typename TO, bool AKM, bool BKM>
+static hipError_t gemm_typed(const GemmDesc& g, hipStream_t s)
+{
+    GemmParams p;
+    p.A = g.A; p.B = g.B; p.C = g.C;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.lda = g.lda; p.ldb = g.ldb; p.ldc = g.ldc;
+    p.sA = g.sA; p.sB = g.sB; p.sC = g.sC;
+    p.alpha = g.alpha; p.accumulate = g.accumulate;
+    int code = g.tile > 0 ? g.tile - 1 : choose_tile(g.M, g.N, g.batch);
+    if (code < 0 || code >= kNumTiles || g.M % kTiles[code].bm || g.N % kTiles[code].bn)
+        return hipErrorInvalidValue;
+    if constexpr (std::is_same<TI, double>::value) {
+        if (g.K % 16) return hipErrorInvalidValue;
+        return launch_tile<TI, TO, 16, AKM, BKM>(code, p, g.batch, s);
+    } else if constexpr (std::is_same<TI, float>::value) {
+        if (g.K % 32 == 0) return launch_tile<TI, TO, 32, AKM, BKM>(code, p, g.batch, s);
+        if (g.K % 16) return hipErrorInvalidValue;
+        return launch_tile<TI, TO, 16, AKM, BKM>(code, p, g.batch, s);
+    } else {
+        if (g.K % 64 == 0) return launch_tile<TI, TO, 64, AKM, BKM>(code, p, g.batch, s);
+        if (g.K % 32) return hipErrorInvalidValue;
+        return launch_tile<TI, TO, 32, AKM, BKM>(code, p, g.batch, s);
+    }
+}
+
+template <bool AKM, bool BKM>
+static hipError_t gemm_layout(const GemmDesc& g, hipStream_t s)
+{
+    if (g.tin == T_F64) {
+        if (g.tout == T_F64) return gemm_typed<double, double, AKM, BKM>(g, s);
+        return hipErrorInvalidValue;
+    }
+    if (g.tin == T_F32) {
+        if (g.tout == T_F32) return gemm_typed<float, float, AKM, BKM>(g, s);
+        if (g.tout == T_F64) return gemm_typed<float, double, AKM, BKM>(g, s);
+        return hipErrorInvalidValue;
+    }
+    if (g.tin == T_BF16) {
+        if (g.tout == T_F32) return gemm_typed<bf16, float, AKM, BKM>(g, s);
+        if (g.tout == T_F64) return gemm_typed<bf16, double, AKM, BKM>(g, s);
+        if (g.tout == T_BF16) return gemm_typed<bf16, bf16, AKM, BKM>(g, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t gemm(const GemmDesc& g, hipStream_t s)
+{
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipErrorInvalidValue;
+    if (g.M % 32 || g.N % 32 || g.K % 16) return hipErrorInvalidValue;
+    if (g.a_kmajor && g.b_kmajor) return gemm_layout<true, true>(g, s);
+    if (g.a_kmajor && !g.b_kmajor) return gemm_layout<true, false>(g, s);
+    if (!g.a_kmajor && !g.b_kmajor) return gemm_layout<false, false>(g, s);
+    return gemm_layout<false, true>(g, s);
+}
+
+// ---------------------------------------------------------------------------
+// MPO mixing
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256) wmix_k(const TI* __restrict__ in, long isx,
+                                              TO* __restrict__ out, long osx, int Bn, int nout,
+                                              const int* __restrict__ optr,
+                                              const long* __restrict__ ooff,
+                                              const long* __restrict__ eoff,
+                                              const double* __restrict__ eval,
+                                              const float* __restrict__ evalf)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= Bn) return;
+    const long x = blockIdx.y;
+    const TI* ip = in + x * isx + b;
+    TO* op = out + x * osx + b;
+    for (int o = 0; o < nout; o++) {
+        const int e0 = optr[o], e1 = optr[o + 1];
+        if constexpr (std::is_same<TI, double>::value) {
+            double acc = 0.0;
+            for (int e = e0; e < e1; e++) acc += eval[e] * to_d(ip[eoff[e]]);
+            op[ooff[o]] = cvt_from_d<TO>(acc);
+        } else {
+            float acc = 0.0f;
+            for (int e = e0; e < e1; e++) acc += evalf[e] * to_f(ip[eoff[e]]);
+            op[ooff[o]] = cvt_from_f<TO>(acc);
+        }
+    }
+}
+
+hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx, int X, int Bn,
+                const MixTable& t, hipStream_t s)
+{
+    dim3 grid((Bn + 255) / 256, X);
+#define WM_CASE(TIc, TOc, TIt, TOt)                                                         \
+    if (tin == TIc && tout == TOc) {                                                        \
+        wmix_k<TIt, TOt><<<grid, 256, 0, s>>>((const TIt*)in, isx, (TOt*)out, osx, Bn,      \
+                                               t.nout, t.optr, t.ooff, t.eoff, t.eval,      \
+                                               t.evalf);                                    \
+        return hipGetLastError();                                                           \
+    }
+    WM_CASE(T_F64, T_F64, double, double)
+    WM_CASE(T_F32, T_F32, float, float)
+    WM_CASE(T_F32, T_BF16, float, bf16)
+    WM_CASE(T_F32, T_F64, float, double)
+#undef WM_CASE
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise
+template <typename TI, typename TO>
+__global__ void convert_k(const TI* __restrict__ in, TO* __restrict__ out, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        out[i] = cvt_from_d<TO>(to_d(in[i]));
+}
+
+hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+#define CV_CASE(TIc, TOc, TIt, TOt)                                                           \
+    if (tin == TIc && tout == TOc) {                                                          \
+        convert_k<TIt, TOt><<<blocks, 256, 0, s>>>((const TIt*)in, (TOt*)out, n);             \
+        return hipGetLastError();                                                             \
+    }
+    CV_CASE(T_F64, T_F32, double, float)
+    CV_CASE(T_F64, T_BF16, double, bf16)
+    CV_CASE(T_F32, T_F64, float, double)
+    CV_CASE(T_BF16, T_F64, bf16, double)
+    CV_CASE(T_F64, T_F64, double, double)
+#undef CV_CASE
+    return hipErrorInvalidValue;
+}
+
+__global__ void fill_zero_k(uint32_t* p, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0u;
+}
+hipError_t fill_zero(void* p, long bytes, hipStream_t s)
+{
+    if (bytes <= 0) return hipSuccess;
+    long n = bytes / 4;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    fill_zero_k<<<blocks, 256, 0, s>>>((uint32_t*)p, n);
+    return hipGetLastError();
+}
+__global__ void fill_value_k(double* p, double v, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = v;
+}
+hipError_t fill_value(double* p, double v, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    fill_value_k<<<blocks, 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
+
+__global__ void copy3_k(const double* __restrict__ in, long si, long sj, long sk,
+                        double* __restrict__ out, int n1, int n2, int n2o)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n2o) return;
+    const int j = blockIdx.y, i = blockIdx.z;
+    out[((long)i * n1 + j) * n2o + k] = (k < n2) ? in[i * si + j * sj + k * sk] : 0.0;
+}
+hipError_t copy3(const double* in, long si, long sj, long sk, double* out, int n0, int n1,
+                 int n2, int n2o, hipStream_t s)
+{
+    if (n0 <= 0 || n1 <= 0 || n2o <= 0) return hipSuccess;
+    dim3 grid((n2o + 255) / 256, n1, n0);
+    copy3_k<<<grid, 256, 0, s>>>(in, si, sj, sk, out, n1, n2, n2o);
+    return hipGetLastError();
+}
+
+__global__ void copy2_k(const double* __restrict__ in, long si0, long si1, int r, int c,
+                        double* __restrict__ out, long so0, long so1, int co)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= co) return;
+    const int i = blockIdx.y;
+    out[i * so0 + j * so1] = (i < r && j < c) ? in[i * si0 + j * si1] : 0.0;
+}
+hipError_t copy2(const double* in, long si0, long si1, int r, int c, double* out, long so0,
+                 long so1, int ro, int co, hipStream_t s)
+{
+    if (ro <= 0 || co <= 0) return hipSuccess;
+    dim3 grid((co + 255) / 256, ro);
+    copy2_k<<<grid, 256, 0, s>>>(in, si0, si1, r, c, out, so0, so1, co);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos vector kernels
+#define MD_BLOCK 256
+#define MD_EPT 8
+#define MD_MAXK 512
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+long mdot_part_size(long n, int k)
+{
+    long blocks = (n + MD_BLOCK * MD_EPT - 1) / (MD_BLOCK * MD_EPT);
+    return blocks * (long)k;
+}
+
+// stage 1: part[i*G + blk] = sum over the block's chunk of V[i]*w
+__global__ void __launch_bounds__(MD_BLOCK) mdot1_k(const double* __restrict__ V, long ldv, int k,
+                                                    const double* __restrict__ w, long n,
+                                                    double* __restrict__ part)
+{
+    __shared__ double red[MD_MAXK][MD_BLOCK / 64];
+    const long base = (long)blockIdx.x * MD_BLOCK * MD_EPT + threadIdx.x;
+    double wr[MD_EPT];
+#pragma unroll
+    for (int e = 0; e < MD_EPT; e++) {
+        long idx = base + (long)e * MD_BLOCK;
+        wr[e] = idx < n ? w[idx] : 0.0;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i < k; i++) {
+        const double* vi = V + (long)i * ldv;
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < MD_EPT; e++) {
+            long idx = base + (long)e * MD_BLOCK;
+            if (idx < n) acc += vi[idx] * wr[e];
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) red[i][wid] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += MD_BLOCK) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < MD_BLOCK / 64; q++) s += red[i][q];
+        part[(long)i * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// stage 2: out[i] = sum_b part[i*G + b]  (one block per i, fixed order)
+__global__ void __launch_bounds__(256) mdot2_k(const double* __restrict__ part, int G,
+                                               double* __restrict__ out)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < G; b += 256) acc += part[(long)i * G + b];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = red[0] + red[1] + red[2] + red[3];
+}
+
+hipError_t mdot(const double* V, long ldv, int k, const double* w, long n, double* out,
+                double* part, int part_cap, hipStream_t s)
+{
+    if (k <= 0) return hipSuccess;
+    if (k > MD_MAXK) return hipErrorInvalidValue;
+    long G = (n + MD_BLOCK * MD_EPT - 1) / (MD_BLOCK * MD_EPT);
+    if (G * k > part_cap) return hipErrorInvalidValue;
+    mdot1_k<<<G, MD_BLOCK, 0, s>>>(V, ldv, k, w, n, part);
+    mdot2_k<<<k, 256, 0, s>>>(part, (int)G, out);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) gemv_acc_k(const double* __restrict__ V, long ldv, int k,
+                                                  const double* __restrict__ coef,
+                                                  double coef_scale, double beta_w,
+                                                  double* __restrict__ w, long n)
+{
+    __shared__ double c[MD_MAXK];
+    for (int i = threadIdx.x; i < k; i += 256) c[i] = coef[i] * coef_scale;
+    __syncthreads();
+    for (long idx = blockIdx.x * 256L + threadIdx.x; idx < n; idx += (long)gridDim.x * 256) {
+        double acc = (beta_w == 0.0) ? 0.0 : beta_w * w[idx];
+        for (int i = 0; i < k; i++) acc += c[i] * V[(long)i * ldv + idx];
+        w[idx] = acc;
+    }
+}
+hipError_t gemv_acc(const double* V, long ldv, int k, const double* coef, double coef_scale,
+                    double beta_w, double* w, long n, hipStream_t s)
+{
+    if (k > MD_MAXK) return hipErrorInvalidValue;
+    long blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    gemv_acc_k<<<blocks, 256, 0, s>>>(V, ldv, k, coef, coef_scale, beta_w, w, n);
+    return hipGetLastError();
+}
+
+__global__ void scale_by_inv_k(double* v, const double* den, long n)
+{
+    const double f = 1.0 / den[0];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) v[i] *= f;
+}
+hipError_t scale_by_inv(double* v, const double* den, long n, hipStream_t s)
+{
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    scale_by_inv_k<<<blocks, 256, 0, s>>>(v, den, n);
+    return hipGetLastError();
+}
+__global__ void axpby_k(double a, const double* x, double b, double* y, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        y[i] = a * x[i] + b * y[i];
+}
+hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStream_t s)
+{
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    axpby_k<<<blocks, 256, 0, s>>>(a, x, b, y, n);
+    return hipGetLastError();
+}
+
+__global__ void dcopy_scalar_k(const double* src, double* dst)
+{
+    if (threadIdx.x == 0) dst[0] = src[0];
+}
+hipError_t dcopy_scalar(const double* src, double* dst, hipStream_t s)
+{
+    dcopy_scalar_k<<<1, 64, 0, s>>>(src, dst);
+    return hipGetLastError();
+}
+__global__ void dsqrt_scalar_k(const double* src, double* dst)
+{
+    if (threadIdx.x == 0) dst[0] = sqrt(fmax(src[0], 0.0));
+}
+hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s)
+{
+    dsqrt_scalar_k<<<1, 64, 0, s>>>(src, dst);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// tridiagonal Ritz solve: one wavefront.  Sturm multisection gives the two
+// lowest eigenvalues; inverse iteration (LU with partial pivoting, one lane)
+// gives the Ritz coefficients of the lowest.
+#define TD_MAXK 512
+
+__device__ int sturm_count_d(int k, const double* a, const double* b2, double x)
+{
+    int cnt = 0;
+    double q = a[0] - x;
+    if (q < 0) cnt++;
+    for (int i = 1; i < k; i++) {
+        if (q == 0.0) q = 1e-300;
+        q = a[i] - x - b2[i - 1] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+
+__device__ double multisect(int k, const double* a, const double* b2, double lo, double hi,
+                            int idx, int lane, double* sh_lo, double* sh_hi)
+{
+    for (int it = 0; it < 16; it++) {
+        const double step = (hi - lo) / 65.0;
+        const double x = lo + step * (lane + 1);
+        const int above = sturm_count_d(k, a, b2, x) > idx;  // x > lambda_idx
+        // smallest lane with 'above'
+        unsigned long long mask = __ballot(above);
+        int first = mask ? __ffsll((long long)mask) - 1 : 64;
+        double nlo = (first == 0) ? lo : lo + step * first;
+        double nhi = (first == 64) ? hi : lo + step * (first + 1);
+        lo = nlo;
+        hi = nhi;
+        if (!(hi - lo > 0.0)) break;
+    }
+    (void)sh_lo;
+    (void)sh_hi;
+    return 0.5 * (lo + hi);
+}
+
+__global__ void __launch_bounds__(64) tridiag_k(const double* __restrict__ alpha,
+                                                const double* __restrict__ beta, int k,
+                                                double* __restrict__ z, double* __restrict__ out)
+{
+    __shared__ double a[TD_MAXK], b[TD_MAXK], b2[TD_MAXK];
+    __shared__ double s_dd[TD_MAXK], s_du[TD_MAXK], s_dl[TD_MAXK], s_du2[TD_MAXK];
+    __shared__ double s_rhs[TD_MAXK], s_zz[TD_MAXK];
+    __shared__ unsigned char s_piv[TD_MAXK];
+    __shared__ double th[2];
+    const int lane = threadIdx.x;
+    for (int i = lane; i < k; i += 64) {
+        a[i] = alpha[i];
+        b[i] = beta[i];
+        b2[i] = beta[i] * beta[i];
+    }
+    __syncthreads();
+    // Gershgorin bounds (all lanes compute the same)
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < k; i++) {
+        double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
+        lo = fmin(lo, a[i] - r);
+        hi = fmax(hi, a[i] + r);
+    }
+    const double span = hi - lo + 1e-300;
+    lo -= 1e-10 * span + 1e-300;
+    hi += 1e-10 * span + 1e-300;
+    double t0 = multisect(k, a, b2, lo, hi, 0, lane, nullptr, nullptr);
+    double t1 = (k > 1) ? multisect(k, a, b2, lo, hi, 1, lane, nullptr, nullptr) : t0;
+    if (lane == 0) {
+        th[0] = t0;
+        th[1] = t1;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        // inverse iteration for the eigenvector at t0 (work arrays in LDS)
+        double* dd = s_dd; double* du = s_du; double* dl = s_dl; double* du2 = s_du2;
+        double* rhs = s_rhs; double* zz = s_zz; unsigned char* piv = s_piv;
+        double scale = 0.0;
+        for (int i = 0; i < k; i++) scale = fmax(scale, fabs(a[i]) + (i < k - 1 ? fabs(b[i]) : 0.0));
+        const double pert = (scale > 0 ? scale : 1.0) * 1e-14;
+        for (int i = 0; i < k; i++) zz[i] = 1.0 / sqrt((double)k) * (1.0 + 0.01 * ((i * 7919) % 13) / 13.0);
+        if (k == 1) zz[0] = 1.0;
+        for (int iter = 0; iter < 3 && k > 1; iter++) {
+            for (int i = 0; i < k; i++) {
+                dd[i] = a[i] - t0;
+                du2[i] = 0.0;
+                if (i < k - 1) {
+                    du[i] = b[i];
+                    dl[i] = b[i];
+                }
+            }
+            for (int i = 0; i < k - 1; i++) {
+                if (fabs(dd[i]) >= fabs(dl[i])) {
+                    piv[i] = 0;
+                    if (dd[i] == 0.0) dd[i] = pert;
+                    double f = dl[i] / dd[i];
+                    dl[i] = f;
+                    dd[i + 1] -= f * du[i];
+                } else {
+                    piv[i] = 1;
+                    double f = dd[i] / dl[i];
+                    dd[i] = dl[i];
+                    dl[i] = f;
+                    double tmp = du[i];
+                    du[i] = dd[i + 1];
+                    dd[i + 1] = tmp - f * dd[i + 1];
+                    if (i < k - 2) {
+                        du2[i] = du[i + 1];
+                        du[i + 1] = -f * du[i + 1];
+                    }
+                }
+            }
+            if (dd[k - 1] == 0.0) dd[k - 1] = pert;
+            for (int i = 0; i < k; i++) rhs[i] = zz[i];
+            for (int i = 0; i < k - 1; i++) {
+                if (piv[i] == 0) rhs[i + 1] -= dl[i] * rhs[i];
+                else {
+                    double t = rhs[i];
+                    rhs[i] = rhs[i + 1];
+                    rhs[i + 1] = t - dl[i] * rhs[i + 1];
+                }
+            }
+            rhs[k - 1] /= dd[k - 1];
+            rhs[k - 2] = (rhs[k - 2] - du[k - 2] * rhs[k - 1]) / dd[k - 2];
+            for (int i = k - 3; i >= 0; i--)
+                rhs[i] = (rhs[i] - du[i] * rhs[i + 1] - du2[i] * rhs[i + 2]) / dd[i];
+            double nr = 0.0;
+            for (int i = 0; i < k; i++) nr += rhs[i] * rhs[i];
+            nr = sqrt(nr);
+            for (int i = 0; i < k; i++) zz[i] = rhs[i] / nr;
+        }
+        for (int i = 0; i < k; i++) z[i] = zz[i];
+        out[0] = th[0];
+        out[1] = th[1];
+        out[2] = fabs(b[k - 1] * zz[k - 1]);
+        out[3] = b[k - 1];
+    }
+}
+
+hipError_t tridiag_ritz(const double* alpha, const double* beta, int k, double* z, double* out,
+                        hipStream_t s)
+{
+    if (k < 1 || k > TD_MAXK) return hipErrorInvalidValue;
+    tridiag_k<<<1, 64, 0, s>>>(alpha, beta, k, z, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// one-sided block Jacobi SVD: one workgroup per column-block pair (2 x 16
+// columns).  The 32x32 Gram matrix is diagonalised in LDS by cyclic parallel
+// Jacobi and the rotation applied to the 32 columns of X and of V.
+#define JB 16
+#define JW 32
+
+__global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx, int p,
+                                                double* __restrict__ V, long ldv, int qv,
+                                                const int* __restrict__ pairs, double tol,
+                                                int* __restrict__ flag)
+{
+    __shared__ double G[JW][JW + 1];
+    __shared__ double J[JW][JW + 1];
+    __shared__ double tile[JW][JW + 1];
+    __shared__ double rc[JW / 2], rs[JW / 2];
+    __shared__ int rp[JW / 2], rq[JW / 2];
+    __shared__ int any_rot;
+    __shared__ int sh_off;
+    const int tid = threadIdx.x;
+    const int pb = pairs[2 * blockIdx.x], qb = pairs[2 * blockIdx.x + 1];
+    auto colidx = [&](int j) { return j < JB ? pb * JB + j : qb * JB + (j - JB); };
+
+    // 1. Gram matrix over row chunks staged in LDS
+    double acc[4] = {0, 0, 0, 0};
+    for (int r0 = 0; r0 < p; r0 += JW) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int idx = tid + 256 * u;
+            const int row = idx & (JW - 1), c = idx >> 5;
+            const int gr = r0 + row;
+            tile[row][c] = (gr < p) ? X[(long)colidx(c) * ldx + gr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = tid + 256 * u;
+            const int gi = e >> 5, gj = e & 31;
+            double s = 0.0;
+#pragma unroll 8
+            for (int rr = 0; rr < JW; rr++) s += tile[rr][gi] * tile[rr][gj];
+            acc[u] += s;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int e = tid + 256 * u;
+        G[e >> 5][e & 31] = acc[u];
+        J[e >> 5][e & 31] = ((e >> 5) == (e & 31)) ? 1.0 : 0.0;
+    }
+    if (tid == 0) sh_off = 0;
+    __syncthreads();
+    // convergence measure of this pair (full 32x32 off-diagonal)
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int e = tid + 256 * u;
+        const int gi = e >> 5, gj = e & 31;
+        if (gi < gj) {
+            const double dd = G[gi][gi] * G[gj][gj];
+            if (dd > 1e-300 && fabs(G[gi][gj]) > tol * sqrt(dd)) sh_off = 1;
+        }
+    }
+    __syncthreads();
+    if (sh_off == 0) return;  // already orthogonal: nothing to rotate
+    if (tid == 0) flag[0] = 1;
+
+    // 2. cyclic parallel Jacobi on G (tournament ordering, 31 rounds per sweep)
+    for (int sweep = 0; sweep < 20; sweep++) {
+        if (tid == 0) any_rot = 0;
+        __syncthreads();
+        for (int round = 0; round < JW - 1; round++) {
+            if (tid < JW / 2) {
+                auto elem = [&](int kpos) { return kpos == 0 ? 0 : ((kpos - 1 + round) % (JW - 1)) + 1; };
+                int a = elem(tid), b = elem(JW - 1 - tid);
+                int P = a < b ? a : b, Q = a < b ? b : a;
+                double app = G[P][P], aqq = G[Q][Q], apq = G[P][Q];
+                double c = 1.0, s = 0.0;
+                if (fabs(apq) > 1e-300 && fabs(apq) > 1e-16 * sqrt(fabs(app * aqq))) {
+                    double theta = (aqq - app) / (2.0 * apq);
+                    double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    c = 1.0 / sqrt(t * t + 1.0);
+                    s = t * c;
+                    any_rot = 1;
+                }
+                rc[tid] = c;
+                rs[tid] = s;
+                rp[tid] = P;
+                rq[tid] = Q;
+            }
+            __syncthreads();
+            // rows: G <- R^T G
+            for (int e = tid; e < (JW / 2) * JW; e += 256) {
+                const int pi = e / JW, col = e % JW;
+                const int P = rp[pi], Q = rq[pi];
+                const double c = rc[pi], s = rs[pi];
+                const double gp = G[P][col], gq = G[Q][col];
+                G[P][col] = c * gp - s * gq;
+                G[Q][col] = s * gp + c * gq;
+            }
+            __syncthreads();
+            // columns: G <- G R, J <- J R
+            for (int e = tid; e < (JW / 2) * JW; e += 256) {
+                const int pi = e / JW, row = e % JW;
+                const int P = rp[pi], Q = rq[pi];
+                const double c = rc[pi], s = rs[pi];
+                const double gp = G[row][P], gq = G[row][Q];
+                G[row][P] = c * gp - s * gq;
+                G[row][Q] = s * gp + c * gq;
+                const double jp = J[row][P], jq = J[row][Q];
+                J[row][P] = c * jp - s * jq;
+                J[row][Q] = s * jp + c * jq;
+            }
+            __syncthreads();
+        }
+        if (any_rot == 0) break;
+        __syncthreads();
+    }
+
+    // 3. X[:, cols] <- X[:, cols] J ; V[:, cols] <- V[:, cols] J
+    for (int row = tid; row < p; row += 256) {
+        double x[JW];
+#pragma unroll
+        for (int c = 0; c < JW; c++) x[c] = X[(long)colidx(c) * ldx + row];
+#pragma unroll 4
+        for (int c2 = 0; c2 < JW; c2++) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < JW; c++) s += x[c] * J[c][c2];
+            X[(long)colidx(c2) * ldx + row] = s;
+        }
+    }
+    for (int row = tid; row < qv; row += 256) {
+        double x[JW];
+#pragma unroll
+        for (int c = 0; c < JW; c++) x[c] = V[(long)colidx(c) * ldv + row];
+#pragma unroll 4
+        for (int c2 = 0; c2 < JW; c2++) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < JW; c++) s += x[c] * J[c][c2];
+            V[(long)colidx(c2) * ldv + row] = s;
+        }
+    }
+}
+
+hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv, const int* pairs,
+                       int npairs, double tol, int* flag, hipStream_t s)
+{
+    if (npairs <= 0) return hipSuccess;
+    jacobi_k<<<npairs, 256, 0, s>>>(X, ldx, p, V, ldv, qv, pairs, tol, flag);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) col_norms_k(const double* __restrict__ X, long ldx, int p,
+                                                   double* __restrict__ s)
+{
+    __shared__ double red[4];
+    const double* x = X + (long)blockIdx.x * ldx;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < p; i += 256) acc += x[i] * x[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) s[blockIdx.x] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+hipError_t col_norms(const double* X, long ldx, int p, int q, double* s, hipStream_t st)
+{
+    if (q <= 0) return hipSuccess;
+    col_norms_k<<<q, 256, 0, st>>>(X, ldx, p, s);
+    return hipGetLastError();
+}
+
+__global__ void set_int_k(int* p, int v)
+{
+    if (threadIdx.x == 0) p[0] = v;
+}
+hipError_t set_int(int* p, int v, hipStream_t s)
+{
+    set_int_k<<<1, 64, 0, s>>>(p, v);
+    return hipGetLastError();
+}
+
+__global__ void set_identity_k(double* V, long ldv, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n) V[(long)j * ldv + i] = (i == j) ? 1.0 : 0.0;
+}
+hipError_t set_identity(double* V, long ldv, int n, hipStream_t s)
+{
+    dim3 grid((n + 255) / 256, n);
+    set_identity_k<<<grid, 256, 0, s>>>(V, ldv, n);
+    return hipGetLastError();
+}
+
+__global__ void svd_gather1_k(const double* __restrict__ X, long ldx, int p,
+                              const int* __restrict__ perm, const double* __restrict__ scl,
+                              double* __restrict__ out, long s0, long s1)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= p) return;
+    out[(long)i * s0 + (long)j * s1] = X[(long)perm[j] * ldx + i] * scl[j];
+}
+__global__ void svd_gather2_k(const double* __restrict__ V, long ldv, int q,
+                              const int* __restrict__ perm, double* __restrict__ out, long s0,
+                              long s1)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= q) return;
+    out[(long)j * s0 + (long)i * s1] = V[(long)perm[j] * ldv + i];
+}
+hipError_t svd_gather(const double* X, long ldx, int p, const double* V, long ldv, int q,
+                      const int* perm, const double* scl, int k, double* out1, long s10, long s11,
+                      double* out2, long s20, long s21, hipStream_t s)
+{
+    if (k <= 0) return hipSuccess;
+    dim3 g1((p + 255) / 256, k), g2((q + 255) / 256, k);
+    svd_gather1_k<<<g1, 256, 0, s>>>(X, ldx, p, perm, scl, out1, s10, s11);
+    svd_gather2_k<<<g2, 256, 0, s>>>(V, ldv, q, perm, out2, s20, s21);
+    return hipGetLastError();
+}
+
+__global__ void set_unit_col_k(double* x, int p, int pos)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < p; i += gridDim.x * 256) x[i] = (i == pos) ? 1.0 : 0.0;
+}
+hipError_t set_unit_col(double* x, int p, int pos, hipStream_t s)
+{
+    int blocks = (p + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    set_unit_col_k<<<blocks, 256, 0, s>>>(x, p, pos);
+    return hipGetLastError();
+}
+
+__global__ void scale_cols_k(double* A, long lda, int r, int c, const double* sv)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j < c && i < r) A[(long)i * lda + j] *= sv[j];
+}
+__global__ void scale_rows_k(double* A, long lda, int r, int c, const double* sv)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j < c && i < r) A[(long)i * lda + j] *= sv[i];
+}
+hipError_t scale_cols(double* A, long lda, int r, int c, const double* sv, hipStream_t s)
+{
+    if (r <= 0 || c <= 0) return hipSuccess;
+    dim3 grid((c + 255) / 256, r);
+    scale_cols_k<<<grid, 256, 0, s>>>(A, lda, r, c, sv);
+    return hipGetLastError();
+}
+hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipStream_t s)
+{
+    if (r <= 0 || c <= 0) return hipSuccess;
+    dim3 grid((c + 255) / 256, r);
+    scale_rows_k<<<grid, 256, 0, s>>>(A, lda, r, c, sv);
+    return hipGetLastError();
+}
+
+}  // namespace tn

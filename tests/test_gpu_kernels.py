@@ -1,0 +1,156 @@
+"""GPU parity of the hot-path kernels against the CPU oracle (and numpy).
+
+Every case calls the HIP path through the C ABI (include/tn_abi.h).
+Tolerances: FP64 1e-13 relative (bit-level agreement is not expected since
+summation orders differ); FP32/BF16 operands are compared against fp64
+products of the *rounded* operands, which pins the matrix-core lane layout.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tensornetworks_amd as tn
+
+pytestmark = pytest.mark.gpu
+
+
+def rand(shape, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, shape)
+
+
+def round_fp32(x):
+    return np.asarray(x, dtype=np.float32).astype(np.float64)
+
+
+def round_bf16(x):
+    f = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+ROUND = {0: lambda x: np.asarray(x, dtype=np.float64), 1: round_fp32, 2: round_bf16}
+LAYOUTS = [(True, True), (True, False), (False, False), (False, True)]
+
+
+@pytest.mark.parametrize("akm,bkm", LAYOUTS)
+@pytest.mark.parametrize("tin,tout", [(0, 0), (1, 1), (1, 0), (2, 1), (2, 0), (2, 2)])
+def test_gemm_matches_rounded_fp64(ctx, tin, tout, akm, bkm):
+    M, N, K, batch = 96, 160, 128, 2
+    A = rand((batch, M, K), 1)
+    B = rand((batch, K, N), 2)
+    Ar, Br = ROUND[tin](A), ROUND[tin](B)
+    ref = Ar @ Br
+    Ast = A if akm else np.ascontiguousarray(A.transpose(0, 2, 1))
+    Bst = np.ascontiguousarray(B.transpose(0, 2, 1)) if bkm else B
+    C = ctx.gemm_test(Ast, Bst, tin, tout, akm, bkm, M, N, K, batch)
+    scale = np.abs(Ar).max() * np.abs(Br).max() * K
+    err = np.abs(C - ref).max() / scale
+    # accumulation: fp64 ~1e-16, fp32 ~ K*2^-24; output rounding for bf16 out
+    tol = {0: 1e-15, 1: 2e-6}.get(tin, 2e-6)
+    if tout == 2:
+        tol = 8e-3 / np.sqrt(K)
+    assert err < tol, (tin, tout, akm, bkm, err)
+
+
+@pytest.mark.parametrize("tile", range(1, 8))
+@pytest.mark.parametrize("tin", [0, 2])
+def test_gemm_every_tile(ctx, tin, tile):
+    M, N, K = 128, 128, 64
+    A = rand((1, M, K), 3)
+    B = rand((1, K, N), 4)
+    ref = ROUND[tin](A) @ ROUND[tin](B)
+    C = ctx.gemm_test(A, np.ascontiguousarray(B.transpose(0, 2, 1)), tin, 0, True, True, M, N, K, 1, tile)
+    err = np.abs(C - ref).max() / (K * 1.0)
+    assert err < (1e-15 if tin == 0 else 2e-6), (tile, err)
+
+
+def sym_env(Dw, D, seed):
+    """Environment whose every w-slice is symmetric (reference layout)."""
+    F = rand((Dw, D, D), seed)
+    F = 0.5 * (F + F.transpose(0, 2, 1))
+    # reference layout F[w + Dw*(i + D*j)]
+    return np.ascontiguousarray(F.transpose(2, 1, 0)).ravel()
+
+
+def sym_W(Dw1, Dw2, d, seed):
+    W = rand((Dw1, Dw2, d, d), seed)
+    W = 0.5 * (W + W.transpose(0, 1, 3, 2))
+    W[np.abs(W) < 0.3] = 0.0  # sparsity like a real MPO
+    return np.ascontiguousarray(W)
+
+
+HEFF_CASES = [(2, 1, 2, 1, 5), (2, 8, 8, 5, 5), (3, 27, 40, 5, 5), (2, 37, 19, 5, 5), (3, 64, 64, 5, 5),
+              (2, 16, 1, 5, 1), (4, 5, 9, 3, 4)]
+
+
+@pytest.mark.parametrize("d,D1,D2,Dw1,Dw2", HEFF_CASES)
+def test_heff_apply_fp64_matches_oracle(ctx, d, D1, D2, Dw1, Dw2):
+    W = sym_W(Dw1, Dw2, d, 10)
+    L = rand(Dw1 * D1 * D1, 11)
+    R = rand(Dw2 * D2 * D2, 12)
+    x = rand(d * D1 * D2, 13)
+    y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    y = ctx.heff_apply(x)
+    err = np.abs(y - y_ref).max() / np.abs(y_ref).max()
+    assert err < 1e-13, err
+    if d * D1 * D2 <= 400:
+        H = O.heff_dense(W, L, R, d, D1, D2)
+        assert np.abs(H @ x - y).max() / np.abs(y).max() < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2)])
+def test_heff_apply_reduced_precision(ctx, prec, tol):
+    d, D1, D2 = 3, 64, 96
+    W = sym_W(5, 5, d, 20)
+    L, R, x = rand(5 * D1 * D1, 21), rand(5 * D2 * D2, 22), rand(d * D1 * D2, 23)
+    y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    ctx.site_bind(W, L, R, d, D1, D2, prec)
+    y = ctx.heff_apply(x)
+    err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("d,D1,D2,Dw1,Dw2", HEFF_CASES)
+def test_env_updates_match_oracle(ctx, d, D1, D2, Dw1, Dw2):
+    W = sym_W(Dw1, Dw2, d, 30)
+    L = rand(Dw1 * D1 * D1, 31)
+    R = rand(Dw2 * D2 * D2, 32)
+    M = rand(d * D1 * D2, 33)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    Lo = ctx.env_update(0, M, Dw2 * D2 * D2)
+    Lr = O.env_left_ref(W, L, M, d, D1, D2)
+    assert np.abs(Lo - Lr).max() / np.abs(Lr).max() < 1e-13
+    Ro = ctx.env_update(1, M, Dw1 * D1 * D1)
+    Rr = O.env_right_ref(W, R, M, d, D1, D2)
+    assert np.abs(Ro - Rr).max() / np.abs(Rr).max() < 1e-13
+
+
+@pytest.mark.parametrize("d,D1,D2", [(2, 4, 6), (3, 9, 9), (2, 12, 10)])
+def test_lanczos_lowest_matches_dense(ctx, d, D1, D2):
+    W = sym_W(5, 5, d, 40)
+    L, R = sym_env(5, D1, 41), sym_env(5, D2, 42)
+    H = O.heff_dense(W, L, R, d, D1, D2)
+    assert np.abs(H - H.T).max() < 1e-13  # the Hermiticity check of LapackEigenSolver.C:71-77
+    w = np.linalg.eigvalsh(H)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    ev, x, nmv, rc = ctx.eig_lowest(rand(d * D1 * D2, 43), tol=1e-11, maxit=400)
+    assert rc == 0
+    assert abs(ev[0] - w[0]) < 1e-10 * max(1, abs(w[0]))
+    assert abs(np.linalg.norm(x) - 1) < 1e-12
+    assert np.linalg.norm(H @ x - ev[0] * x) < 1e-9
+
+
+@pytest.mark.parametrize("m,n,rank", [(40, 24, 24), (24, 40, 24), (64, 64, 64), (96, 30, 7), (5, 3, 3),
+                                      (1, 8, 1), (130, 100, 100)])
+def test_svd_matches_numpy(ctx, m, n, rank):
+    A = rand((m, rank), 50) @ rand((rank, n), 51)
+    U, s, Vt = ctx.svd(A)
+    k = min(m, n)
+    s_np = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(s - s_np).max() < 1e-12 * s_np[0]
+    assert np.abs(U @ np.diag(s) @ Vt - A).max() < 1e-12 * s_np[0]
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
+    assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12
+    assert np.all(np.diff(s) <= 1e-15 * s_np[0])

@@ -1,0 +1,79 @@
+"""End-to-end one-site sweep on the GPU vs the reference's golden energies
+and vs the CPU oracle run from the identical seeded state."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tensornetworks_amd as tn
+from tensornetworks_amd.models import xxz_mpo, identity_mpo
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+
+
+def run_fixed_sweeps(ctx, L, S, D, nsweeps, seed, opts=None, Dz=0.0):
+    Ws = xxz_mpo(L, S, Dz=Dz)
+    ch = tn.Chain(ctx, Ws, D)
+    ch.initialize(1, seed)
+    ch.normalize(1)
+    opts = opts or tn.SweepOpts.default(eig_tol=1e-12)
+    ch.load_caches(opts)
+    for _ in range(nsweeps):
+        ch.sweep(0, opts)
+        ch.sweep(1, opts)
+    return ch
+
+
+def test_L10_D8_ten_sweeps_matches_reference_value(ctx):
+    # PyTensorNetworks/test_Unittest1.py:22 : L=10 S=1/2 D=8 -> -0.473113399629092 per bond
+    ch = run_fixed_sweeps(ctx, 10, 0.5, 8, 10, 20261202)
+    E = ch.expectation()
+    assert abs(E / 9 - (-0.473113399629092)) < 1e-11
+    # variational: above exact diagonalisation (ExactDiagTests.C:265-270)
+    assert E > -4.258035207282882
+    assert ch.expectation(identity_mpo(10, 2)) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_same_seed_same_energy_as_oracle(ctx):
+    L, D, seed = 8, 6, 7
+    ch = run_fixed_sweeps(ctx, L, 0.5, D, 3, seed)
+    m = O.OracleMPS(L, 2, D, init=1, seed=seed)
+    m.normalize(1)
+    m.load_caches()
+    for _ in range(3):
+        m.sweep(0, 1e-12, 1)
+        m.sweep(1, 1e-12, 1)
+    assert abs(ch.expectation() - m.expectation(0)) < 1e-10
+    assert ch.dims() == m.dims()
+
+
+def test_normalisation_makes_right_canonical(ctx):
+    L, D = 9, 5
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), D)
+    ch.initialize(1, 3)
+    ch.normalize(1)
+    D1, D2 = ch.dims()
+    for i in range(1, L):
+        M = ch.site(i).reshape(D2[i], D1[i], 2)  # [i2][i1][m]
+        Bn = np.einsum("bam,bcm->ac", M, M)       # sum_m B B^T
+        assert np.abs(Bn - np.eye(D1[i])).max() < 1e-12
+    # bond dimensions min(D, d^k) from the right edge (MPSSite_Normalization.C:45-78)
+    assert D1[L - 1] == 2 and D1[L - 2] == 4
+
+
+@pytest.mark.parametrize("case", [c for c in GOLD["dmrg_per_bond"] if c["L"] <= 10 and c["S"] == 0.5])
+def test_variational_ground_state_schedule(ctx, case):
+    L, D, Dstart = case["L"], case["D"], case["Dstart"]
+    Ws = xxz_mpo(L, case["S"])
+    ch = tn.Chain(ctx, Ws, Dstart, Dcap=max(D, Dstart))
+    ch.initialize(1, 1234 + L)
+    sched = [tn.ScheduleLine(100, D, 1, eps_energy=1e-11, eps_lambda=1e-9, eps_eigen=1e-12)]
+    res = tn.FindVariationalGroundState(ch, sched, tn.SweepOpts.default(eig_tol=1e-12))
+    E = ch.expectation() / (L - 1)
+    assert abs(E - case["E"]) < max(case["tol"], 1e-10), (case, E)
+    if case["var"] is not None:
+        assert res.de2 < case["var"]

@@ -1,0 +1,301 @@
+"""Pin the CPU oracle before trusting it as the checker.
+
+Anchors: the reference's own golden values (tests/golden/reference_values.json,
+each with its file:line), closed forms (E0(L=2) = -S(S+1), Neel energy), and
+independent libraries (numpy eigh / svd) on random inputs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tensornetworks_amd.models import xxz_mpo, mpo_square, spin_ops
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_values.json")))
+
+
+def rand(shape, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, shape)
+
+
+def h12(d, Jxy=1.0, Jz=1.0):
+    Sp, Sm, Sz = O.spin_ops(d)
+    return 0.5 * Jxy * (np.kron(Sp, Sm) + np.kron(Sm, Sp)) + Jz * np.kron(Sz, Sz)
+
+
+@pytest.mark.parametrize("case", GOLD["h12_strings"])
+def test_h12_matches_reference_string(case):
+    d = int(round(2 * case["S"] + 1))
+    assert np.allclose(h12(d), np.array(case["rows"]), atol=1e-12)
+
+
+def test_spin_ops_match_product_models():
+    for d in (2, 3, 4, 6):
+        a = O.spin_ops(d)
+        b = spin_ops(d)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        Sp, Sm, Sz = a
+        # [S+, S-] = 2 Sz and S- = S+^T
+        assert np.allclose(Sp @ Sm - Sm @ Sp, 2 * Sz)
+        assert np.allclose(Sm, Sp.T)
+
+
+@pytest.mark.parametrize("case", GOLD["two_site_spectra"])
+def test_two_site_spectrum(case):
+    d = int(round(2 * case["S"] + 1))
+    assert np.allclose(np.linalg.eigvalsh(h12(d)), case["evs"], atol=1e-12)
+
+
+@pytest.mark.parametrize("case", GOLD["ed_totals"])
+def test_ed_totals(case):
+    d = int(round(2 * case["S"] + 1))
+    E = O.ed_ground(case["L"], d, tol=1e-13, maxit=400)
+    assert abs(E - case["E"]) < case["tol"]
+
+
+@pytest.mark.parametrize("case", GOLD["ed_per_bond"])
+def test_ed_per_bond(case):
+    d = int(round(2 * case["S"] + 1))
+    E = O.ed_ground(case["L"], d, tol=1e-13, maxit=400)
+    assert abs(E / (case["L"] - 1) - case["E"]) < case["tol"]
+
+
+def dense_from_mpo(Ws):
+    """Contract an MPO into its d^L x d^L matrix (site 0 slowest)."""
+    T = Ws[0][0]  # (Dw2, d, d)
+    T = np.transpose(T, (0, 1, 2))
+    cur = T  # [w, bra, ket]
+    for W in Ws[1:]:
+        cur = np.einsum("wab,wvmn->vambn", cur, W)
+        v, a, m, b, n = cur.shape
+        cur = cur.reshape(v, a * m, b * n)
+    return cur[0]
+
+
+@pytest.mark.parametrize("L,S,Jxy,Jz,hz,Dz", [(6, 0.5, 1.0, 1.0, 0.0, 0.0), (5, 0.5, 0.7, 1.6, 0.3, 0.0),
+                                              (4, 1.0, 1.0, 1.0, 0.0, 0.1), (4, 1.0, 1.2, 0.4, -0.2, 0.3)])
+def test_ed_uses_mpo_convention(L, S, Jxy, Jz, hz, Dz):
+    # hz once per site and Dz*Sz^2: ED == lowest eigenvalue of the contracted MPO
+    d = int(round(2 * S + 1))
+    H = dense_from_mpo(xxz_mpo(L, S, Jxy, Jz, hz, Dz))
+    assert np.allclose(H, H.T)
+    e_exact = np.linalg.eigvalsh(H)[0]
+    assert abs(O.ed_ground(L, d, Jxy, Jz, hz, Dz, 1e-13, 400) - e_exact) < 1e-11
+
+
+def test_product_mpo_matches_oracle_and_squares_H():
+    Ws = xxz_mpo(4, 0.5, hz=0.2)
+    W2 = mpo_square(Ws)
+    for W, P2 in zip(Ws, W2):
+        a, b, d, _ = W.shape
+        ref = np.zeros(a * a * b * b * d * d)
+        O.lib.orc_mpo_product(d, a, b, O.P(np.ascontiguousarray(W)), a, b, O.P(np.ascontiguousarray(W)), O.P(ref))
+        assert np.allclose(P2.ravel(), ref)
+    H = dense_from_mpo(Ws)
+    assert np.allclose(dense_from_mpo(W2), H @ H)
+
+
+def test_xxz_site_W_matches_oracle():
+    for form in (0, 1):
+        Wb = O.xxz_bulk_W(3, 1.0, 0.8, 0.1, 0.2, form)
+        Ws = xxz_mpo(5, 1.0, 1.0, 0.8, 0.1, 0.2, form)
+        for pos, i in ((0, 0), (1, 2), (2, 4)):
+            assert np.array_equal(O.site_W(3, Wb, form, pos), Ws[i])
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_upper_and_lower_forms_same_operator(form):
+    H0 = dense_from_mpo(xxz_mpo(5, 0.5, 1.0, 1.0, 0.3, 0.0, 0))
+    H1 = dense_from_mpo(xxz_mpo(5, 0.5, 1.0, 1.0, 0.3, 0.0, form))
+    assert np.allclose(H0, H1)
+
+
+@pytest.mark.parametrize("S2", [1, 2, 3, 4, 5])
+def test_neel_closed_form(S2):
+    # MPOTests.C:115-136: <H> = -S^2 (L-1) for the Neel state (Jxy=Jz=1, hz=0)
+    S, L = S2 / 2.0, 10
+    m = O.OracleMPS(L, S2 + 1, 3, init=2)
+    assert abs(m.expectation(0) - (-S * S * (L - 1))) < 1e-12
+    assert abs(m.expectation(2) - 1.0) < 1e-14
+
+
+def test_environment_closure_invariant():
+    # MPOTests.C:186-275: full left zipper == full right zipper == <H>
+    L, d = 6, 2
+    m = O.OracleMPS(L, d, 4, init=1, seed=5)
+    m.normalize(1)
+    Ws = xxz_mpo(L, 0.5)
+    D1, D2 = m.dims()
+    F = np.ones(1)
+    for i in range(L):
+        F = O.env_left_ref(Ws[i], F, m.site(i), d, D1[i], D2[i])
+    G = np.ones(1)
+    for i in reversed(range(L)):
+        G = O.env_right_ref(Ws[i], G, m.site(i), d, D1[i], D2[i])
+    assert abs(F[0] - G[0]) < 1e-12
+    assert abs(F[0] - m.expectation(0)) < 1e-12
+
+
+@pytest.mark.parametrize("d,D1,D2,Dw1,Dw2", [(2, 3, 4, 5, 5), (3, 5, 2, 5, 5), (2, 1, 4, 1, 5), (3, 6, 1, 5, 1)])
+def test_heff_forms_agree(d, D1, D2, Dw1, Dw2):
+    W = rand((Dw1, Dw2, d, d), 1)
+    L, R, x = rand(Dw1 * D1 * D1, 2), rand(Dw2 * D2 * D2, 3), rand(d * D1 * D2, 4)
+    H = O.heff_dense(W, L, R, d, D1, D2)
+    y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    assert np.allclose(H @ x, y_ref, atol=1e-13)
+    # GEMM-ordered path in the internal layout
+    Li = O.lenv_ref_to_int(L, Dw1, D1)
+    Ri = O.renv_ref_to_int(R, Dw2, D2)
+    y_int = O.heff_apply_int(W, Li, Ri, O.psi_ref_to_int(x, d, D1, D2), d, D1, D2)
+    assert np.allclose(O.psi_int_to_ref(y_int, d, D1, D2), y_ref, atol=1e-13)
+    # brute force of the GetHeff formula (MPSSite_Contractions.C:155-193)
+    Lr = L.reshape(D1, D1, Dw1)  # [j][i][w]
+    Rr = R.reshape(D2, D2, Dw2)
+    Hb = np.einsum("jiw,wvmn,lkv->ikmjln", Lr, W, Rr)  # (i1,i2,m ; j1,j2,n)
+    # reorder to psi index m + d*(i1 + D1*i2)
+    Hb = Hb.transpose(1, 0, 2, 4, 3, 5).reshape(D2 * D1 * d, D2 * D1 * d)
+    assert np.allclose(Hb, H, atol=1e-13)
+
+
+@pytest.mark.parametrize("d,D1,D2,Dw1,Dw2", [(2, 3, 4, 5, 5), (3, 5, 2, 5, 5), (2, 1, 4, 1, 5), (3, 6, 1, 5, 1)])
+def test_env_forms_agree(d, D1, D2, Dw1, Dw2):
+    W = rand((Dw1, Dw2, d, d), 5)
+    L, R, M = rand(Dw1 * D1 * D1, 6), rand(Dw2 * D2 * D2, 7), rand(d * D1 * D2, 8)
+    A = O.psi_ref_to_int(M, d, D1, D2)
+    Lo = O.env_left_ref(W, L, M, d, D1, D2)
+    Li = O.env_left_int(W, O.lenv_ref_to_int(L, Dw1, D1), A, d, D1, D2)
+    assert np.allclose(O.lenv_int_to_ref(Li, Dw2, D2), Lo, atol=1e-13)
+    Ro = O.env_right_ref(W, R, M, d, D1, D2)
+    Ri = O.env_right_int(W, O.renv_ref_to_int(R, Dw2, D2), A, d, D1, D2)
+    assert np.allclose(O.renv_int_to_ref(Ri, Dw1, D1), Ro, atol=1e-13)
+    # brute force of IterateLeft_F (MPSSite_Contractions.C:292-348)
+    Mr = M.reshape(D2, D1, d)  # [i2][i1][m]
+    Lr = L.reshape(D1, D1, Dw1)  # [j1][i1][w1]
+    F = np.einsum("bam,vwmn,jav,cjn->cbw", Mr, W, Lr, Mr)  # F(w2,i2=b,j2=c) stored [j2][i2][w2]
+    assert np.allclose(F.ravel(), Lo, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 40, 90])
+def test_syev_matches_numpy_and_eigentester_invariants(n):
+    A = rand((n, n), n)
+    A = A + A.T
+    w, Z = O.syev(A)
+    assert np.allclose(w, np.linalg.eigvalsh(A), atol=1e-12 * max(1, np.abs(w).max()))
+    eps = 1e-13  # LinearAlgebraTests.H:206-218: UtU=I within 2N eps, UtAU=diag within N eps
+    assert np.abs(Z.T @ Z - np.eye(n)).max() < 2 * n * eps
+    assert np.abs(Z.T @ A @ Z - np.diag(w)).max() < n * eps * max(1, np.abs(w).max())
+
+
+@pytest.mark.parametrize("m,n,rank", [(30, 20, 20), (20, 30, 20), (25, 25, 8), (12, 3, 3), (1, 6, 1), (40, 10, 0)])
+def test_svd_matches_numpy(m, n, rank):
+    A = rand((m, rank), 1) @ rand((rank, n), 2) if rank else np.zeros((m, n))
+    U, s, Vt = O.svd(A)
+    k = min(m, n)
+    assert np.allclose(s, np.linalg.svd(A, compute_uv=False), atol=1e-12)
+    assert np.allclose(U @ np.diag(s) @ Vt, A, atol=1e-12)
+    assert np.allclose(U.T @ U, np.eye(k), atol=1e-12)
+    assert np.allclose(Vt @ Vt.T, np.eye(k), atol=1e-12)
+
+
+def test_svcompress_closed_forms():
+    s = np.array([0.8, 0.5, 0.3, 0.1])
+    tot = (s ** 2).sum()
+    D, disc, r = O.svcompress(s, 2, 0.0)
+    assert D == 2 and abs(disc - 0.10) < 1e-15 and abs(r - tot / 0.89) < 1e-15
+    D, disc, r = O.svcompress(s, 0, 0.02)  # drop 0.1^2 = 0.01 only
+    assert D == 3 and abs(disc - 0.01) < 1e-15 and abs(r - tot / (tot - 0.01)) < 1e-14
+    D, disc, r = O.svcompress(s, 2, 0.02)  # eps says 3, Dmax caps to 2 (discarded keeps eps sum)
+    assert D == 2 and abs(disc - 0.01) < 1e-15
+    D, disc, r = O.svcompress(s, 10, 1e-6)  # nothing dropped
+    assert D == 4 and disc == 0.0 and r == 1.0
+
+
+def test_lanczos_lowest_matches_numpy():
+    n = 60
+    A = rand((n, n), 9)
+    A = A + A.T
+    w = np.linalg.eigvalsh(A)
+    H = A.copy()
+
+    @O.ctypes.CFUNCTYPE(None, O.ctypes.c_void_p, O._dp, O._dp)
+    def mv(_, x, y):
+        xv = np.ctypeslib.as_array(x, (n,))
+        yv = np.ctypeslib.as_array(y, (n,))
+        yv[:] = H @ xv
+
+    x0 = rand(n, 10)
+    e0, e1 = O.ctypes.c_double(), O.ctypes.c_double()
+    O.lib.orc_lanczos.argtypes = [O.ctypes.c_long, O.ctypes.c_void_p, O.ctypes.c_void_p, O._dp, O.ctypes.c_double,
+                                  O.ctypes.c_int, O._dp, O._dp]
+    O.lib.orc_lanczos(n, O.ctypes.cast(mv, O.ctypes.c_void_p), None, O.P(x0), 1e-12, 200,
+                      O.ctypes.byref(e0), O.ctypes.byref(e1))
+    assert abs(e0.value - w[0]) < 1e-11
+    assert abs(e1.value - w[1]) < 1e-9
+    assert np.linalg.norm(H @ x0 - e0.value * x0) < 1e-10
+
+
+def oracle_gs(L, S, Dstart, D, mode=1, seed=4321):
+    d = int(round(2 * S + 1))
+    m = O.OracleMPS(L, d, Dstart, init=1, seed=seed)
+    de2 = m.find_ground_state([(100, D, 1, 1e-12, 1e-12, 1e-12)], mode=mode)
+    return m, de2
+
+
+@pytest.mark.parametrize("case", [c for c in GOLD["dmrg_per_bond"] if c["L"] <= 10 and c["S"] < 2])
+def test_oracle_dmrg_matches_reference_values(case):
+    L = case["L"]
+    m, de2 = oracle_gs(L, case["S"], case["Dstart"], case["D"])
+    E = m.expectation(0) / (L - 1)
+    assert abs(E - case["E"]) < case["tol"]
+    if case["var"] is not None:
+        assert de2 < case["var"]
+
+
+def test_oracle_reference_faithful_mode_L10():
+    # dense Heff + full eigensolve, exactly the reference's Refine
+    m, de2 = oracle_gs(10, 0.5, 8, 8, mode=0, seed=11)
+    assert abs(m.expectation(0) / 9 - (-0.473113399629092)) < 1e-11
+    assert de2 < 5e-5
+    assert m.norm_status() == "B" * 10
+
+
+def test_L2_exact_variance():
+    # VariationalGroundStateTests.C:86-109: L=2 D=2 -> -0.75 and <H^2> = <H>^2
+    m, de2 = oracle_gs(2, 0.5, 2, 2)
+    assert abs(m.expectation(0) + 0.75) < 1e-12
+    assert abs(m.expectation(1) - m.expectation(0) ** 2) < 1e-14
+
+
+def test_one_site_dmrg_is_variational_above_ed():
+    m, _ = oracle_gs(10, 0.5, 8, 8)
+    E = m.expectation(0)
+    assert E > -4.258035207282882
+    gap = E / 9 - (-0.473115023031432)
+    assert 1e-7 < gap < 3e-6
+
+
+def test_two_site_oracle_reaches_one_site_value():
+    L = 10
+    m = O.OracleMPS(L, 2, 8, init=1, seed=3)
+    m.normalize(1)
+    m.load_caches()
+    for _ in range(8):
+        m.sweep2(0, 8, 0.0, 1e-12)
+        m.sweep2(1, 8, 0.0, 1e-12)
+    # truncation after each two-site step leaves the fixed point within the
+    # discarded weight of the one-site value
+    assert abs(m.expectation(0) / 9 - (-0.473113399629092)) < 1e-7
+
+
+def test_increase_bond_dimensions_shapes():
+    # MPSImp.C:123-147: min(D, d^k) from both edges
+    m = O.OracleMPS(10, 2, 2, init=1, seed=1)
+    m.normalize(1)
+    m.load_caches()
+    m.find_ground_state([(2, 6, 2, 1e-12, 1e-12, 1e-12)], mode=1)
+    D1, D2 = m.dims()
+    assert D1[0] == 1 and D2[-1] == 1
+    assert max(D2) <= 6 and D2[0] == 2 and D1[-1] == 2
