@@ -86,8 +86,9 @@ int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* dis
 /* Per-site MPO: dw1[i], dw2[i] and W blocks concatenated in site order.
  * Sites start as in MPSImp::InitSitesAndBonds (TensorNetworksImp/MPS/MPSImp.C:69-90):
  * site 0 is 1 x D, bulk D x D, site L-1 is D x 1; bonds hold 1/D.
- * Dcap bounds every bond dimension the chain will ever hold (device memory
- * is sized from it). */
+ * Device memory is sized for bond dimensions up to Dcap; growing beyond it
+ * (tn_chain_increase_bond_dims) reallocates and invalidates the environment
+ * caches, which tn_chain_load_caches rebuilds. */
 int tn_chain_create(tn_ctx* ctx, int L, int d, const int* dw1, const int* dw2,
                     const double* wcat, int D, int Dcap, tn_chain** chain);
 int tn_chain_destroy(tn_chain* chain);
@@ -120,6 +121,7 @@ typedef struct tn_sweep_stats {
     double heff_flops;         /* algorithmic flops of all Heff applies */
     double heff_ms;            /* device time of all Heff applies (HIP events) */
     double heff_gemm_ms;       /* device time of the two GEMM stages only */
+    double heff_gemm_flops;    /* algorithmic flops of the two GEMM stages */
     double env_flops, env_ms;
     double svd_ms, lanczos_ms, wall_ms;
     double energy;             /* last site eigenvalue */
@@ -154,6 +156,14 @@ int tn_chain_sweep2(tn_chain* chain, int dir, int Dmax, double eps_sv,
                     const tn_sweep_opts* opts, tn_sweep_stats* st, double* max_trunc);
 
 /* ---- diagnostics (no counterpart in the reference) ---------------------- */
+/* Timing of `reps` Heff applications at one site of the chain in the given
+ * precision (HIP events on the library stream): average device time of the
+ * first GEMM stage, the MPO stage and the second GEMM stage, in ms, and the
+ * algorithmic flops of one GEMM stage launch (2 d Dw D1 D2 max(D1,D2) form).
+ * The site tensor is left unchanged. */
+int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, double* s1_ms,
+                        double* mix_ms, double* s3_ms, double* s1_flops, double* s3_flops);
+
 /* One batched GEMM through the device kernel, for numerics tests of the
  * matrix-core path: C[b] = A[b] B[b] with operands rounded to tin
  * (enum tn_precision values 0 fp64, 1 fp32, 2 bf16) and the result stored as

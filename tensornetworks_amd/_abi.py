@@ -50,6 +50,7 @@ class SweepStats(ctypes.Structure):
         ("heff_flops", ctypes.c_double),
         ("heff_ms", ctypes.c_double),
         ("heff_gemm_ms", ctypes.c_double),
+        ("heff_gemm_flops", ctypes.c_double),
         ("env_flops", ctypes.c_double),
         ("env_ms", ctypes.c_double),
         ("svd_ms", ctypes.c_double),
@@ -83,6 +84,7 @@ def _load():
         "tn_svd": (i, [v, i, i, _dp, _dp, _dp, _dp]),
         "tn_svcompress_rank": (i, [i, _dp, i, d, _dp, _dp]),
         "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i]),
+        "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
         "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
         "tn_chain_destroy": (i, [v]),
         "tn_chain_initialize": (i, [v, i, u64]),
@@ -292,6 +294,13 @@ class Chain:
         if n < 0:
             raise TnError(n, "tn_chain_bond_sv")
         return buf[:n].copy()
+
+    def heff_bench(self, site, precision, reps=10):
+        """Average device ms of (GEMM 1, MPO stage, GEMM 2) and flops per GEMM launch."""
+        vals = [ctypes.c_double() for _ in range(5)]
+        self.ctx.check(lib.tn_chain_heff_bench(self.h, site, precision, reps, *[ctypes.byref(x) for x in vals]),
+                       "tn_chain_heff_bench")
+        return [x.value for x in vals]
 
     def expectation(self, Ws=None):
         dw1, dw2, wcat = self._mpo if Ws is None else _mpo_arrays(Ws)

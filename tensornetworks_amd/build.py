@@ -85,6 +85,13 @@ def build(verbose=True):
         os.replace(tmp, api)
         if verbose:
             print("built", api)
+        test_src = os.path.join(ROOT, "tests", "cpp", "test_variational.cpp")
+        if os.path.exists(test_src):
+            exe = os.path.join(LIBDIR, "test_variational")
+            subprocess.run(["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-o", exe, test_src, "-L", LIBDIR,
+                            "-ltensornetworks", "-ltn_hip", "-Wl,-rpath,$ORIGIN"], check=True)
+            if verbose:
+                print("built", exe)
     return lib
 
 

@@ -169,6 +169,11 @@ void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y)
     if (timed) {
         heff_timing_read(c, &hc->st->heff_gemm_ms, &hc->st->heff_ms);
         hc->st->heff_flops += hc->op.flops_alg(hc->D1, hc->D2);
+        {
+            const HeffOp& o = hc->op;
+            const double D1 = hc->D1, D2 = hc->D2;
+            hc->st->heff_gemm_flops += 2.0 * D1 * o.p * o.Dw2 * D2 * D2 + 2.0 * D1 * o.p * D2 * o.Dw1 * D1;
+        }
         hc->st->n_matvec++;
         if (hc->op.precision == TN_FP64) hc->st->n_matvec_fp64++;
     }
@@ -380,6 +385,47 @@ void two_site_W(int d, int Dw1, int Dwm, int Dw3, const double* Wa, const double
                                 W12[(((size_t)w * Dw3 + w2) * p + (m1 * d + m2)) * p + (n1 * d + n2)] +=
                                     a * Wb[((wp * Dw3 + w2) * d + m2) * d + n2];
                     }
+}
+
+// grow the per-site device buffers so every bond dimension up to Dcap fits;
+// site data are kept, environments must be reloaded afterwards
+void chain_reserve(tn_chain* ch, int Dcap)
+{
+    if (Dcap <= ch->Dcap) return;
+    tn_ctx* c = ch->ctx;
+    sync(c);
+    const int Dcapp = pad32(Dcap);
+    const size_t site_cap = (size_t)Dcapp * ch->d * Dcapp;
+    const size_t env_cap = (size_t)Dcapp * ch->Dwmax * Dcapp;
+    DBuf nsites;
+    nsites.ensure(site_cap * 8 * ch->L);
+    TN_HIP(fill_zero(nsites.p, (long)nsites.bytes, c->stream));
+    for (int i = 0; i < ch->L; i++) {
+        const size_t n = (size_t)pad32(ch->D1[i]) * ch->d * pad32(ch->D2[i]);
+        TN_HIP(hipMemcpyAsync(nsites.as<double>() + (size_t)i * site_cap, ch->A[i], n * 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    }
+    sync(c);
+    ch->sites_buf.release();
+    ch->sites_buf = nsites;
+    ch->env_buf.release();
+    ch->spare_buf.release();
+    ch->lo_buf.release();
+    ch->lo_prec = -1;
+    ch->Dcap = Dcap;
+    ch->Dcapp = Dcapp;
+    ch->site_cap = site_cap;
+    ch->env_cap = env_cap;
+    ch->spare_buf.ensure(site_cap * 8);
+    ch->env_buf.ensure(env_cap * 8 * 2 * ch->L);
+    TN_HIP(fill_zero(ch->env_buf.p, (long)ch->env_buf.bytes, c->stream));
+    for (int i = 0; i < ch->L; i++) {
+        ch->A[i] = ch->sites_buf.as<double>() + (size_t)i * site_cap;
+        ch->Lenv[i] = ch->env_buf.as<double>() + (size_t)(2 * i) * env_cap;
+        ch->Renv[i] = ch->env_buf.as<double>() + (size_t)(2 * i + 1) * env_cap;
+    }
+    ch->spare = ch->spare_buf.as<double>();
+    sync(c);
 }
 
 }  // namespace
@@ -915,7 +961,7 @@ int tn_chain_increase_bond_dims(tn_chain* ch, int D)
     if (!ch || D < 1) return TN_ERR_ARG;
     tn_ctx* c = ch->ctx;
     ABI_BEGIN
-    TN_REQUIRE(D <= ch->Dcap, TN_ERR_ARG, "D exceeds Dcap");
+    chain_reserve(ch, D);
     // MPSImp.C:123-147
     const int L = ch->L, d = ch->d;
     long Dideal = 1;
@@ -1021,7 +1067,11 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
     tn_sweep_opts o;
     tn_sweep_opts_default(&o);
     if (opts) o = *opts;
-    TN_REQUIRE(Dmax <= ch->Dcap, TN_ERR_ARG, "Dmax exceeds Dcap");
+    TN_REQUIRE(Dmax >= 1, TN_ERR_ARG, "Dmax >= 1");
+    if (Dmax > ch->Dcap) {
+        chain_reserve(ch, Dmax);
+        TN_REQUIRE(false, TN_ERR_STATE, "tn_chain_sweep2: Dmax grew the chain; reload caches first");
+    }
     hipStream_t s = c->stream;
     const int d = ch->d, L = ch->L, p2 = d * d;
     double mtr = 0.0;
@@ -1128,6 +1178,37 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
     sync(c);
     if (st) st->wall_ms += now_ms() - t0w;
     if (max_trunc) *max_trunc = mtr;
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, double* s1_ms, double* mix_ms,
+                        double* s3_ms, double* s1_flops, double* s3_flops)
+{
+    if (!ch || site < 0 || site >= ch->L || reps < 1) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    if (precision != TN_FP64) ensure_lo(ch, precision);
+    HeffOp op = make_heff(ch, site, precision, 0);
+    const long N = op.n();
+    c->b_y.ensure((size_t)N * 8);
+    double a = 0, b = 0, e = 0;
+    heff_apply(c, op, ch->A[site], c->b_y.as<double>(), true);  // warm
+    sync(c);
+    for (int r = 0; r < reps; r++) {
+        heff_apply(c, op, ch->A[site], c->b_y.as<double>(), true);
+        double x1, x2, x3;
+        heff_stage_times(c, &x1, &x2, &x3);
+        a += x1;
+        b += x2;
+        e += x3;
+    }
+    const double D1 = ch->D1[site], D2 = ch->D2[site];
+    if (s1_ms) *s1_ms = a / reps;
+    if (mix_ms) *mix_ms = b / reps;
+    if (s3_ms) *s3_ms = e / reps;
+    if (s1_flops) *s1_flops = 2.0 * D1 * op.p * op.Dw2 * D2 * D2;
+    if (s3_flops) *s3_flops = 2.0 * D1 * op.p * D2 * op.Dw1 * D1;
     return TN_OK;
     ABI_END(c)
 }

@@ -101,6 +101,8 @@ struct HeffOp {
 // timing != nullptr (events recorded; read after the next sync).
 void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed);
 void heff_timing_read(tn_ctx* c, double* gemm_ms, double* total_ms);
+// per-stage times of the last timed apply (ms): s1, mix, s3
+void heff_stage_times(tn_ctx* c, double* s1, double* mix, double* s3);
 
 // --- environment updates ---------------------------------------------------
 // left : Lout[D2p][Dw2][D2p]  from Lin[D1p][Dw1][D1p] and A[D1p][p][D2p]

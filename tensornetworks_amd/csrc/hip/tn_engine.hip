@@ -216,6 +216,18 @@ void heff_timing_read(tn_ctx* c, double* gemm_ms, double* total_ms)
     if (total_ms) *total_ms += (double)t01 + t12 + t23 + t34;
 }
 
+void heff_stage_times(tn_ctx* c, double* s1, double* mix, double* s3)
+{
+    float t12 = 0, t23 = 0, t34 = 0;
+    TN_HIP(hipEventSynchronize(c->ev[4]));
+    TN_HIP(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]));
+    TN_HIP(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]));
+    TN_HIP(hipEventElapsedTime(&t34, c->ev[3], c->ev[4]));
+    *s1 = t12;
+    *mix = t23;
+    *s3 = t34;
+}
+
 void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n)
 {
     if (prec == TN_FP64) return;
