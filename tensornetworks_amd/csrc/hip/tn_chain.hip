@@ -272,6 +272,18 @@ void rescale_site(tn_chain* ch, int i)
     TN_HIP(scale_by_inv(ch->A[i], h, n, c->stream));
 }
 
+// The neighbour receives s/|s| rather than s: the overall norm of the state
+// is irrelevant to the sweep (the site eigenproblem and the edge rescale
+// normalise it), and forwarding raw s multiplies the tensors of a random
+// high-D start by ~0.02 per site, which underflows FP64 along a long chain.
+void normalised_copy(const std::vector<double>& s, std::vector<double>& out)
+{
+    double nr = 0;
+    for (double v : s) nr += v * v;
+    nr = std::sqrt(nr);
+    for (size_t k = 0; k < s.size(); k++) out[k] = nr > 0 ? s[k] / nr : 0.0;
+}
+
 // MPSSite_Normalization.C:45-78 SVDNormalize(lr, nullptr) incl. HandleEdges :20-43
 void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
 {
@@ -303,9 +315,9 @@ void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         // neighbour: M_{i+1} <- diag(s) Vt M_{i+1}
         const int j = i + 1, D3p = pad32(ch->D2[j]);
         std::vector<double> sv_pad(kp, 0.0);
-        std::copy(sv.begin(), sv.end(), sv_pad.begin());
+        normalised_copy(sv, sv_pad);
         c->pack.ensure(kp * 8 + 64);
-        TN_HIP(hipMemcpyAsync(c->pack.p, sv_pad.data(), kp * 8, hipMemcpyHostToDevice, s));
+        h2d(c, c->pack.p, sv_pad.data(), kp * 8);
         TN_HIP(scale_rows(c->svd_Vt.as<double>(), D2p, kp, D2p, c->pack.as<double>(), s));
         GemmDesc g{};
         g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
@@ -332,9 +344,9 @@ void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         // neighbour: M_{i-1} <- M_{i-1} U diag(s)
         const int j = i - 1, D0p = pad32(ch->D1[j]);
         std::vector<double> sv_pad(kp, 0.0);
-        std::copy(sv.begin(), sv.end(), sv_pad.begin());
+        normalised_copy(sv, sv_pad);
         c->pack.ensure(kp * 8 + 64);
-        TN_HIP(hipMemcpyAsync(c->pack.p, sv_pad.data(), kp * 8, hipMemcpyHostToDevice, s));
+        h2d(c, c->pack.p, sv_pad.data(), kp * 8);
         TN_HIP(scale_cols(c->svd_U.as<double>(), kp, D1p, kp, c->pack.as<double>(), s));
         GemmDesc g{};
         g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
@@ -1151,7 +1163,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         std::vector<double> scl(std::max(Dkp, 32), 0.0);
         for (int t = 0; t < Dk; t++) scl[t] = sk[t] / nr;
         c->pack.ensure((size_t)scl.size() * 8 + 64);
-        TN_HIP(hipMemcpyAsync(c->pack.p, scl.data(), scl.size() * 8, hipMemcpyHostToDevice, s));
+        h2d(c, c->pack.p, scl.data(), scl.size() * 8);
         if (dir == 0) {
             // A_i <- U[:, :Dk] ; A_j <- diag(s) Vt[:Dk]
             TN_HIP(copy2(c->svd_U.as<double>(), kp, 1, rr, Dk, ch->A[i], Dkp, 1, D1p * d, Dkp, s));

@@ -426,6 +426,15 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     if (tall) TN_HIP(copy2(A, cs, rs, q, p, X, ldx, 1, qp, p, s));
     else TN_HIP(copy2(A, rs, cs, q, p, X, ldx, 1, qp, p, s));
     TN_HIP(set_identity(V, qp, qp, s));
+    // scale invariance: X /= max|X| (Gram entries of tiny matrices would underflow)
+    c->svd_s.ensure((size_t)(qp + 64) * 8 * 2 + 1024 * 8);
+    double* amax_d = c->svd_s.as<double>() + 2 * qp + 128;
+    c->part.ensure(1024 * 8);
+    TN_HIP(absmax(X, (long)ldx * qp, amax_d, c->part.as<double>(), 1024, s));
+    double amax = 0.0;
+    d2h(c, &amax, amax_d, 8);
+    if (!std::isfinite(amax)) throw TnError{TN_ERR_STATE, "svd: non-finite matrix entries"};
+    if (amax > 0.0) TN_HIP(scale_by_inv(X, amax_d, (long)ldx * qp, s));
     // tournament pairs for nb = qp/16 column blocks
     const int nb = qp / 16;
     std::vector<int> pairs((size_t)(nb - 1) * nb);
@@ -436,8 +445,7 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
             pairs[(size_t)rd * nb + 2 * i + 1] = elem(nb - 1 - i);
         }
     c->svd_pairs.ensure(pairs.size() * sizeof(int) + 64);
-    TN_HIP(hipMemcpyAsync(c->svd_pairs.p, pairs.data(), pairs.size() * sizeof(int),
-                          hipMemcpyHostToDevice, s));
+    h2d(c, c->svd_pairs.p, pairs.data(), pairs.size() * sizeof(int));
     c->svd_flag.ensure(256);
     int* flag = c->svd_flag.as<int>();
     const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)p));
@@ -451,7 +459,6 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
         sync(c);
     }
     // singular values = column norms; sort descending
-    c->svd_s.ensure((size_t)(qp + 64) * 8 * 2);
     double* sd = c->svd_s.as<double>();
     TN_HIP(col_norms(X, ldx, p, q, sd, s));
     std::vector<double> sv(q);
@@ -471,7 +478,7 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     }
     // normalise the good columns of X
     double* dinv = sd + qp + 32;
-    TN_HIP(hipMemcpyAsync(dinv, inv.data(), qp * 8, hipMemcpyHostToDevice, s));
+    h2d(c, dinv, inv.data(), qp * 8);
     TN_HIP(scale_rows(X, ldx, q, p, dinv, s));
     // orthonormal completion of (exactly) zero columns, in the sorted order
     for (int jj = 0; jj < q; jj++) {
@@ -481,21 +488,19 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
         double* xj = X + (long)j * ldx;
         long pcap = mdot_part_size(p, q);
         c->part.ensure(pcap * 8);
-        c->lanczos_small.ensure(sizeof(double) * (4 * 512 + 64));
-        double* h = c->lanczos_small.as<double>();
-        c->pack.ensure((size_t)q * 8 + 64);
-        double* hc = c->pack.as<double>();
+        c->pack.ensure((size_t)2 * q * 8 + 256);
+        double* h = c->pack.as<double>();
+        double* hc = h + q + 16;
         for (int tries = 0; tries < p && !good[j]; tries++) {
             TN_HIP(set_unit_col(xj, p, (jj + tries) % p, s));
             for (int pass = 0; pass < 2; pass++) {
                 std::vector<double> coef(q);
-                if (q > 512) throw TnError{TN_ERR_ARG, "svd completion: q > 512"};
                 TN_HIP(mdot(X, ldx, q, xj, p, h, c->part.as<double>(), (int)pcap, s));
                 TN_HIP(hipMemcpyAsync(coef.data(), h, q * 8, hipMemcpyDeviceToHost, s));
                 sync(c);
                 for (int t = 0; t < q; t++)
                     if (!good[t] || t == j) coef[t] = 0.0;
-                TN_HIP(hipMemcpyAsync(hc, coef.data(), q * 8, hipMemcpyHostToDevice, s));
+                h2d(c, hc, coef.data(), q * 8);
                 TN_HIP(gemv_acc(X, ldx, q, hc, -1.0, 1.0, xj, p, s));
             }
             double nr2 = 0;
@@ -505,7 +510,7 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
             if (nr2 > 0.25) {
                 double f = 1.0 / std::sqrt(nr2);
                 std::vector<double> one(1, f);
-                TN_HIP(hipMemcpyAsync(hc, one.data(), 8, hipMemcpyHostToDevice, s));
+                h2d(c, hc, one.data(), 8);
                 TN_HIP(scale_rows(xj, p, 1, p, hc, s));
                 sync(c);
                 good[j] = 1;
@@ -520,8 +525,8 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     dperm = c->svd_perm.as<int>();
     double* ones = (double*)(dperm + ((q + 63) / 64) * 64);
     std::vector<double> onev(k, 1.0);
-    TN_HIP(hipMemcpyAsync(dperm, perm.data(), k * sizeof(int), hipMemcpyHostToDevice, s));
-    TN_HIP(hipMemcpyAsync(ones, onev.data(), k * 8, hipMemcpyHostToDevice, s));
+    h2d(c, dperm, perm.data(), k * sizeof(int));
+    h2d(c, ones, onev.data(), k * 8);
     if (tall) {
         // U (r x k) from X, Vt (k x c) from V
         TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, U, ldu, 1, Vt, ldvt, 1, s));
@@ -530,7 +535,7 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
         TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, Vt, 1, ldvt, U, 1, ldu, s));
     }
     s_out.resize(k);
-    for (int j = 0; j < k; j++) s_out[j] = sv[perm[j]];
+    for (int j = 0; j < k; j++) s_out[j] = sv[perm[j]] * (amax > 0.0 ? amax : 1.0);
     sync(c);
     return k;
 }

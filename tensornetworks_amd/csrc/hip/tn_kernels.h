@@ -69,6 +69,8 @@ long mdot_part_size(long n, int k);
 // w[n] = beta_w * w[n] + sum_i coef[i] * V[i*ldv + n]
 hipError_t gemv_acc(const double* V, long ldv, int k, const double* coef, double coef_scale,
                     double beta_w, double* w, long n, hipStream_t s);
+// out[0] = max_i |x[i]|  (part: >= 1024 doubles of scratch)
+hipError_t absmax(const double* x, long n, double* out, double* part, int part_cap, hipStream_t s);
 // v[n] *= 1/den[0] (den on device)
 hipError_t scale_by_inv(double* v, const double* den, long n, hipStream_t s);
 hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStream_t s);

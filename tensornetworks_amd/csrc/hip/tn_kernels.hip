@@ -563,11 +563,13 @@ hipError_t mdot(const double* V, long ldv, int k, const double* w, long n, doubl
                 double* part, int part_cap, hipStream_t s)
 {
     if (k <= 0) return hipSuccess;
-    if (k > MD_MAXK) return hipErrorInvalidValue;
     long G = (n + MD_BLOCK * MD_EPT - 1) / (MD_BLOCK * MD_EPT);
-    if (G * k > part_cap) return hipErrorInvalidValue;
-    mdot1_k<<<G, MD_BLOCK, 0, s>>>(V, ldv, k, w, n, part);
-    mdot2_k<<<k, 256, 0, s>>>(part, (int)G, out);
+    for (int i0 = 0; i0 < k; i0 += MD_MAXK) {
+        const int kc = (k - i0) < MD_MAXK ? (k - i0) : MD_MAXK;
+        if (G * kc > part_cap) return hipErrorInvalidValue;
+        mdot1_k<<<G, MD_BLOCK, 0, s>>>(V + (long)i0 * ldv, ldv, kc, w, n, part);
+        mdot2_k<<<kc, 256, 0, s>>>(part, (int)G, out + i0);
+    }
     return hipGetLastError();
 }
 
@@ -588,10 +590,54 @@ __global__ void __launch_bounds__(256) gemv_acc_k(const double* __restrict__ V, 
 hipError_t gemv_acc(const double* V, long ldv, int k, const double* coef, double coef_scale,
                     double beta_w, double* w, long n, hipStream_t s)
 {
-    if (k > MD_MAXK) return hipErrorInvalidValue;
     long blocks = (n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
-    gemv_acc_k<<<blocks, 256, 0, s>>>(V, ldv, k, coef, coef_scale, beta_w, w, n);
+    if (k <= 0) {
+        if (beta_w != 1.0) gemv_acc_k<<<blocks, 256, 0, s>>>(V, ldv, 0, coef, coef_scale, beta_w, w, n);
+        return hipGetLastError();
+    }
+    for (int i0 = 0; i0 < k; i0 += MD_MAXK) {
+        const int kc = (k - i0) < MD_MAXK ? (k - i0) : MD_MAXK;
+        gemv_acc_k<<<blocks, 256, 0, s>>>(V + (long)i0 * ldv, ldv, kc, coef + i0, coef_scale,
+                                          i0 == 0 ? beta_w : 1.0, w, n);
+    }
+    return hipGetLastError();
+}
+
+// max |x| over n entries: stage 1 per block, stage 2 one block; out[0] = max
+__global__ void __launch_bounds__(256) absmax1_k(const double* __restrict__ x, long n,
+                                                 double* __restrict__ part)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) m = fmax(m, fabs(x[i]));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+__global__ void __launch_bounds__(256) absmax2_k(const double* __restrict__ part, int G,
+                                                 double* __restrict__ out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < G; i += 256) m = fmax(m, part[i]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+hipError_t absmax(const double* x, long n, double* out, double* part, int part_cap, hipStream_t s)
+{
+    if (n <= 0) return hipErrorInvalidValue;
+    long G = (n + 255) / 256;
+    if (G > 1024) G = 1024;
+    if (G > part_cap) G = part_cap;
+    if (G < 1) return hipErrorInvalidValue;
+    absmax1_k<<<G, 256, 0, s>>>(x, n, part);
+    absmax2_k<<<1, 256, 0, s>>>(part, (int)G, out);
     return hipGetLastError();
 }
 
