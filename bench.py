@@ -35,7 +35,9 @@ def parse():
     p.add_argument("--Dz", type=float, default=0.1)
     p.add_argument("--precision", default="bf16", choices=["bf16", "fp32", "fp64"])
     p.add_argument("--eig-tol", type=float, default=1e-8)
-    p.add_argument("--max-krylov", type=int, default=32)
+    p.add_argument("--max-krylov", type=int, default=24)
+    p.add_argument("--max-restarts", type=int, default=0)
+    p.add_argument("--gauge", default="qr", choices=["qr", "svd"])
     p.add_argument("--no-fp64-compare", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=20261202)
@@ -43,7 +45,7 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(D, d, reps=2):
+def cpu_baseline(D, d, reps=4):
     """The oracle's GEMM-ordered FP64 Heff apply (OpenMP) on the host cores:
     a bounded sample of `reps` applies at the workload's bulk shape."""
     import numpy as np
@@ -84,11 +86,14 @@ def main():
     ch = tn.Chain(ctx, Ws, a.D)
     ch.initialize(1, a.seed)
     t0 = time.perf_counter()
-    ch.normalize(1)
-    o_mixed = tn.SweepOpts.default(eig_tol=a.eig_tol, max_krylov=a.max_krylov, precision=prec,
-                                   env_precision=prec, refine=0)
-    o_fp64 = tn.SweepOpts.default(eig_tol=a.eig_tol, max_krylov=a.max_krylov, precision=tn.TN_FP64,
-                                  env_precision=tn.TN_FP64, refine=0)
+    gauge = 1 if a.gauge == "qr" else 0
+    if gauge:
+        ch.normalize_qr(1)
+    else:
+        ch.normalize(1)
+    common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, refine=0, gauge=gauge)
+    o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, **common)
+    o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, **common)
     ch.load_caches(o_mixed)
     setup_s = time.perf_counter() - t0
     print(f"# setup {setup_s:.1f}s", file=sys.stderr, flush=True)
@@ -128,7 +133,8 @@ def main():
         res_fp64 = {"sweep_s": wall64 / a.steps, "E_per_site": E64 / a.L,
                     "heff_tflops": st64.heff_flops / max(st64.heff_ms, 1e-12) / 1e9,
                     "n_matvec": st64.n_matvec, "dE_per_site": abs(E_mixed - E64) / a.L,
-                    "svd_ms": st64.svd_ms, "env_ms": st64.env_ms, "lanczos_ms": st64.lanczos_ms}
+                    "gauge_ms": st64.svd_ms, "env_ms": st64.env_ms, "lanczos_ms": st64.lanczos_ms,
+                    "n_qr_fallback": st64.n_qr_fallback}
         print(f"# fp64 {wall64:.2f}s E/L={E64 / a.L:.12f} |dE|/L={abs(E_mixed - E64) / a.L:.3e}",
               file=sys.stderr, flush=True)
     value = st.heff_flops / max(st.heff_ms, 1e-12) / 1e9
@@ -148,12 +154,14 @@ def main():
         "dtype": a.precision,
         "data": "synthetic (seeded random initial MPS)",
         "config": {"workload": f"S={a.S:g} Heisenberg + Dz={a.Dz:g} Sz^2 open chain, L={a.L}, d={d}, Dw=5, D={a.D}, "
-                               f"one-site sweep, contractions {a.precision}, FP64 Krylov/Ritz/SVD/energy",
-                   "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov},
+                               f"one-site sweep, contractions {a.precision}, FP64 Krylov/Ritz/gauge/energy",
+                   "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov,
+                   "max_restarts": a.max_restarts, "gauge": a.gauge},
         "sweep_s": wall / a.steps,
         "E_per_site": E_mixed / a.L,
         "breakdown_ms": {"heff": st.heff_ms, "heff_gemm": st.heff_gemm_ms, "lanczos": st.lanczos_ms,
-                         "env": st.env_ms, "svd": st.svd_ms, "wall": st.wall_ms},
+                         "env": st.env_ms, "gauge": st.svd_ms, "wall": st.wall_ms},
+        "n_qr": st.n_qr, "n_qr_fallback": st.n_qr_fallback,
         "n_matvec": st.n_matvec,
         "fp64": res_fp64,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,

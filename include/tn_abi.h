@@ -111,6 +111,11 @@ typedef struct tn_sweep_opts {
     int refine;          /* FP64 refinement after reduced-precision Lanczos */
     double refine_tol;
     int gemm_tile;       /* 0 = automatic tile choice */
+    int gauge;           /* 0 = SVD gauge moves (MPSSite::SVDNormalize, bond singular
+                            values kept); 1 = QR gauge moves (MPSSite::NormalizeQR,
+                            MPSSite_Normalization.C:80-107: bond values cleared as in
+                            Bond::ClearSingularValues), by CholeskyQR2 with fallback
+                            to the SVD when it is not accurate */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {
@@ -125,6 +130,8 @@ typedef struct tn_sweep_stats {
     double env_flops, env_ms;
     double svd_ms, lanczos_ms, wall_ms;
     double energy;             /* last site eigenvalue */
+    int64_t n_qr;              /* QR gauge moves */
+    int64_t n_qr_fallback;     /* QR gauge moves that fell back to the SVD */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);
@@ -132,6 +139,9 @@ void tn_sweep_opts_default(tn_sweep_opts* o);
  * (TensorNetworks/MPS.C:8, TensorNetworksImp/MPS/MPSSite_Normalization.C:45-78).
  * dir 0 = DLeft, 1 = DRight. */
 int tn_chain_normalize(tn_chain* chain, int dir);
+/* MPS::NormalizeQR(Direction) (TensorNetworksImp/MPS/MPSSite_Normalization.C:80-107):
+ * QR gauge moves (CholeskyQR2 on the device, SVD fallback); bond values cleared. */
+int tn_chain_normalize_qr(tn_chain* chain, int dir);
 /* MPSImp::LoadHeffCaches (TensorNetworksImp/MPS/MPSImp_Variational.C:120-124) */
 int tn_chain_load_caches(tn_chain* chain, const tn_sweep_opts* opts);
 /* MPSImp::Sweep (TensorNetworksImp/MPS/MPSImp_Variational.C:64-76): Refine every

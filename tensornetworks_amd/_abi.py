@@ -30,6 +30,7 @@ class SweepOpts(ctypes.Structure):
         ("refine", ctypes.c_int),
         ("refine_tol", ctypes.c_double),
         ("gemm_tile", ctypes.c_int),
+        ("gauge", ctypes.c_int),
     ]
 
     @classmethod
@@ -57,6 +58,8 @@ class SweepStats(ctypes.Structure):
         ("lanczos_ms", ctypes.c_double),
         ("wall_ms", ctypes.c_double),
         ("energy", ctypes.c_double),
+        ("n_qr", ctypes.c_int64),
+        ("n_qr_fallback", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -93,6 +96,7 @@ def _load():
         "tn_chain_set_site": (i, [v, i, i, i, _dp]),
         "tn_sweep_opts_default": (None, [ctypes.POINTER(SweepOpts)]),
         "tn_chain_normalize": (i, [v, i]),
+        "tn_chain_normalize_qr": (i, [v, i]),
         "tn_chain_load_caches": (i, [v, ctypes.POINTER(SweepOpts)]),
         "tn_chain_sweep": (i, [v, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats)]),
         "tn_chain_refine_site": (i, [v, i, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats)]),
@@ -254,6 +258,9 @@ class Chain:
 
     def normalize(self, direction):
         self.ctx.check(lib.tn_chain_normalize(self.h, direction), "tn_chain_normalize")
+
+    def normalize_qr(self, direction):
+        self.ctx.check(lib.tn_chain_normalize_qr(self.h, direction), "tn_chain_normalize_qr")
 
     def load_caches(self, opts=None):
         opts = opts or SweepOpts.default()

@@ -284,8 +284,71 @@ void normalised_copy(const std::vector<double>& s, std::vector<double>& out)
     for (size_t k = 0; k < s.size(); k++) out[k] = nr > 0 ? s[k] / nr : 0.0;
 }
 
+// MPSSite_Normalization.C:80-107 NormalizeQR: A = Q R (left move) or A = L Q
+// (right move) by CholeskyQR2; the bond's singular values are cleared
+// (Bond::TransferQR -> ClearSingularValues, Bond.C:108-135).  Returns false
+// (nothing changed) when the matrix is wide or CholeskyQR2 is not accurate.
+bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
+{
+    (void)st;
+    tn_ctx* c = ch->ctx;
+    hipStream_t s = c->stream;
+    const int d = ch->d;
+    const int D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+    const double tol = 1e-11;
+    if (dir == 0) {
+        if (D1 * d < D2) return false;
+        const int m = D1p * d, n = D2p;
+        c->spare.ensure((size_t)n * n * 8);
+        double* Lacc = c->spare.as<double>();
+        if (!cholqr_device(c, ch->A[i], D2p, m, n, D2, ch->spare, Lacc, tol)) return false;
+        // site i <- Q (already in ch->spare: swap buffers)
+        std::swap(ch->A[i], ch->spare);
+        // neighbour: M_{i+1} <- R M_{i+1},  R = Lacc^T
+        const int j = i + 1, D3p = pad32(ch->D2[j]);
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.a_kmajor = false; g.b_kmajor = false;
+        g.M = n; g.N = d * D3p; g.K = n; g.batch = 1;
+        g.A = Lacc; g.lda = n; g.B = ch->A[j]; g.ldb = (long)d * D3p;
+        g.C = ch->spare; g.ldc = (long)d * D3p; g.alpha = 1.0f;
+        TN_HIP(gemm(g, s));
+        std::swap(ch->A[j], ch->spare);
+        rescale_site(ch, j);  // keep the norm O(1) along the chain (see normalised_copy)
+        ch->lam[i].assign(D2, 0.0);
+        ch->maxDelta[i] = 0.0;
+        return true;
+    }
+    if (d * D2 < D1) return false;
+    // A^T ((d*D2p) x D1p) by transposed copy, then A^T = Q R  ->  A = R^T Q^T
+    const int m = d * D2p, n = D1p;
+    c->svd_A.ensure((size_t)m * n * 8);
+    double* At = c->svd_A.as<double>();
+    TN_HIP(copy2(ch->A[i], (long)d * D2p, 1, D1p, d * D2p, At, 1, D1p, D1p, d * D2p, s));
+    c->spare.ensure((size_t)n * n * 8);
+    double* Lacc = c->spare.as<double>();
+    c->svd_U.ensure((size_t)m * n * 8);
+    double* Qt = c->svd_U.as<double>();
+    if (!cholqr_device(c, At, n, m, n, D1, Qt, Lacc, tol)) return false;
+    // site i <- Q^T
+    TN_HIP(copy2(Qt, 1, n, D1p, d * D2p, ch->A[i], (long)d * D2p, 1, D1p, d * D2p, s));
+    // neighbour: M_{i-1} <- M_{i-1} Lacc
+    const int j = i - 1, D0p = pad32(ch->D1[j]);
+    GemmDesc g{};
+    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+    g.M = D0p * d; g.N = n; g.K = n; g.batch = 1;
+    g.A = ch->A[j]; g.lda = n; g.B = Lacc; g.ldb = n;
+    g.C = ch->spare; g.ldc = n; g.alpha = 1.0f;
+    // Lacc lives in ctx->spare, the product goes to the chain's spare buffer
+    TN_HIP(gemm(g, s));
+    std::swap(ch->A[j], ch->spare);
+    rescale_site(ch, j);
+    ch->lam[i - 1].assign(D1, 0.0);
+    ch->maxDelta[i - 1] = 0.0;
+    return true;
+}
+
 // MPSSite_Normalization.C:45-78 SVDNormalize(lr, nullptr) incl. HandleEdges :20-43
-void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
+void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge = 0)
 {
     tn_ctx* c = ch->ctx;
     hipStream_t s = c->stream;
@@ -304,6 +367,15 @@ void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         return;
     }
     double t0 = now_ms();
+    if (gauge == 1 && qr_normalize(ch, i, dir, st)) {
+        if (st) {
+            sync(c);
+            st->svd_ms += now_ms() - t0;
+            st->n_qr++;
+        }
+        return;
+    }
+    if (gauge == 1 && st) st->n_qr_fallback++;
     std::vector<double> sv;
     if (dir == 0) {
         // A as (D1*d) x D2: row (a,m) at (a*d+m)*D2p, column b
@@ -376,7 +448,7 @@ void refine_site(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_
     ch->Emin[i] = e0;
     ch->GapE[i] = e1 - 1.0;  // MPSSite_Variational.C:29 as written
     if (st) st->energy = e0;
-    svd_normalize(ch, i, dir, st);
+    svd_normalize(ch, i, dir, st, o.gauge);
     env_update(ch, i, dir, o, st);
 }
 
@@ -513,6 +585,7 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->refine = 1;
     o->refine_tol = 1e-10;
     o->gemm_tile = 0;
+    o->gauge = 0;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
@@ -916,6 +989,20 @@ int tn_chain_normalize(tn_chain* ch, int dir)
         for (int i = 0; i < ch->L; i++) svd_normalize(ch, i, 0, nullptr);
     else
         for (int i = ch->L - 1; i >= 0; i--) svd_normalize(ch, i, 1, nullptr);
+    sync(c);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_normalize_qr(tn_chain* ch, int dir)
+{
+    if (!ch || (dir != 0 && dir != 1)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    if (dir == 0)
+        for (int i = 0; i < ch->L; i++) svd_normalize(ch, i, 0, nullptr, 1);
+    else
+        for (int i = ch->L - 1; i >= 0; i--) svd_normalize(ch, i, 1, nullptr, 1);
     sync(c);
     return TN_OK;
     ABI_END(c)

@@ -70,7 +70,7 @@ struct tn_ctx {
     std::string err;
     // scratch
     tn::DBuf T1, T2, xlo, part, krylov, lanczos_small, svd_X, svd_V, svd_U, svd_Vt, svd_s,
-        svd_perm, svd_pairs, svd_flag, svd_A, spare, pack;
+        svd_perm, svd_pairs, svd_flag, svd_A, spare, pack, qr_W, qr_G, qr_L, qr_T, qr_linv, qr_Q;
     tn::Scalars sc;
     hipEvent_t ev[8] = {};
     // seam state (tn_site_bind)
@@ -132,6 +132,14 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
 // [0, zr) x [0, ldu) and [0, zk) x [0, ldvt).
 int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int c_, double* U, long ldu,
                int zr, double* Vt, long ldvt, int zk, std::vector<double>& s);
+
+// CholeskyQR2 of a row-major m x n matrix A (lda), m >= n, both multiples of
+// 32, leading nl columns logical (the rest zero padding).  On success Q
+// (row-major m x n) has orthonormal logical columns, zero padded columns, and
+// Lacc (n x n row-major, lower) satisfies A = Q Lacc^T.  Returns false if the
+// Cholesky breaks down or max|Q^T Q - I| > tol (the caller falls back to SVD).
+bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q,
+                   double* Lacc, double tol);
 
 void sync(tn_ctx* c);
 // stream-ordered copies that return when the data has arrived

@@ -540,4 +540,111 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     return k;
 }
 
+// ---------------------------------------------------------------------------
+// CholeskyQR2 (two passes of G = A^T A, G = L L^T, Q = A L^-T), every large
+// step a GEMM on the matrix cores; the 32x32 diagonal blocks are factored in
+// LDS by chol_diag.
+static bool cholqr_pass(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* L)
+{
+    hipStream_t s = c->stream;
+    const int nb = n / 32;
+    double* G = c->qr_G.as<double>();
+    double* W = c->qr_W.as<double>();
+    double* linv = c->qr_linv.as<double>();
+    int* flag = c->svd_flag.as<int>();
+    // G = A^T A
+    GemmDesc g{};
+    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = false; g.b_kmajor = false;
+    g.M = n; g.N = n; g.K = m; g.batch = 1;
+    g.A = A; g.lda = lda; g.B = A; g.ldb = lda; g.C = G; g.ldc = n; g.alpha = 1.0f;
+    TN_HIP(gemm(g, s));
+    TN_HIP(add_diag(G, n, nl, n, 1.0, s));  // identity on the zero padding
+    TN_HIP(fill_zero(L, (long)n * n * 8, s));
+    TN_HIP(set_int(flag, 0, s));
+    for (int kb = 0; kb < nb; kb++) {
+        const int k0 = kb * 32, r0 = k0 + 32, rem = n - r0;
+        TN_HIP(chol_diag(G, n, k0, L, n, linv + (size_t)kb * 1024, flag, s));
+        if (rem <= 0) continue;
+        GemmDesc p{};  // L[r0:, k0:k0+32] = G[r0:, k0:k0+32] Linv_kk^T
+        p.tin = T_F64; p.tout = T_F64; p.a_kmajor = true; p.b_kmajor = true;
+        p.M = rem; p.N = 32; p.K = 32; p.batch = 1;
+        p.A = G + (size_t)r0 * n + k0; p.lda = n;
+        p.B = linv + (size_t)kb * 1024; p.ldb = 32;
+        p.C = L + (size_t)r0 * n + k0; p.ldc = n; p.alpha = 1.0f;
+        TN_HIP(gemm(p, s));
+        GemmDesc t{};  // G[r0:, r0:] -= Lp Lp^T
+        t.tin = T_F64; t.tout = T_F64; t.a_kmajor = true; t.b_kmajor = true;
+        t.M = rem; t.N = rem; t.K = 32; t.batch = 1;
+        t.A = L + (size_t)r0 * n + k0; t.lda = n;
+        t.B = L + (size_t)r0 * n + k0; t.ldb = n;
+        t.C = G + (size_t)r0 * n + r0; t.ldc = n; t.alpha = -1.0f; t.accumulate = 1;
+        TN_HIP(gemm(t, s));
+    }
+    int hflag = 0;
+    d2h(c, &hflag, flag, sizeof(int));
+    if (hflag) return false;
+    // Q = A L^-T by blocked forward substitution on a work copy W of A
+    TN_HIP(copy2(A, lda, 1, m, n, W, n, 1, m, n, s));
+    for (int kb = 0; kb < nb; kb++) {
+        const int k0 = kb * 32, r0 = k0 + 32, rem = n - r0;
+        GemmDesc q{};  // Q[:, k0:k0+32] = W[:, k0:k0+32] Linv_kk^T
+        q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = true;
+        q.M = m; q.N = 32; q.K = 32; q.batch = 1;
+        q.A = W + k0; q.lda = n;
+        q.B = linv + (size_t)kb * 1024; q.ldb = 32;
+        q.C = Q + k0; q.ldc = n; q.alpha = 1.0f;
+        TN_HIP(gemm(q, s));
+        if (rem <= 0) continue;
+        GemmDesc u{};  // W[:, r0:] -= Q[:, k0:k0+32] L[r0:, k0:k0+32]^T
+        u.tin = T_F64; u.tout = T_F64; u.a_kmajor = true; u.b_kmajor = true;
+        u.M = m; u.N = rem; u.K = 32; u.batch = 1;
+        u.A = Q + k0; u.lda = n;
+        u.B = L + (size_t)r0 * n + k0; u.ldb = n;
+        u.C = W + r0; u.ldc = n; u.alpha = -1.0f; u.accumulate = 1;
+        TN_HIP(gemm(u, s));
+    }
+    return true;
+}
+
+bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* Lacc,
+                   double tol)
+{
+    hipStream_t s = c->stream;
+    TN_REQUIRE(m % 32 == 0 && n % 32 == 0 && m >= n && nl <= n, TN_ERR_ARG, "cholqr: bad shape");
+    c->qr_G.ensure((size_t)n * n * 8);
+    c->qr_L.ensure((size_t)n * n * 8);
+    c->qr_T.ensure((size_t)n * n * 8);
+    c->qr_W.ensure((size_t)m * n * 8);
+    c->qr_Q.ensure((size_t)m * n * 8);
+    c->qr_linv.ensure((size_t)(n / 32) * 1024 * 8);
+    c->svd_flag.ensure(256);
+    c->part.ensure(1024 * 8);
+    double* Q1 = c->qr_Q.as<double>();
+    // pass 1: A -> Q1, L1 (into Lacc)
+    if (!cholqr_pass(c, A, lda, m, n, nl, Q1, Lacc)) return false;
+    // pass 2: Q1 -> Q, L2
+    double* L2 = c->qr_L.as<double>();
+    if (!cholqr_pass(c, Q1, n, m, n, nl, Q, L2)) return false;
+    // Lacc <- L1 L2  (A = Q1 L1^T = Q L2^T L1^T = Q (L1 L2)^T)
+    GemmDesc g{};
+    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+    g.M = n; g.N = n; g.K = n; g.batch = 1;
+    g.A = Lacc; g.lda = n; g.B = L2; g.ldb = n; g.C = c->qr_T.p; g.ldc = n; g.alpha = 1.0f;
+    TN_HIP(gemm(g, s));
+    TN_HIP(hipMemcpyAsync(Lacc, c->qr_T.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, s));
+    // orthogonality check on the logical columns
+    GemmDesc o{};
+    o.tin = T_F64; o.tout = T_F64; o.a_kmajor = false; o.b_kmajor = false;
+    o.M = n; o.N = n; o.K = m; o.batch = 1;
+    o.A = Q; o.lda = n; o.B = Q; o.ldb = n; o.C = c->qr_G.p; o.ldc = n; o.alpha = 1.0f;
+    TN_HIP(gemm(o, s));
+    double* dev = c->svd_s.as<double>();
+    c->svd_s.ensure(64 * 8);
+    dev = c->svd_s.as<double>();
+    TN_HIP(eye_deviation(c->qr_G.as<double>(), n, nl, dev, c->part.as<double>(), s));
+    double h = 0;
+    d2h(c, &h, dev, 8);
+    return std::isfinite(h) && h <= tol;
+}
+
 }  // namespace tn

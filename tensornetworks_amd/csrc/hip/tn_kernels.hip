@@ -1096,4 +1096,107 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// CholeskyQR pieces.  chol_diag_k factors one 32x32 diagonal block of a
+// symmetric positive definite G in LDS (one wavefront) and writes L_kk (lower)
+// and its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
+__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0,
+                                                  double* __restrict__ L, long ldl,
+                                                  double* __restrict__ Linv, int* __restrict__ flag)
+{
+    __shared__ double A[32][33];
+    __shared__ double X[32][33];
+    __shared__ int bad;
+    const int t = threadIdx.x;
+    if (t == 0) bad = 0;
+    for (int e = t; e < 1024; e += 64) {
+        const int i = e >> 5, j = e & 31;
+        A[i][j] = G[(long)(k0 + i) * ldg + k0 + j];
+    }
+    __syncthreads();
+    for (int j = 0; j < 32; j++) {
+        if (t == 0) {
+            const double dj = A[j][j];
+            if (!(dj > 0.0) || !isfinite(dj)) {
+                bad = 1;
+                A[j][j] = 1.0;
+            } else {
+                A[j][j] = sqrt(dj);
+            }
+        }
+        __syncthreads();
+        const double djj = A[j][j];
+        if (t > j && t < 32) A[t][j] /= djj;
+        __syncthreads();
+        for (int e = t; e < 1024; e += 64) {
+            const int i = e >> 5, kk = e & 31;
+            if (i > j && kk > j && kk <= i) A[i][kk] -= A[i][j] * A[kk][j];
+        }
+        __syncthreads();
+    }
+    // inverse of the lower factor: thread c solves L x = e_c
+    if (t < 32) {
+        const int c = t;
+        for (int i = 0; i < 32; i++) {
+            double v = (i == c) ? 1.0 : 0.0;
+            for (int kk = c; kk < i; kk++) v -= A[i][kk] * X[kk][c];
+            X[i][c] = (i < c) ? 0.0 : v / A[i][i];
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < 1024; e += 64) {
+        const int i = e >> 5, j = e & 31;
+        L[(long)(k0 + i) * ldl + k0 + j] = (j <= i) ? A[i][j] : 0.0;
+        Linv[i * 32 + j] = X[i][j];
+    }
+    if (t == 0 && bad) flag[0] = 1;
+}
+
+hipError_t chol_diag(const double* G, long ldg, int k0, double* L, long ldl, double* Linv, int* flag,
+                     hipStream_t s)
+{
+    chol_diag_k<<<1, 64, 0, s>>>(G, ldg, k0, L, ldl, Linv, flag);
+    return hipGetLastError();
+}
+
+// G[i*ldg + i] += v for i in [i0, i1)
+__global__ void add_diag_k(double* G, long ldg, int i0, int i1, double v)
+{
+    const int i = i0 + blockIdx.x * 256 + threadIdx.x;
+    if (i < i1) G[(long)i * ldg + i] += v;
+}
+hipError_t add_diag(double* G, long ldg, int i0, int i1, double v, hipStream_t s)
+{
+    if (i1 <= i0) return hipSuccess;
+    add_diag_k<<<(i1 - i0 + 255) / 256, 256, 0, s>>>(G, ldg, i0, i1, v);
+    return hipGetLastError();
+}
+
+// out[0] = max_{i,j<n} |G[i][j] - delta_ij|   (deviation from the identity)
+__global__ void __launch_bounds__(256) eye_dev_k(const double* __restrict__ G, long ldg, int n,
+                                                 double* __restrict__ part)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    const long tot = (long)n * n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+        const long i = e / n, j = e - i * n;
+        const double v = G[i * ldg + j] - (i == j ? 1.0 : 0.0);
+        m = fmax(m, isfinite(v) ? fabs(v) : 1e300);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* part, hipStream_t s)
+{
+    long G_ = ((long)n * n + 255) / 256;
+    if (G_ > 1024) G_ = 1024;
+    eye_dev_k<<<G_, 256, 0, s>>>(G, ldg, n, part);
+    absmax2_k<<<1, 256, 0, s>>>(part, (int)G_, out);
+    return hipGetLastError();
+}
+
 }  // namespace tn

@@ -77,3 +77,40 @@ def test_variational_ground_state_schedule(ctx, case):
     assert abs(E - case["E"]) < max(case["tol"], 1e-10), (case, E)
     if case["var"] is not None:
         assert res.de2 < case["var"]
+
+
+def test_qr_gauge_same_fixed_point(ctx):
+    # NormalizeQR gauge moves (MPSSite_Normalization.C:80-107) reach the same
+    # variational minimum; bond values are cleared as in Bond::ClearSingularValues
+    L, D = 10, 8
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), D)
+    ch.initialize(1, 20261202)
+    ch.normalize_qr(1)
+    opts = tn.SweepOpts.default(eig_tol=1e-12, gauge=1)
+    ch.load_caches(opts)
+    st = tn.SweepStats()
+    for _ in range(10):
+        ch.sweep(0, opts, st)
+        ch.sweep(1, opts, st)
+    assert abs(ch.expectation() / 9 - (-0.473113399629092)) < 1e-11
+    assert st.n_qr > 0
+    assert np.all(ch.bond_sv(4) == 0.0)
+
+
+def test_qr_normalisation_right_canonical_large(ctx):
+    L, D = 12, 96
+    ch = tn.Chain(ctx, xxz_mpo(L, 1.0, Dz=0.1), D)
+    ch.initialize(1, 11)
+    ch.normalize_qr(1)
+    D1, D2 = ch.dims()
+    for i in range(1, L):
+        M = ch.site(i).reshape(D2[i], D1[i], 3)
+        Bn = np.einsum("bam,bcm->ac", M, M)
+        assert np.abs(Bn - np.eye(D1[i])).max() < 1e-12, i
+    # same state as the SVD normalisation up to the gauge: equal energies
+    ch2 = tn.Chain(ctx, xxz_mpo(L, 1.0, Dz=0.1), D)
+    ch2.initialize(1, 11)
+    ch2.normalize(1)
+    e1 = ch.expectation() / ch.expectation(identity_mpo(L, 3))
+    e2 = ch2.expectation() / ch2.expectation(identity_mpo(L, 3))
+    assert abs(e1 - e2) < 1e-10 * max(1.0, abs(e2))
