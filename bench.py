@@ -6,11 +6,16 @@ initial MPS (U(-1,1), normalised).  One step = one sweep = a left pass plus a
 right pass = 2L one-site updates (eigensolve + SVD gauge move + environment
 update), everything resident in HBM.
 
-Timed region: K sweeps in the headline precision (BF16 contractions with FP32
-accumulation, FP64 Krylov/Ritz/SVD/energy).  The same K sweeps are then run in
-FP64 from the identical saved state; |dE|/L compares the FP64 energies
-<H> of the two final MPS.  value = algorithmic Heff flops / device time of the
-Heff applications (HIP events) during the timed sweeps.
+Warmup: W sweeps from the random state in the headline precision bring the
+MPS close to its fixed point; the state is saved.  Timed region: K sweeps in
+the headline precision ("mixed": Heff contractions in BF16 on the matrix cores
+with FP32 accumulation against FP32 environments, the Ritz vector refined by
+FP32-contraction Lanczos restarts to the FP64 tolerance; FP64 Krylov vectors,
+re-orthogonalisation, tridiagonal solve, energies and QR/SVD gauge moves).
+The same K sweeps are then run in FP64 from the identical saved state; |dE|/L
+compares the FP64 energies <H> of the two final MPS.  value = algorithmic Heff
+flops / device time of the Heff applications (HIP events) during the timed
+sweeps.
 """
 import argparse
 import json
@@ -28,12 +33,12 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--L", type=int, default=100)
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--S", type=float, default=1.0)
     p.add_argument("--Dz", type=float, default=0.1)
-    p.add_argument("--precision", default="bf16", choices=["bf16", "fp32", "fp64"])
+    p.add_argument("--precision", default="mixed", choices=["mixed", "bf16", "fp32", "fp64"])
     p.add_argument("--eig-tol", type=float, default=1e-8)
     p.add_argument("--max-krylov", type=int, default=24)
     p.add_argument("--max-restarts", type=int, default=0)
@@ -79,7 +84,8 @@ def main():
     import tensornetworks_amd as tn
     from tensornetworks_amd.models import xxz_mpo
 
-    prec = {"bf16": tn.TN_BF16, "fp32": tn.TN_FP32, "fp64": tn.TN_FP64}[a.precision]
+    P = {"bf16": tn.TN_BF16, "fp32": tn.TN_FP32, "fp64": tn.TN_FP64}
+    prec = tn.TN_BF16 if a.precision == "mixed" else P[a.precision]
     d = int(round(2 * a.S + 1))
     ctx = tn.Context(0)
     Ws = xxz_mpo(a.L, a.S, Dz=a.Dz)
@@ -91,16 +97,21 @@ def main():
         ch.normalize_qr(1)
     else:
         ch.normalize(1)
-    common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, refine=0, gauge=gauge)
-    o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, **common)
-    o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, **common)
+    common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, gauge=gauge)
+    if a.precision == "mixed":
+        o_mixed = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
+                                       refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
+    else:
+        o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, refine=0, **common)
+    o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, refine=0, **common)
     ch.load_caches(o_mixed)
     setup_s = time.perf_counter() - t0
     print(f"# setup {setup_s:.1f}s", file=sys.stderr, flush=True)
     for w in range(a.warmup):
         ch.sweep(0, o_mixed)
         ch.sweep(1, o_mixed)
-        print(f"# warmup sweep {w} E/L={ch.expectation() / a.L:.10f}", file=sys.stderr, flush=True)
+        print(f"# warmup sweep {w} E/L={ch.expectation() / a.L:.12f} ({time.perf_counter() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
     saved = None
     if not a.no_fp64_compare:
         D1, D2 = ch.dims()
@@ -114,10 +125,13 @@ def main():
     wall = time.perf_counter() - t0
     # ---------------------------------------------------------
     E_mixed = ch.expectation()
-    print(f"# timed {a.steps} sweep(s) {wall:.2f}s E/L={E_mixed / a.L:.12f} matvecs={st.n_matvec}",
+    print(f"# timed {a.steps} sweep(s) {wall:.2f}s E/L={E_mixed / a.L:.12f} matvecs={st.n_matvec} "
+          f"(fp64 {st.n_matvec_fp64}) qr={st.n_qr} shifted={st.n_qr_shifted} fallback={st.n_qr_fallback}",
           file=sys.stderr, flush=True)
     site = a.L // 2
     s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
+    print(f"# heff bench site {site}: stage1 {s1:.3f} ms stage3 {s3:.3f} ms mix {mix:.3f} ms",
+          file=sys.stderr, flush=True)
     res_fp64 = None
     if saved is not None:
         for i, (d1, d2, M) in enumerate(saved):
@@ -134,12 +148,14 @@ def main():
                     "heff_tflops": st64.heff_flops / max(st64.heff_ms, 1e-12) / 1e9,
                     "n_matvec": st64.n_matvec, "dE_per_site": abs(E_mixed - E64) / a.L,
                     "gauge_ms": st64.svd_ms, "env_ms": st64.env_ms, "lanczos_ms": st64.lanczos_ms,
-                    "n_qr_fallback": st64.n_qr_fallback}
+                    "heff_ms": st64.heff_ms, "n_qr_fallback": st64.n_qr_fallback,
+                    "ms_per_matvec": st64.heff_ms / max(st64.n_matvec, 1)}
         print(f"# fp64 {wall64:.2f}s E/L={E64 / a.L:.12f} |dE|/L={abs(E_mixed - E64) / a.L:.3e}",
               file=sys.stderr, flush=True)
     value = st.heff_flops / max(st.heff_ms, 1e-12) / 1e9
     achieved = f1 / (s1 * 1e-3) / 1e12
-    B1 = {"bf16": 2, "fp32": 4, "fp64": 8}[a.precision]
+    hp = "bf16" if a.precision == "mixed" else a.precision
+    B1 = {"bf16": 2, "fp32": 4, "fp64": 8}[hp]
     out = {
         "metric": "DMRG sweep time + Heff TFLOP/s at D=1024; mixed vs FP64 at |dE/L|<=1e-8",
         "value": value,
@@ -151,22 +167,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.precision,
+        "dtype": hp,
         "data": "synthetic (seeded random initial MPS)",
         "config": {"workload": f"S={a.S:g} Heisenberg + Dz={a.Dz:g} Sz^2 open chain, L={a.L}, d={d}, Dw=5, D={a.D}, "
-                               f"one-site sweep, contractions {a.precision}, FP64 Krylov/Ritz/gauge/energy",
+                               f"one-site sweep, precision {a.precision} (Heff {hp}"
+                               + (", FP32 environments, FP32-contraction refinement" if a.precision == "mixed" else "")
+                               + "), FP64 Krylov/Ritz/gauge/energy",
+                   "precision": a.precision,
                    "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov,
                    "max_restarts": a.max_restarts, "gauge": a.gauge},
         "sweep_s": wall / a.steps,
         "E_per_site": E_mixed / a.L,
         "breakdown_ms": {"heff": st.heff_ms, "heff_gemm": st.heff_gemm_ms, "lanczos": st.lanczos_ms,
                          "env": st.env_ms, "gauge": st.svd_ms, "wall": st.wall_ms},
-        "n_qr": st.n_qr, "n_qr_fallback": st.n_qr_fallback,
-        "n_matvec": st.n_matvec,
+        "n_qr": st.n_qr, "n_qr_fallback": st.n_qr_fallback, "n_qr_shifted": st.n_qr_shifted,
+        "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64,
+        "ms_per_matvec": st.heff_ms / max(st.n_matvec, 1),
         "fp64": res_fp64,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[a.precision], "unit": "TFLOP/s",
-                     "frac": achieved / PEAKS[a.precision], "traffic": None,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[hp], "unit": "TFLOP/s",
+                     "frac": achieved / PEAKS[hp], "traffic": None,
                      "kernel": "Heff stage-1 GEMM (psi x R, NT) at the bulk site",
                      "per_launch_flops": f1, "avg_launch_ms": s1,
                      "stage3_ms": s3, "stage3_flops": f3, "mpo_stage_ms": mix,

@@ -99,9 +99,10 @@ int tn_chain_get_dims(const tn_chain* chain, int* D1, int* D2);
 int tn_chain_get_site(tn_chain* chain, int site, double* M); /* reference layout */
 int tn_chain_set_site(tn_chain* chain, int site, int D1, int D2, const double* M);
 
-/* Options of the local update.  precision: Heff arithmetic; env_precision:
- * environment-update arithmetic; refine: FP64 Lanczos polish of the Ritz
- * vector after a reduced-precision solve (0/1). */
+/* Options of the local update.  precision: Heff arithmetic of the Lanczos
+ * iterations; env_precision: environment-update arithmetic; refine: restart
+ * the Lanczos from the reduced-precision Ritz vector with the
+ * refine_precision operator until refine_tol (0/1). */
 typedef struct tn_sweep_opts {
     double eig_tol;      /* Lanczos residual tolerance |H x - theta x| */
     int max_krylov;      /* Krylov dimension cap (restarted beyond) */
@@ -116,6 +117,8 @@ typedef struct tn_sweep_opts {
                             MPSSite_Normalization.C:80-107: bond values cleared as in
                             Bond::ClearSingularValues), by CholeskyQR2 with fallback
                             to the SVD when it is not accurate */
+    int refine_precision;  /* operator precision of the refinement (FP64 or FP32),
+                              used when refine=1 and it is finer than precision */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {
@@ -132,6 +135,7 @@ typedef struct tn_sweep_stats {
     double energy;             /* last site eigenvalue */
     int64_t n_qr;              /* QR gauge moves */
     int64_t n_qr_fallback;     /* QR gauge moves that fell back to the SVD */
+    int64_t n_qr_shifted;      /* QR gauge moves whose first pass needed the shift */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);
@@ -182,6 +186,15 @@ int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, doub
  * of 32, K of 32.  tile: 0 automatic, else a forced tile code. */
 int tn_gemm_test(tn_ctx* ctx, int tin, int tout, int a_kmajor, int b_kmajor, int M, int N, int K,
                  int batch, const double* A, const double* B, double* C, int tile);
+
+/* The QR gauge kernel on its own, for numerics tests: A (row-major m x n,
+ * real, m >= n) = Q R with Q (m x n) orthonormal columns and R (n x n) upper
+ * triangular, by shifted CholeskyQR on the device (the factorisation behind
+ * tn_sweep_opts.gauge = 1; reference MPSSite_Normalization.C:80-107 uses a
+ * Householder QR for the same step).  Returns 0 when the factorisation met
+ * max|Q^T Q - I| <= tol, 1 when it did not (Q, R still written), < 0 on
+ * error.  *shifted = 1 when the first pass needed the shift. */
+int tn_cholqr_test(tn_ctx* ctx, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,7 @@ class SweepOpts(ctypes.Structure):
         ("refine_tol", ctypes.c_double),
         ("gemm_tile", ctypes.c_int),
         ("gauge", ctypes.c_int),
+        ("refine_precision", ctypes.c_int),
     ]
 
     @classmethod
@@ -60,6 +61,7 @@ class SweepStats(ctypes.Structure):
         ("energy", ctypes.c_double),
         ("n_qr", ctypes.c_int64),
         ("n_qr_fallback", ctypes.c_int64),
+        ("n_qr_shifted", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -87,6 +89,7 @@ def _load():
         "tn_svd": (i, [v, i, i, _dp, _dp, _dp, _dp]),
         "tn_svcompress_rank": (i, [i, _dp, i, d, _dp, _dp]),
         "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i]),
+        "tn_cholqr_test": (i, [v, i, i, _dp, ctypes.c_double, _dp, _dp, _ip]),
         "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
         "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
         "tn_chain_destroy": (i, [v]),
@@ -204,6 +207,18 @@ class Context:
         self.check(lib.tn_gemm_test(self.h, tin, tout, int(a_kmajor), int(b_kmajor), M, N, K, batch, P(A), P(B),
                                     P(C), tile), "tn_gemm_test")
         return C.reshape(batch, M, N)
+
+    def cholqr_test(self, A, tol=1e-11):
+        """A = Q R on the device (shifted CholeskyQR); returns (Q, R, ok, shifted)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        m, n = A.shape
+        Q = np.zeros((m, n))
+        R = np.zeros((n, n))
+        sh = ctypes.c_int(0)
+        rc = lib.tn_cholqr_test(self.h, m, n, P(A.ravel()), tol, P(Q.ravel()), P(R.ravel()), ctypes.byref(sh))
+        if rc < 0:
+            self.check(rc, "tn_cholqr_test")
+        return Q, R, rc == 0, sh.value
 
 
 def _mpo_arrays(Ws):

@@ -24,12 +24,18 @@ struct tn_chain {
     std::vector<std::vector<double>> W;
     std::vector<int> D1, D2;
     size_t site_cap = 0, env_cap = 0;
-    DBuf sites_buf, spare_buf, env_buf, lo_buf, bnd_buf;
+    DBuf sites_buf, spare_buf, env_buf, bnd_buf;
     std::vector<double*> A, Lenv, Renv;
-    std::vector<void*> Llo, Rlo;
-    int lo_prec = -1;
+    // reduced-precision copies of the environments, one set per precision
+    // (index TN_FP32 / TN_BF16); each set is refreshed whenever an FP64
+    // environment changes
+    struct LoSet {
+        bool active = false;
+        DBuf buf;
+        std::vector<void*> L, R;
+        void *bL = nullptr, *bR = nullptr;
+    } lo[3];
     double *bL = nullptr, *bR = nullptr;  // boundary environments, fp64
-    void *bLlo = nullptr, *bRlo = nullptr;
     std::vector<DevMix> mixH, mixL, mixR;
     // bonds b = 0..L-2
     std::vector<std::vector<double>> lam;
@@ -96,26 +102,50 @@ void download_site(tn_chain* ch, int i, std::vector<double>& Mref)
 void ensure_lo(tn_chain* ch, int prec)
 {
     if (prec == TN_FP64) return;
-    if (ch->lo_prec == prec) return;
+    tn_chain::LoSet& s = ch->lo[prec];
+    if (s.active) return;
     tn_ctx* c = ch->ctx;
-    ch->lo_buf.ensure((size_t)2 * ch->L * ch->env_cap * 4 + 2 * 32 * 32 * ch->Dwmax * 4 + 1024);
-    char* base = (char*)ch->lo_buf.p;
     const size_t es = prec == TN_FP32 ? 4 : 2;
+    s.buf.ensure((size_t)2 * ch->L * ch->env_cap * es + 2 * 32 * 32 * ch->Dwmax * es + 1024);
+    char* base = (char*)s.buf.p;
+    s.L.assign(ch->L, nullptr);
+    s.R.assign(ch->L, nullptr);
     for (int i = 0; i < ch->L; i++) {
-        ch->Llo[i] = base + (size_t)(2 * i) * ch->env_cap * es;
-        ch->Rlo[i] = base + (size_t)(2 * i + 1) * ch->env_cap * es;
+        s.L[i] = base + (size_t)(2 * i) * ch->env_cap * es;
+        s.R[i] = base + (size_t)(2 * i + 1) * ch->env_cap * es;
     }
     char* bb = base + (size_t)2 * ch->L * ch->env_cap * es;
-    ch->bLlo = bb;
-    ch->bRlo = bb + (size_t)32 * 32 * ch->Dwmax * es + 256;
-    ch->lo_prec = prec;
-    // refresh every valid environment
-    make_lowprec(c, prec, ch->bL, ch->bLlo, (long)32 * 32 * ch->Dwmax);
-    make_lowprec(c, prec, ch->bR, ch->bRlo, (long)32 * 32 * ch->Dwmax);
+    s.bL = bb;
+    s.bR = bb + (size_t)32 * 32 * ch->Dwmax * es + 256;
+    s.active = true;
+    make_lowprec(c, prec, ch->bL, s.bL, (long)32 * 32 * ch->Dwmax);
+    make_lowprec(c, prec, ch->bR, s.bR, (long)32 * 32 * ch->Dwmax);
     for (int i = 0; i < ch->L; i++) {
-        make_lowprec(c, prec, ch->Lenv[i], ch->Llo[i], (long)ch->env_cap);
-        make_lowprec(c, prec, ch->Renv[i], ch->Rlo[i], (long)ch->env_cap);
+        make_lowprec(c, prec, ch->Lenv[i], s.L[i], (long)ch->env_cap);
+        make_lowprec(c, prec, ch->Renv[i], s.R[i], (long)ch->env_cap);
     }
+}
+
+void drop_lo(tn_chain* ch)
+{
+    for (auto& s : ch->lo) {
+        s.buf.release();
+        s.active = false;
+    }
+}
+
+// reduced-precision left/right environment feeding site i (boundary at the edges)
+const void* lo_left(tn_chain* ch, int prec, int i)
+{
+    if (prec == TN_FP64) return nullptr;
+    ensure_lo(ch, prec);
+    return i > 0 ? ch->lo[prec].L[i - 1] : ch->lo[prec].bL;
+}
+const void* lo_right(tn_chain* ch, int prec, int i)
+{
+    if (prec == TN_FP64) return nullptr;
+    ensure_lo(ch, prec);
+    return i < ch->L - 1 ? ch->lo[prec].R[i + 1] : ch->lo[prec].bR;
 }
 
 // --- environment update of site i (dir 0: left env, dir 1: right env) ---
@@ -124,23 +154,22 @@ void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_s
     tn_ctx* c = ch->ctx;
     const int d = ch->d, D1p = pad32(ch->D1[i]), D2p = pad32(ch->D2[i]);
     const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
-    int eprec = o.env_precision;
-    if (eprec != TN_FP64 && eprec != o.precision) eprec = TN_FP64;
-    const int lo = (o.precision != TN_FP64) ? o.precision : -1;
-    if (lo >= 0) ensure_lo(ch, lo);
+    const int eprec = o.env_precision;
     double t0 = now_ms();
     if (dir == 0) {
         mix_build(ch->mixL[i], MIX_ENVL, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
         const double* Lin = i > 0 ? ch->Lenv[i - 1] : ch->bL;
-        const void* Llo = i > 0 ? ch->Llo[i - 1] : ch->bLlo;
-        env_left(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixL[i], Lin, Llo, ch->A[i], ch->Lenv[i], o.gemm_tile);
-        if (lo >= 0) make_lowprec(c, lo, ch->Lenv[i], ch->Llo[i], (long)D2p * Dw2 * D2p);
+        env_left(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixL[i], Lin, lo_left(ch, eprec, i), ch->A[i], ch->Lenv[i],
+                 o.gemm_tile);
+        for (int p = TN_FP32; p <= TN_BF16; p++)
+            if (ch->lo[p].active) make_lowprec(c, p, ch->Lenv[i], ch->lo[p].L[i], (long)D2p * Dw2 * D2p);
     } else {
         mix_build(ch->mixR[i], MIX_ENVR, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
         const double* Rin = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
-        const void* Rlo = i < ch->L - 1 ? ch->Rlo[i + 1] : ch->bRlo;
-        env_right(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixR[i], Rin, Rlo, ch->A[i], ch->Renv[i], o.gemm_tile);
-        if (lo >= 0) make_lowprec(c, lo, ch->Renv[i], ch->Rlo[i], (long)Dw1 * D1p * D1p);
+        env_right(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixR[i], Rin, lo_right(ch, eprec, i), ch->A[i], ch->Renv[i],
+                  o.gemm_tile);
+        for (int p = TN_FP32; p <= TN_BF16; p++)
+            if (ch->lo[p].active) make_lowprec(c, p, ch->Renv[i], ch->lo[p].R[i], (long)Dw1 * D1p * D1p);
     }
     if (st) {
         sync(c);
@@ -190,8 +219,8 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
     op.L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
     op.R = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
     op.precision = prec;
-    op.Llo = prec == TN_FP64 ? nullptr : (i > 0 ? ch->Llo[i - 1] : ch->bLlo);
-    op.Rlo = prec == TN_FP64 ? nullptr : (i < ch->L - 1 ? ch->Rlo[i + 1] : ch->bRlo);
+    op.Llo = lo_left(ch, prec, i);
+    op.Rlo = lo_right(ch, prec, i);
     mix_build(ch->mixH[i], MIX_HEFF, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream);
     op.mix = &ch->mixH[i];
     op.tile = tile;
@@ -202,7 +231,6 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
 void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1)
 {
     tn_ctx* c = ch->ctx;
-    if (o.precision != TN_FP64) ensure_lo(ch, o.precision);
     HeffCall hc;
     hc.op = make_heff(ch, i, o.precision, o.gemm_tile);
     hc.st = st;
@@ -210,11 +238,17 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     hc.D2 = ch->D2[i];
     const long N = hc.op.n();
     double t0 = now_ms();
-    LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts);
-    if (o.precision != TN_FP64 && o.refine) {
-        HeffCall h64 = hc;
-        h64.op = make_heff(ch, i, TN_FP64, o.gemm_tile);
-        LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts);
+    // a reduced-precision operator cannot resolve residuals below its own
+    // rounding (~2^-8 for BF16, ~2^-23 for FP32 operands, relative to |theta|)
+    // when a refinement follows, stop there and let the refinement finish
+    const bool refining = o.refine && o.refine_precision < o.precision;
+    const double rel = refining ? (o.precision == TN_BF16 ? 4e-3 : 4e-7) : 0.0;
+    LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel);
+    if (refining) {
+        // polish the reduced-precision Ritz vector with a more accurate operator
+        HeffCall hr = hc;
+        hr.op = make_heff(ch, i, o.refine_precision, o.gemm_tile);
+        LanczosResult r2 = lanczos(c, heff_cb, &hr, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts);
         r.e0 = r2.e0;
         r.e1 = r2.e1;
     }
@@ -285,12 +319,12 @@ void normalised_copy(const std::vector<double>& s, std::vector<double>& out)
 }
 
 // MPSSite_Normalization.C:80-107 NormalizeQR: A = Q R (left move) or A = L Q
-// (right move) by CholeskyQR2; the bond's singular values are cleared
+// (right move) by shifted CholeskyQR; the bond's singular values are cleared
 // (Bond::TransferQR -> ClearSingularValues, Bond.C:108-135).  Returns false
-// (nothing changed) when the matrix is wide or CholeskyQR2 is not accurate.
+// (nothing changed) when the matrix is wide or CholeskyQR is not accurate.
 bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
 {
-    (void)st;
+    int shifted = 0;
     tn_ctx* c = ch->ctx;
     hipStream_t s = c->stream;
     const int d = ch->d;
@@ -301,7 +335,8 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         const int m = D1p * d, n = D2p;
         c->spare.ensure((size_t)n * n * 8);
         double* Lacc = c->spare.as<double>();
-        if (!cholqr_device(c, ch->A[i], D2p, m, n, D2, ch->spare, Lacc, tol)) return false;
+        if (!cholqr_device(c, ch->A[i], D2p, m, n, D2, ch->spare, Lacc, tol, &shifted)) return false;
+        if (st) st->n_qr_shifted += shifted;
         // site i <- Q (already in ch->spare: swap buffers)
         std::swap(ch->A[i], ch->spare);
         // neighbour: M_{i+1} <- R M_{i+1},  R = Lacc^T
@@ -328,7 +363,8 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
     double* Lacc = c->spare.as<double>();
     c->svd_U.ensure((size_t)m * n * 8);
     double* Qt = c->svd_U.as<double>();
-    if (!cholqr_device(c, At, n, m, n, D1, Qt, Lacc, tol)) return false;
+    if (!cholqr_device(c, At, n, m, n, D1, Qt, Lacc, tol, &shifted)) return false;
+    if (st) st->n_qr_shifted += shifted;
     // site i <- Q^T
     TN_HIP(copy2(Qt, 1, n, D1p, d * D2p, ch->A[i], (long)d * D2p, 1, D1p, d * D2p, s));
     // neighbour: M_{i-1} <- M_{i-1} Lacc
@@ -494,8 +530,7 @@ void chain_reserve(tn_chain* ch, int Dcap)
     ch->sites_buf = nsites;
     ch->env_buf.release();
     ch->spare_buf.release();
-    ch->lo_buf.release();
-    ch->lo_prec = -1;
+    drop_lo(ch);
     ch->Dcap = Dcap;
     ch->Dcapp = Dcapp;
     ch->site_cap = site_cap;
@@ -586,6 +621,7 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->refine_tol = 1e-10;
     o->gemm_tile = 0;
     o->gauge = 0;
+    o->refine_precision = TN_FP64;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
@@ -840,8 +876,6 @@ int tn_chain_create(tn_ctx* c, int L, int d, const int* dw1, const int* dw2, con
     ch->A.resize(L);
     ch->Lenv.resize(L);
     ch->Renv.resize(L);
-    ch->Llo.assign(L, nullptr);
-    ch->Rlo.assign(L, nullptr);
     for (int i = 0; i < L; i++) {
         ch->A[i] = ch->sites_buf.as<double>() + (size_t)i * ch->site_cap;
         ch->Lenv[i] = ch->env_buf.as<double>() + (size_t)(2 * i) * ch->env_cap;
@@ -892,7 +926,7 @@ int tn_chain_destroy(tn_chain* ch)
     ch->sites_buf.release();
     ch->spare_buf.release();
     ch->env_buf.release();
-    ch->lo_buf.release();
+    drop_lo(ch);
     ch->bnd_buf.release();
     ch->theta.release();
     ch->w12buf.release();
@@ -1175,7 +1209,6 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
     const int d = ch->d, L = ch->L, p2 = d * d;
     double mtr = 0.0;
     double t0w = now_ms();
-    if (o.precision != TN_FP64) ensure_lo(ch, o.precision);
     for (int step = 0; step < L - 1; step++) {
         const int i = (dir == 0) ? step : L - 2 - step, j = i + 1;
         const int D1 = ch->D1[i], Dm = ch->D2[i], D3 = ch->D2[j];
@@ -1200,8 +1233,8 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.op.L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
         hc.op.R = j < L - 1 ? ch->Renv[j + 1] : ch->bR;
         hc.op.precision = o.precision;
-        hc.op.Llo = o.precision == TN_FP64 ? nullptr : (i > 0 ? ch->Llo[i - 1] : ch->bLlo);
-        hc.op.Rlo = o.precision == TN_FP64 ? nullptr : (j < L - 1 ? ch->Rlo[j + 1] : ch->bRlo);
+        hc.op.Llo = lo_left(ch, o.precision, i);
+        hc.op.Rlo = lo_right(ch, o.precision, j);
         ch->mix2H[i].wsrc = nullptr;  // W12 rebuilt each visit
         mix_build(ch->mix2H[i], MIX_HEFF, ch->W12[i].data(), Dw1, Dw3, p2, D1p, D3p, s);
         hc.op.mix = &ch->mix2H[i];
@@ -1211,9 +1244,11 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.D2 = D3;
         double tl = now_ms();
         LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->theta.as<double>(), o.eig_tol, o.max_krylov, o.max_restarts);
-        if (o.precision != TN_FP64 && o.refine) {
+        if (o.refine && o.refine_precision < o.precision) {
             HeffCall h64 = hc;
-            h64.op.precision = TN_FP64;
+            h64.op.precision = o.refine_precision;
+            h64.op.Llo = lo_left(ch, o.refine_precision, i);
+            h64.op.Rlo = lo_right(ch, o.refine_precision, j);
             LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->theta.as<double>(), o.refine_tol, o.max_krylov, o.max_restarts);
             r.e0 = r2.e0;
             r.e1 = r2.e1;
@@ -1287,7 +1322,6 @@ int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, double*
     if (!ch || site < 0 || site >= ch->L || reps < 1) return TN_ERR_ARG;
     tn_ctx* c = ch->ctx;
     ABI_BEGIN
-    if (precision != TN_FP64) ensure_lo(ch, precision);
     HeffOp op = make_heff(ch, site, precision, 0);
     const long N = op.n();
     c->b_y.ensure((size_t)N * 8);
@@ -1349,6 +1383,37 @@ int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, i
     d2h(c, C, c64.p, nC * 8);
     for (auto* b : {&a64, &b64, &alo, &blo, &cout, &c64}) b->release();
     return TN_OK;
+    ABI_END(c)
+}
+
+int tn_cholqr_test(tn_ctx* c, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted)
+{
+    if (!c || !A || !Q || !R) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(m > 0 && n > 0 && m >= n, TN_ERR_ARG, "tn_cholqr_test: need m >= n > 0");
+    const int mp = pad32(m), np_ = pad32(n);
+    TN_REQUIRE(mp >= np_, TN_ERR_ARG, "tn_cholqr_test: padded m < padded n");
+    tn::DBuf a, q, l;
+    a.ensure((size_t)mp * np_ * 8);
+    q.ensure((size_t)mp * np_ * 8);
+    l.ensure((size_t)np_ * np_ * 8);
+    TN_HIP(fill_zero(a.p, (long)mp * np_ * 8, c->stream));
+    std::vector<double> h((size_t)mp * np_, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) h[(size_t)i * np_ + j] = A[(size_t)i * n + j];
+    h2d(c, a.p, h.data(), h.size() * 8);
+    int sh = 0;
+    const bool ok = cholqr_device(c, a.as<double>(), np_, mp, np_, n, q.as<double>(), l.as<double>(), tol, &sh);
+    d2h(c, h.data(), q.p, h.size() * 8);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) Q[(size_t)i * n + j] = h[(size_t)i * np_ + j];
+    std::vector<double> hl((size_t)np_ * np_);
+    d2h(c, hl.data(), l.p, hl.size() * 8);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) R[(size_t)i * n + j] = hl[(size_t)j * np_ + i];  // R = L^T
+    if (shifted) *shifted = sh;
+    for (auto* b : {&a, &q, &l}) b->release();
+    return ok ? TN_OK : 1;
     ABI_END(c)
 }
 

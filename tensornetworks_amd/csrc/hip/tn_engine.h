@@ -122,8 +122,9 @@ struct LanczosResult {
 };
 // generic operator callback: y = A x (device fp64)
 typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
+// converged when |H x - theta x| < tol or < tol_rel*|theta| (tol_rel = 0: off)
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
-                      int max_krylov, int max_restarts);
+                      int max_krylov, int max_restarts, double tol_rel = 0.0);
 
 // --- SVD -----------------------------------------------------------------
 // Thin SVD of a device matrix view A(i,j) = A[i*rs + j*cs], r x c.
@@ -133,13 +134,15 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
 int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int c_, double* U, long ldu,
                int zr, double* Vt, long ldvt, int zk, std::vector<double>& s);
 
-// CholeskyQR2 of a row-major m x n matrix A (lda), m >= n, both multiples of
-// 32, leading nl columns logical (the rest zero padding).  On success Q
-// (row-major m x n) has orthonormal logical columns, zero padded columns, and
-// Lacc (n x n row-major, lower) satisfies A = Q Lacc^T.  Returns false if the
-// Cholesky breaks down or max|Q^T Q - I| > tol (the caller falls back to SVD).
+// Shifted CholeskyQR (two passes, three when needed) of a row-major m x n
+// matrix A (lda), m >= n, both multiples of 32, leading nl columns logical
+// (the rest zero padding).  On success Q (row-major m x n) has orthonormal
+// logical columns, zero padded columns, and Lacc (n x n row-major, lower)
+// satisfies A = Q Lacc^T.  Returns false if max|Q^T Q - I| > tol after the
+// passes (the caller falls back to SVD).  *shifted = 1 when the first pass
+// needed the shift.
 bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q,
-                   double* Lacc, double tol);
+                   double* Lacc, double tol, int* shifted = nullptr);
 
 void sync(tn_ctx* c);
 // stream-ordered copies that return when the data has arrived

@@ -321,7 +321,7 @@ void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, c
 // Krylov vectors, tridiagonal solve, Ritz vector and residual stay in FP64 on
 // the device; the host reads 4 scalars per iteration for loop control.
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
-                      int max_krylov, int max_restarts)
+                      int max_krylov, int max_restarts, double tol_rel)
 {
     hipStream_t s = c->stream;
     LanczosResult res;
@@ -388,7 +388,8 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
             res.residual = hs[2];
             const double b = hs[3];
             if (!std::isfinite(res.e0)) throw TnError{TN_ERR_STATE, "Lanczos: non-finite Ritz value"};
-            if (res.residual < tol || b < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
+            if (res.residual < tol || res.residual < tol_rel * std::fabs(res.e0) ||
+                b < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
                 conv = true;
                 break;
             }
@@ -541,110 +542,159 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
 }
 
 // ---------------------------------------------------------------------------
-// CholeskyQR2 (two passes of G = A^T A, G = L L^T, Q = A L^-T), every large
-// step a GEMM on the matrix cores; the 32x32 diagonal blocks are factored in
-// LDS by chol_diag.
-static bool cholqr_pass(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* L)
+// CholeskyQR (G = A^T A, G = L L^T, Q = A L^-T) with every large step a GEMM
+// on the matrix cores.  The 64x64 diagonal blocks of L are factored and
+// inverted in LDS (chol_diag); L^-1 is completed by recursive doubling of
+// block pairs, inv([L11 0; L21 L22]) = [L11^-1 0; -L22^-1 L21 L11^-1 L22^-1],
+// one batched GEMM pair per level, so Q is a single GEMM.
+
+// G = A^T A (n x n, row-major, ld n)
+static void qr_gram(tn_ctx* c, const double* A, long lda, int m, int n, double* G)
 {
-    hipStream_t s = c->stream;
-    const int nb = n / 32;
-    double* G = c->qr_G.as<double>();
-    double* W = c->qr_W.as<double>();
-    double* linv = c->qr_linv.as<double>();
-    int* flag = c->svd_flag.as<int>();
-    // G = A^T A
     GemmDesc g{};
     g.tin = T_F64; g.tout = T_F64; g.a_kmajor = false; g.b_kmajor = false;
     g.M = n; g.N = n; g.K = m; g.batch = 1;
     g.A = A; g.lda = lda; g.B = A; g.ldb = lda; g.C = G; g.ldc = n; g.alpha = 1.0f;
-    TN_HIP(gemm(g, s));
+    TN_HIP(gemm(g, c->stream));
+}
+
+// G (destroyed) = L L^T; L and Linv lower triangular n x n.  A pivot <= 0
+// sets flag[0].  shift > 0 adds shift * trace(G) to the logical diagonal first.
+static void qr_factor(tn_ctx* c, double* G, int n, int nl, double shift, double* L, double* Linv, int* flag)
+{
+    hipStream_t s = c->stream;
+    if (shift > 0.0) TN_HIP(shift_diag(G, n, nl, shift, s));
     TN_HIP(add_diag(G, n, nl, n, 1.0, s));  // identity on the zero padding
     TN_HIP(fill_zero(L, (long)n * n * 8, s));
-    TN_HIP(set_int(flag, 0, s));
-    for (int kb = 0; kb < nb; kb++) {
-        const int k0 = kb * 32, r0 = k0 + 32, rem = n - r0;
-        TN_HIP(chol_diag(G, n, k0, L, n, linv + (size_t)kb * 1024, flag, s));
+    TN_HIP(fill_zero(Linv, (long)n * n * 8, s));
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int nbk = std::min(64, n - k0), r0 = k0 + nbk, rem = n - r0;
+        TN_HIP(chol_diag(G, n, k0, nbk, L, n, Linv, n, flag, s));
         if (rem <= 0) continue;
-        GemmDesc p{};  // L[r0:, k0:k0+32] = G[r0:, k0:k0+32] Linv_kk^T
+        GemmDesc p{};  // L[r0:, k0:r0] = G[r0:, k0:r0] Linv_kk^T
         p.tin = T_F64; p.tout = T_F64; p.a_kmajor = true; p.b_kmajor = true;
-        p.M = rem; p.N = 32; p.K = 32; p.batch = 1;
+        p.M = rem; p.N = nbk; p.K = nbk; p.batch = 1;
         p.A = G + (size_t)r0 * n + k0; p.lda = n;
-        p.B = linv + (size_t)kb * 1024; p.ldb = 32;
+        p.B = Linv + (size_t)k0 * n + k0; p.ldb = n;
         p.C = L + (size_t)r0 * n + k0; p.ldc = n; p.alpha = 1.0f;
         TN_HIP(gemm(p, s));
         GemmDesc t{};  // G[r0:, r0:] -= Lp Lp^T
         t.tin = T_F64; t.tout = T_F64; t.a_kmajor = true; t.b_kmajor = true;
-        t.M = rem; t.N = rem; t.K = 32; t.batch = 1;
+        t.M = rem; t.N = rem; t.K = nbk; t.batch = 1;
         t.A = L + (size_t)r0 * n + k0; t.lda = n;
         t.B = L + (size_t)r0 * n + k0; t.ldb = n;
         t.C = G + (size_t)r0 * n + r0; t.ldc = n; t.alpha = -1.0f; t.accumulate = 1;
         TN_HIP(gemm(t, s));
     }
-    int hflag = 0;
-    d2h(c, &hflag, flag, sizeof(int));
-    if (hflag) return false;
-    // Q = A L^-T by blocked forward substitution on a work copy W of A
-    TN_HIP(copy2(A, lda, 1, m, n, W, n, 1, m, n, s));
-    for (int kb = 0; kb < nb; kb++) {
-        const int k0 = kb * 32, r0 = k0 + 32, rem = n - r0;
-        GemmDesc q{};  // Q[:, k0:k0+32] = W[:, k0:k0+32] Linv_kk^T
-        q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = true;
-        q.M = m; q.N = 32; q.K = 32; q.batch = 1;
-        q.A = W + k0; q.lda = n;
-        q.B = linv + (size_t)kb * 1024; q.ldb = 32;
-        q.C = Q + k0; q.ldc = n; q.alpha = 1.0f;
-        TN_HIP(gemm(q, s));
-        if (rem <= 0) continue;
-        GemmDesc u{};  // W[:, r0:] -= Q[:, k0:k0+32] L[r0:, k0:k0+32]^T
-        u.tin = T_F64; u.tout = T_F64; u.a_kmajor = true; u.b_kmajor = true;
-        u.M = m; u.N = rem; u.K = 32; u.batch = 1;
-        u.A = Q + k0; u.lda = n;
-        u.B = L + (size_t)r0 * n + k0; u.ldb = n;
-        u.C = W + r0; u.ldc = n; u.alpha = -1.0f; u.accumulate = 1;
-        TN_HIP(gemm(u, s));
+    // recursive doubling: at the start of level b every aligned b-block (and
+    // the shorter tail block) of Linv holds the inverse of that block of L
+    double* X = c->qr_T.as<double>();
+    for (int b = 64; b < n; b *= 2) {
+        const int P = n / (2 * b), tail = n - P * 2 * b;
+        for (int part = 0; part < 2; part++) {
+            const int np = part == 0 ? P : (tail > b ? 1 : 0);
+            if (np == 0) continue;
+            const int s2 = part == 0 ? b : tail - b;  // size of L22
+            const size_t o = part == 0 ? 0 : (size_t)P * 2 * b;
+            GemmDesc x{};  // X = L21 L11^-1   (s2 x b)
+            x.tin = T_F64; x.tout = T_F64; x.a_kmajor = true; x.b_kmajor = false;
+            x.M = s2; x.N = b; x.K = b; x.batch = np;
+            x.A = L + (o + b) * n + o; x.lda = n; x.sA = 2L * b * n + 2L * b;
+            x.B = Linv + o * n + o; x.ldb = n; x.sB = 2L * b * n + 2L * b;
+            x.C = X; x.ldc = b; x.sC = (long)b * b; x.alpha = 1.0f;
+            TN_HIP(gemm(x, s));
+            GemmDesc y{};  // Linv21 = -L22^-1 X
+            y.tin = T_F64; y.tout = T_F64; y.a_kmajor = true; y.b_kmajor = false;
+            y.M = s2; y.N = b; y.K = s2; y.batch = np;
+            y.A = Linv + (o + b) * n + o + b; y.lda = n; y.sA = 2L * b * n + 2L * b;
+            y.B = X; y.ldb = b; y.sB = (long)b * b;
+            y.C = Linv + (o + b) * n + o; y.ldc = n; y.sC = 2L * b * n + 2L * b; y.alpha = -1.0f;
+            TN_HIP(gemm(y, s));
+        }
     }
-    return true;
 }
 
+// Q = A Linv^T  (m x n)
+static void qr_apply(tn_ctx* c, const double* A, long lda, int m, int n, const double* Linv, double* Q)
+{
+    GemmDesc q{};
+    q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = true;
+    q.M = m; q.N = n; q.K = n; q.batch = 1;
+    q.A = A; q.lda = lda; q.B = Linv; q.ldb = n; q.C = Q; q.ldc = n; q.alpha = 1.0f;
+    TN_HIP(gemm(q, c->stream));
+}
+
+// Lacc <- Lacc Lk  (A = Q_prev Lacc^T = Q Lk^T Lacc^T)
+static void qr_accumulate(tn_ctx* c, double* Lacc, const double* Lk, int n)
+{
+    GemmDesc g{};
+    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+    g.M = n; g.N = n; g.K = n; g.batch = 1;
+    g.A = Lacc; g.lda = n; g.B = Lk; g.ldb = n; g.C = c->qr_T.p; g.ldc = n; g.alpha = 1.0f;
+    TN_HIP(gemm(g, c->stream));
+    TN_HIP(hipMemcpyAsync(Lacc, c->qr_T.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, c->stream));
+}
+
+// Shifted CholeskyQR: pass 1 factors A^T A directly and, if a pivot breaks
+// down (cond(A) beyond ~1e8), again with the shift 11 (m n + n (n+1)) u ||A||_F^2
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42
+// (2020) A477), which makes cond(Q1) moderate; passes 2 (and 3 when needed)
+// restore orthogonality.  The result is accepted when max|Q^T Q - I| <= tol.
 bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* Lacc,
-                   double tol)
+                   double tol, int* shifted)
 {
     hipStream_t s = c->stream;
     TN_REQUIRE(m % 32 == 0 && n % 32 == 0 && m >= n && nl <= n, TN_ERR_ARG, "cholqr: bad shape");
     c->qr_G.ensure((size_t)n * n * 8);
     c->qr_L.ensure((size_t)n * n * 8);
     c->qr_T.ensure((size_t)n * n * 8);
-    c->qr_W.ensure((size_t)m * n * 8);
+    c->qr_linv.ensure((size_t)n * n * 8);
     c->qr_Q.ensure((size_t)m * n * 8);
-    c->qr_linv.ensure((size_t)(n / 32) * 1024 * 8);
     c->svd_flag.ensure(256);
     c->part.ensure(1024 * 8);
-    double* Q1 = c->qr_Q.as<double>();
-    // pass 1: A -> Q1, L1 (into Lacc)
-    if (!cholqr_pass(c, A, lda, m, n, nl, Q1, Lacc)) return false;
-    // pass 2: Q1 -> Q, L2
-    double* L2 = c->qr_L.as<double>();
-    if (!cholqr_pass(c, Q1, n, m, n, nl, Q, L2)) return false;
-    // Lacc <- L1 L2  (A = Q1 L1^T = Q L2^T L1^T = Q (L1 L2)^T)
-    GemmDesc g{};
-    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
-    g.M = n; g.N = n; g.K = n; g.batch = 1;
-    g.A = Lacc; g.lda = n; g.B = L2; g.ldb = n; g.C = c->qr_T.p; g.ldc = n; g.alpha = 1.0f;
-    TN_HIP(gemm(g, s));
-    TN_HIP(hipMemcpyAsync(Lacc, c->qr_T.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, s));
-    // orthogonality check on the logical columns
-    GemmDesc o{};
-    o.tin = T_F64; o.tout = T_F64; o.a_kmajor = false; o.b_kmajor = false;
-    o.M = n; o.N = n; o.K = m; o.batch = 1;
-    o.A = Q; o.lda = n; o.B = Q; o.ldb = n; o.C = c->qr_G.p; o.ldc = n; o.alpha = 1.0f;
-    TN_HIP(gemm(o, s));
-    double* dev = c->svd_s.as<double>();
     c->svd_s.ensure(64 * 8);
-    dev = c->svd_s.as<double>();
-    TN_HIP(eye_deviation(c->qr_G.as<double>(), n, nl, dev, c->part.as<double>(), s));
-    double h = 0;
-    d2h(c, &h, dev, 8);
-    return std::isfinite(h) && h <= tol;
+    double* G = c->qr_G.as<double>();
+    double* Lk = c->qr_L.as<double>();
+    double* Linv = c->qr_linv.as<double>();
+    double* Q1 = c->qr_Q.as<double>();
+    int* flag = c->svd_flag.as<int>();
+    double* dev = c->svd_s.as<double>();
+    if (shifted) *shifted = 0;
+    // pass 1: A -> Q1, Lacc
+    TN_HIP(set_int(flag, 0, s));
+    qr_gram(c, A, lda, m, n, G);
+    qr_factor(c, G, n, nl, 0.0, Lacc, Linv, flag);
+    int hflag = 0;
+    d2h(c, &hflag, flag, sizeof(int));
+    if (hflag) {
+        const double u = std::ldexp(1.0, -53);
+        const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * u;
+        TN_HIP(set_int(flag, 0, s));
+        qr_gram(c, A, lda, m, n, G);
+        qr_factor(c, G, n, nl, shift, Lacc, Linv, flag);
+        if (shifted) *shifted = 1;
+    }
+    qr_apply(c, A, lda, m, n, Linv, Q1);
+    // pass 2: Q1 -> Q
+    qr_gram(c, Q1, n, m, n, G);
+    qr_factor(c, G, n, nl, 0.0, Lk, Linv, flag);
+    qr_apply(c, Q1, n, m, n, Linv, Q);
+    qr_accumulate(c, Lacc, Lk, n);
+    for (int pass = 3;; pass++) {
+        qr_gram(c, Q, n, m, n, G);
+        TN_HIP(eye_deviation(G, n, nl, dev, c->part.as<double>(), s));
+        double h = 0;
+        d2h(c, &h, dev, 8);
+        d2h(c, &hflag, flag, sizeof(int));
+        if (hflag || !std::isfinite(h)) return false;
+        if (h <= tol) return true;
+        if (pass > 3) return false;
+        // pass 3 on the Gram matrix just formed
+        qr_factor(c, G, n, nl, 0.0, Lk, Linv, flag);
+        qr_apply(c, Q, n, m, n, Linv, Q1);
+        TN_HIP(hipMemcpyAsync(Q, Q1, (size_t)m * n * 8, hipMemcpyDeviceToDevice, s));
+        qr_accumulate(c, Lacc, Lk, n);
+    }
 }
 
 }  // namespace tn

@@ -102,12 +102,15 @@ hipError_t set_unit_col(double* x, int p, int pos, hipStream_t s);
 hipError_t scale_cols(double* A, long lda, int r, int c, const double* sv, hipStream_t s);
 hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipStream_t s);
 
-// CholeskyQR pieces (fp64): factor the 32x32 diagonal block at k0 of G into
-// L (lower, ldl) and write its inverse (32x32 row-major); flag[0]=1 on a
-// non-positive pivot.  add_diag: G[i][i] += v, i in [i0,i1).  eye_deviation:
-// out[0] = max |G - I| over the leading n x n block (part: 1024 doubles).
-hipError_t chol_diag(const double* G, long ldg, int k0, double* L, long ldl, double* Linv, int* flag,
-                     hipStream_t s);
+// CholeskyQR pieces (fp64): chol_diag factors the nbk x nbk (nbk <= 64)
+// diagonal block at k0 of G into L (lower, ldl) and writes its inverse into
+// the same block of Linv (ldi); flag[0]=1 on a non-positive pivot.
+// shift_diag: G[i][i] += factor * trace(G), i < n.  add_diag: G[i][i] += v,
+// i in [i0,i1).  eye_deviation: out[0] = max |G - I| over the leading n x n
+// block (part: 1024 doubles).
+hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
+                     int* flag, hipStream_t s);
+hipError_t shift_diag(double* G, long ldg, int n, double factor, hipStream_t s);
 hipError_t add_diag(double* G, long ldg, int i0, int i1, double v, hipStream_t s);
 hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* part, hipStream_t s);
 

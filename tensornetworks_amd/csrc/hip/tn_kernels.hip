@@ -1097,24 +1097,28 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 }
 
 // ---------------------------------------------------------------------------
-// CholeskyQR pieces.  chol_diag_k factors one 32x32 diagonal block of a
+// CholeskyQR pieces.  chol_diag_k factors one CB x CB diagonal block of a
 // symmetric positive definite G in LDS (one wavefront) and writes L_kk (lower)
 // and its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
-__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0,
-                                                  double* __restrict__ L, long ldl,
-                                                  double* __restrict__ Linv, int* __restrict__ flag)
+#define CB 64
+__global__ void __launch_bounds__(256) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
+                                                   double* __restrict__ L, long ldl,
+                                                   double* __restrict__ Linv, long ldi,
+                                                   int* __restrict__ flag)
 {
-    __shared__ double A[32][33];
-    __shared__ double X[32][33];
+    __shared__ double A[CB][CB + 1];
+    __shared__ double X[CB][CB + 1];
     __shared__ int bad;
     const int t = threadIdx.x;
     if (t == 0) bad = 0;
-    for (int e = t; e < 1024; e += 64) {
-        const int i = e >> 5, j = e & 31;
-        A[i][j] = G[(long)(k0 + i) * ldg + k0 + j];
+    // a block narrower than CB (the last one) is completed with the identity
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e / CB, j = e % CB;
+        A[i][j] = (i < nbk && j < nbk) ? G[(long)(k0 + i) * ldg + k0 + j] : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int j = 0; j < 32; j++) {
+    // right-looking Cholesky on the lower triangle, one column per step
+    for (int j = 0; j < CB; j++) {
         if (t == 0) {
             const double dj = A[j][j];
             if (!(dj > 0.0) || !isfinite(dj)) {
@@ -1125,37 +1129,63 @@ __global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, 
             }
         }
         __syncthreads();
-        const double djj = A[j][j];
-        if (t > j && t < 32) A[t][j] /= djj;
+        if (t > j && t < CB) A[t][j] /= A[j][j];
         __syncthreads();
-        for (int e = t; e < 1024; e += 64) {
-            const int i = e >> 5, kk = e & 31;
-            if (i > j && kk > j && kk <= i) A[i][kk] -= A[i][j] * A[kk][j];
+        const int rem = CB - 1 - j;
+        for (int e = t; e < rem * rem; e += 256) {
+            const int r = j + 1 + e / rem, cc = j + 1 + e % rem;
+            if (cc <= r) A[r][cc] -= A[r][j] * A[cc][j];
         }
         __syncthreads();
     }
-    // inverse of the lower factor: thread c solves L x = e_c
-    if (t < 32) {
+    // inverse of the lower factor: thread c < CB solves L x = e_c
+    if (t < CB) {
         const int c = t;
-        for (int i = 0; i < 32; i++) {
+        for (int i = 0; i < CB; i++) {
             double v = (i == c) ? 1.0 : 0.0;
             for (int kk = c; kk < i; kk++) v -= A[i][kk] * X[kk][c];
             X[i][c] = (i < c) ? 0.0 : v / A[i][i];
         }
     }
     __syncthreads();
-    for (int e = t; e < 1024; e += 64) {
-        const int i = e >> 5, j = e & 31;
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e / CB, j = e % CB;
+        if (i >= nbk || j >= nbk) continue;
         L[(long)(k0 + i) * ldl + k0 + j] = (j <= i) ? A[i][j] : 0.0;
-        Linv[i * 32 + j] = X[i][j];
+        Linv[(long)(k0 + i) * ldi + k0 + j] = X[i][j];
     }
     if (t == 0 && bad) flag[0] = 1;
 }
 
-hipError_t chol_diag(const double* G, long ldg, int k0, double* L, long ldl, double* Linv, int* flag,
-                     hipStream_t s)
+hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
+                     int* flag, hipStream_t s)
 {
-    chol_diag_k<<<1, 64, 0, s>>>(G, ldg, k0, L, ldl, Linv, flag);
+    if (nbk <= 0 || nbk > CB) return hipErrorInvalidValue;
+    chol_diag_k<<<1, 256, 0, s>>>(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag);
+    return hipGetLastError();
+}
+
+// G[i][i] += factor * trace(G[0:n, 0:n]) for i < n  (the shift of shifted
+// CholeskyQR; trace(A^T A) = ||A||_F^2 bounds ||A||_2^2 from above)
+__global__ void __launch_bounds__(256) shift_diag_k(double* __restrict__ G, long ldg, int n, double factor)
+{
+    __shared__ double red[4];
+    __shared__ double sh;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) acc += G[(long)i * ldg + i];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) sh = factor * (red[0] + red[1] + red[2] + red[3]);
+    __syncthreads();
+    const double v = sh;
+    for (int i = threadIdx.x; i < n; i += 256) G[(long)i * ldg + i] += v;
+}
+hipError_t shift_diag(double* G, long ldg, int n, double factor, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    shift_diag_k<<<1, 256, 0, s>>>(G, ldg, n, factor);
     return hipGetLastError();
 }
 

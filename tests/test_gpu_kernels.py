@@ -154,3 +154,56 @@ def test_svd_matches_numpy(ctx, m, n, rank):
     assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
     assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12
     assert np.all(np.diff(s) <= 1e-15 * s_np[0])
+
+
+def _qr_positive(A):
+    """Householder QR with R's diagonal made positive: the unique QR of a full
+    column rank A, which every correct QR must reproduce."""
+    q, r = np.linalg.qr(A)
+    sg = np.sign(np.diag(r))
+    sg[sg == 0] = 1.0
+    return q * sg, (r.T * sg).T
+
+
+@pytest.mark.parametrize("m,n", [(64, 64), (200, 96), (300, 200), (640, 520), (1100, 1024)])
+def test_cholqr_matches_householder(ctx, m, n):
+    # well conditioned: the QR with positive diag(R) is unique, so the device's
+    # shifted CholeskyQR must agree with LAPACK's Householder QR
+    rng = np.random.default_rng(m * 7 + n)
+    A = rng.standard_normal((m, n))
+    Q, R, ok, shifted = ctx.cholqr_test(A)
+    assert ok and not shifted
+    q, r = _qr_positive(A)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-12
+    assert np.abs(np.tril(R, -1)).max() == 0.0
+    assert np.abs(Q @ R - A).max() < 1e-12 * np.abs(A).max() * np.sqrt(m)
+    assert np.abs(R - r).max() < 1e-10 * np.abs(r).max()
+    assert np.abs(Q - q).max() < 1e-9
+
+
+@pytest.mark.parametrize("cond", [1e10, 1e14])
+def test_cholqr_ill_conditioned_uses_shift(ctx, cond):
+    # A = U diag(s) V^T with s spanning `cond`: plain Cholesky of A^T A breaks
+    # down, the shifted first pass keeps Q orthonormal and Q R = A
+    rng = np.random.default_rng(5)
+    m, n = 384, 256
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.logspace(0, -np.log10(cond), n)
+    A = (U * s) @ V.T
+    Q, R, ok, shifted = ctx.cholqr_test(A)
+    assert ok and shifted
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-11
+    assert np.abs(Q @ R - A).max() < 1e-13
+    assert np.abs(np.tril(R, -1)).max() == 0.0
+
+
+def test_cholqr_rank_deficient(ctx):
+    # exactly repeated columns (Schmidt values that are exactly zero)
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((256, 80))
+    A = np.concatenate([B, B[:, :48]], axis=1)
+    Q, R, ok, shifted = ctx.cholqr_test(A)
+    assert ok and shifted
+    assert np.abs(Q.T @ Q - np.eye(128)).max() < 1e-11
+    assert np.abs(Q @ R - A).max() < 1e-12
