@@ -8,14 +8,13 @@ update), everything resident in HBM.
 
 Warmup: W sweeps from the random state in the headline precision bring the
 MPS close to its fixed point; the state is saved.  Timed region: K sweeps in
-the headline precision ("mixed": Heff contractions in BF16 on the matrix cores
-with FP32 accumulation against FP32 environments, the Ritz vector refined by
-FP32-contraction Lanczos restarts to the FP64 tolerance; FP64 Krylov vectors,
-re-orthogonalisation, tridiagonal solve, energies and QR/SVD gauge moves).
-The same K sweeps are then run in FP64 from the identical saved state; |dE|/L
-compares the FP64 energies <H> of the two final MPS.  value = algorithmic Heff
-flops / device time of the Heff applications (HIP events) during the timed
-sweeps.
+the headline precision (default "fp32": Heff and environment contractions in
+FP32 on the matrix cores; Krylov vectors, re-orthogonalisation, tridiagonal
+solve, energies and QR/SVD gauge moves in FP64).  From the identical saved
+state the same K sweeps are then timed with BF16 contractions refined by FP32
+ones, and in FP64; |dE|/L compares the FP64 energies <H> of the final MPS of
+the headline and FP64 runs.  value = algorithmic Heff flops / device time of
+the Heff applications (HIP events) during the timed sweeps.
 """
 import argparse
 import json
@@ -38,15 +37,23 @@ def parse():
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--S", type=float, default=1.0)
     p.add_argument("--Dz", type=float, default=0.1)
-    p.add_argument("--precision", default="mixed", choices=["mixed", "bf16", "fp32", "fp64"])
+    p.add_argument("--precision", default="fp32", choices=["mixed", "bf16", "fp32", "fp64"],
+                   help="fp32: FP32 contractions (FP32 environments); mixed: BF16 contractions refined by "
+                        "FP32 ones; Krylov vectors, Ritz solve, gauge and energies are FP64 in every mode")
+    p.add_argument("--no-bf16-compare", action="store_true",
+                   help="skip the extra timed sweep of the BF16-then-FP32 scheme")
     p.add_argument("--eig-tol", type=float, default=1e-8)
     p.add_argument("--max-krylov", type=int, default=24)
     p.add_argument("--max-restarts", type=int, default=0)
     p.add_argument("--gauge", default="qr", choices=["qr", "svd"])
     p.add_argument("--no-fp64-compare", action="store_true")
+    p.add_argument("--no-autotune", action="store_true", help="built-in GEMM tile choice instead of per-shape tuning")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--out", default=None)
+    p.add_argument("--heff-only", action="store_true",
+                   help="set up the chain and time only the bulk-site Heff stages (for counter runs), "
+                        "plus one plain device copy of known size that calibrates the byte counters")
     return p.parse_args()
 
 
@@ -88,6 +95,8 @@ def main():
     prec = tn.TN_BF16 if a.precision == "mixed" else P[a.precision]
     d = int(round(2 * a.S + 1))
     ctx = tn.Context(0)
+    if not a.no_autotune:
+        ctx.set_autotune(True)
     Ws = xxz_mpo(a.L, a.S, Dz=a.Dz)
     ch = tn.Chain(ctx, Ws, a.D)
     ch.initialize(1, a.seed)
@@ -104,9 +113,21 @@ def main():
     else:
         o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, refine=0, **common)
     o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, refine=0, **common)
+    o_bf16 = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
+                                  refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
     ch.load_caches(o_mixed)
     setup_s = time.perf_counter() - t0
     print(f"# setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+    if a.heff_only:
+        site = a.L // 2
+        s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
+        print(f"# heff bench site {site} dims {ch.dims()[0][site]}x{ch.dims()[1][site]}: stage1 {s1:.4f} ms "
+              f"stage3 {s3:.4f} ms mix {mix:.4f} ms", file=sys.stderr, flush=True)
+        n = 64 << 20  # 64 Mi doubles = 512 MiB read + 512 MiB written
+        ms = ctx.copy_bench(n)
+        print(f"# calibration copy (scale_copy_k): {n * 8} bytes read, {n * 8} bytes written, {ms:.3f} ms "
+              f"({2 * n * 8 / ms / 1e9:.0f} GB/s)", file=sys.stderr, flush=True)
+        return
     for w in range(a.warmup):
         ch.sweep(0, o_mixed)
         ch.sweep(1, o_mixed)
@@ -132,18 +153,36 @@ def main():
     s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
     print(f"# heff bench site {site}: stage1 {s1:.3f} ms stage3 {s3:.3f} ms mix {mix:.3f} ms",
           file=sys.stderr, flush=True)
-    res_fp64 = None
-    if saved is not None:
-        for i, (d1, d2, M) in enumerate(saved):
-            ch.set_site(i, d1, d2, M)
-        ch.load_caches(o_fp64)
-        st64 = tn.SweepStats()
+    def timed_from_saved(opts):
+        """K sweeps from the saved state (after one untimed sweep that tunes
+        the GEMM shapes of this precision when autotuning is on)."""
+        def restore():
+            for i, (d1, d2, M) in enumerate(saved):
+                ch.set_site(i, d1, d2, M)
+            ch.load_caches(opts)
+        restore()
+        if not a.no_autotune:
+            ch.sweep(0, opts)
+            ch.sweep(1, opts)
+            restore()
+        stx = tn.SweepStats()
         t1 = time.perf_counter()
         for k in range(a.steps):
-            ch.sweep(0, o_fp64, st64)
-            ch.sweep(1, o_fp64, st64)
-        wall64 = time.perf_counter() - t1
-        E64 = ch.expectation()
+            ch.sweep(0, opts, stx)
+            ch.sweep(1, opts, stx)
+        return stx, time.perf_counter() - t1, ch.expectation()
+
+    res_fp64 = None
+    res_bf16 = None
+    if saved is not None and a.precision != "mixed" and not a.no_bf16_compare:
+        stb, wallb, Eb = timed_from_saved(o_bf16)
+        res_bf16 = {"scheme": "BF16 contractions, FP32 environments, FP32-contraction refinement",
+                    "sweep_s": wallb / a.steps, "E_per_site": Eb / a.L, "n_matvec": stb.n_matvec,
+                    "heff_tflops": stb.heff_flops / max(stb.heff_ms, 1e-12) / 1e9}
+        print(f"# bf16+fp32 refine {wallb:.2f}s E/L={Eb / a.L:.12f} matvecs={stb.n_matvec}", file=sys.stderr,
+              flush=True)
+    if saved is not None:
+        st64, wall64, E64 = timed_from_saved(o_fp64)
         res_fp64 = {"sweep_s": wall64 / a.steps, "E_per_site": E64 / a.L,
                     "heff_tflops": st64.heff_flops / max(st64.heff_ms, 1e-12) / 1e9,
                     "n_matvec": st64.n_matvec, "dE_per_site": abs(E_mixed - E64) / a.L,
@@ -152,6 +191,8 @@ def main():
                     "ms_per_matvec": st64.heff_ms / max(st64.n_matvec, 1)}
         print(f"# fp64 {wall64:.2f}s E/L={E64 / a.L:.12f} |dE|/L={abs(E_mixed - E64) / a.L:.3e}",
               file=sys.stderr, flush=True)
+    if not a.no_autotune:
+        print("# tuned GEMM shapes:\n" + ctx.tune_report(), file=sys.stderr, flush=True)
     value = st.heff_flops / max(st.heff_ms, 1e-12) / 1e9
     achieved = f1 / (s1 * 1e-3) / 1e12
     hp = "bf16" if a.precision == "mixed" else a.precision
@@ -175,7 +216,7 @@ def main():
                                + "), FP64 Krylov/Ritz/gauge/energy",
                    "precision": a.precision,
                    "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov,
-                   "max_restarts": a.max_restarts, "gauge": a.gauge},
+                   "max_restarts": a.max_restarts, "gauge": a.gauge, "autotune": not a.no_autotune},
         "sweep_s": wall / a.steps,
         "E_per_site": E_mixed / a.L,
         "breakdown_ms": {"heff": st.heff_ms, "heff_gemm": st.heff_gemm_ms, "lanczos": st.lanczos_ms,
@@ -184,6 +225,7 @@ def main():
         "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64,
         "ms_per_matvec": st.heff_ms / max(st.n_matvec, 1),
         "fp64": res_fp64,
+        "bf16_then_fp32": res_bf16,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[hp], "unit": "TFLOP/s",
                      "frac": achieved / PEAKS[hp], "traffic": None,

@@ -42,6 +42,15 @@ enum tn_precision { TN_FP64 = 0, TN_FP32 = 1, TN_BF16 = 2 };
  * and scratch memory.  One context per chain or per batch, no globals. */
 int tn_ctx_create(int device, tn_ctx** ctx);
 int tn_ctx_destroy(tn_ctx* ctx);
+/* GEMM autotuning for this context (no reference counterpart: the reference's
+ * contractions are fixed loops).  When on, every matrix-core GEMM whose shape
+ * is seen for the first time is timed over the tile shapes and k-splits of the
+ * kernel menu (into scratch memory) and the fastest is kept for that shape.
+ * Off by default: results then do not depend on timing. */
+int tn_ctx_set_autotune(tn_ctx* ctx, int on);
+/* Text table of the tuned shapes ("f32->f64 KT M=.. N=.. K=.. : tile 128x64
+ * ksplit 2" per line) into buf (len bytes); returns the number of shapes. */
+int tn_ctx_tune_report(tn_ctx* ctx, char* buf, int len);
 const char* tn_last_error(const tn_ctx* ctx);
 int tn_version(void);
 /* number of visible HIP devices (0 when none or when the runtime fails) */
@@ -91,6 +100,7 @@ int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* dis
  * caches, which tn_chain_load_caches rebuilds. */
 int tn_chain_create(tn_ctx* ctx, int L, int d, const int* dw1, const int* dw2,
                     const double* wcat, int D, int Dcap, tn_chain** chain);
+/* Destroy a chain; call before tn_ctx_destroy of its context. */
 int tn_chain_destroy(tn_chain* chain);
 /* MPSSite::InitializeWith (TensorNetworksImp/MPS/MPSSite.C:84-140); state
  * 0 = Product, 1 = Random (seeded U(-1,1) / sqrt(d D1 D2)), 2 = Neel. */
@@ -183,9 +193,16 @@ int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, doub
  * (enum tn_precision values 0 fp64, 1 fp32, 2 bf16) and the result stored as
  * tout before conversion back to double.  a_kmajor: A[b] is M x K row-major
  * (else K x M); b_kmajor: B[b] is N x K row-major (else K x N).  M, N multiples
- * of 32, K of 32.  tile: 0 automatic, else a forced tile code. */
+ * of 32, K of 32.  tile: 0 automatic, else a forced tile code.  ksplit > 1
+ * splits K into that many partial products summed by a reduction kernel
+ * (batch 1, K a multiple of 64 ksplit). */
 int tn_gemm_test(tn_ctx* ctx, int tin, int tout, int a_kmajor, int b_kmajor, int M, int N, int K,
-                 int batch, const double* A, const double* B, double* C, int tile);
+                 int batch, const double* A, const double* B, double* C, int tile, int ksplit);
+
+/* Plain device copy of n doubles (read n*8 bytes, write n*8 bytes) by the
+ * library's streaming kernel, once, into freshly allocated buffers; *ms = its
+ * duration.  A byte-counter run uses it as the known-size calibration. */
+int tn_copy_bench(tn_ctx* ctx, int64_t n, double* ms);
 
 /* The QR gauge kernel on its own, for numerics tests: A (row-major m x n,
  * real, m >= n) = Q R with Q (m x n) orthonormal columns and R (n x n) upper

@@ -1,6 +1,7 @@
 """ctypes binding of include/tn_abi.h (libtn_hip.so)."""
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -88,8 +89,11 @@ def _load():
         "tn_eig_lowest": (i, [v, i64, i, d, i, _dp, _dp, _ip]),
         "tn_svd": (i, [v, i, i, _dp, _dp, _dp, _dp]),
         "tn_svcompress_rank": (i, [i, _dp, i, d, _dp, _dp]),
-        "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i]),
+        "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i, i]),
         "tn_cholqr_test": (i, [v, i, i, _dp, ctypes.c_double, _dp, _dp, _ip]),
+        "tn_ctx_set_autotune": (i, [v, i]),
+        "tn_copy_bench": (i, [v, ctypes.c_int64, _dp]),
+        "tn_ctx_tune_report": (i, [v, ctypes.c_char_p, i]),
         "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
         "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
         "tn_chain_destroy": (i, [v]),
@@ -148,14 +152,33 @@ class Context:
         if rc != 0:
             raise TnError(rc, f"tn_ctx_create(device={device}) failed")
         self.h = h
+        self._chains = weakref.WeakSet()
 
     def close(self):
+        """Destroy the context; chains created on it are destroyed first."""
         if getattr(self, "h", None):
+            for ch in list(getattr(self, "_chains", ())):
+                ch.close()
             lib.tn_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
         self.close()
+
+    def set_autotune(self, on=True):
+        """Per-shape GEMM tile / k-split autotuning for this context."""
+        self.check(lib.tn_ctx_set_autotune(self.h, int(bool(on))), "tn_ctx_set_autotune")
+
+    def copy_bench(self, n):
+        """Duration (ms) of one plain device copy of n doubles."""
+        ms = ctypes.c_double(0.0)
+        self.check(lib.tn_copy_bench(self.h, int(n), ctypes.byref(ms)), "tn_copy_bench")
+        return ms.value
+
+    def tune_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib.tn_ctx_tune_report(self.h, buf, len(buf))
+        return buf.value.decode()
 
     def check(self, rc, what):
         if rc < 0:
@@ -200,12 +223,12 @@ class Context:
         self.check(lib.tn_svd(self.h, m, n, P(Af), P(U), P(s), P(Vt)), "tn_svd")
         return U.reshape(k, m).T.copy(), s, Vt.reshape(n, k).T.copy()
 
-    def gemm_test(self, A, B, tin, tout, a_kmajor, b_kmajor, M, N, K, batch=1, tile=0):
+    def gemm_test(self, A, B, tin, tout, a_kmajor, b_kmajor, M, N, K, batch=1, tile=0, ksplit=1):
         A = np.ascontiguousarray(A, dtype=np.float64).ravel()
         B = np.ascontiguousarray(B, dtype=np.float64).ravel()
         C = np.zeros(batch * M * N)
         self.check(lib.tn_gemm_test(self.h, tin, tout, int(a_kmajor), int(b_kmajor), M, N, K, batch, P(A), P(B),
-                                    P(C), tile), "tn_gemm_test")
+                                    P(C), tile, ksplit), "tn_gemm_test")
         return C.reshape(batch, M, N)
 
     def cholqr_test(self, A, tol=1e-11):
@@ -243,10 +266,12 @@ class Chain:
         ctx.check(lib.tn_chain_create(ctx.h, self.L, self.d, IP(dw1), IP(dw2), P(wcat), D, Dcap or D, ctypes.byref(h)),
                   "tn_chain_create")
         self.h = h
+        ctx._chains.add(self)
 
     def close(self):
         if getattr(self, "h", None):
-            lib.tn_chain_destroy(self.h)
+            if getattr(self.ctx, "h", None):
+                lib.tn_chain_destroy(self.h)
             self.h = None
 
     def __del__(self):

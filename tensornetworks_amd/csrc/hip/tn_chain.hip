@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "tn_engine.h"
@@ -346,7 +347,7 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         g.M = n; g.N = d * D3p; g.K = n; g.batch = 1;
         g.A = Lacc; g.lda = n; g.B = ch->A[j]; g.ldb = (long)d * D3p;
         g.C = ch->spare; g.ldc = (long)d * D3p; g.alpha = 1.0f;
-        TN_HIP(gemm(g, s));
+        gemm_run(c, g);
         std::swap(ch->A[j], ch->spare);
         rescale_site(ch, j);  // keep the norm O(1) along the chain (see normalised_copy)
         ch->lam[i].assign(D2, 0.0);
@@ -375,7 +376,7 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
     g.A = ch->A[j]; g.lda = n; g.B = Lacc; g.ldb = n;
     g.C = ch->spare; g.ldc = n; g.alpha = 1.0f;
     // Lacc lives in ctx->spare, the product goes to the chain's spare buffer
-    TN_HIP(gemm(g, s));
+    gemm_run(c, g);
     std::swap(ch->A[j], ch->spare);
     rescale_site(ch, j);
     ch->lam[i - 1].assign(D1, 0.0);
@@ -433,7 +434,7 @@ void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge =
         g.A = c->svd_Vt.p; g.lda = D2p;
         g.B = ch->A[j]; g.ldb = (long)d * D3p;
         g.C = ch->spare; g.ldc = (long)d * D3p; g.alpha = 1.0f;
-        TN_HIP(gemm(g, s));
+        gemm_run(c, g);
         std::swap(ch->A[j], ch->spare);
         ch->D1[j] = k;
         // site i <- U  (rows D1p*d, cols kp, zero padded)
@@ -462,7 +463,7 @@ void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge =
         g.A = ch->A[j]; g.lda = D1p;
         g.B = c->svd_U.p; g.ldb = kp;
         g.C = ch->spare; g.ldc = kp; g.alpha = 1.0f;
-        TN_HIP(gemm(g, s));
+        gemm_run(c, g);
         std::swap(ch->A[j], ch->spare);
         ch->D2[j] = k;
         // site i <- Vt (k x d*D2) into [kp][d][D2p]
@@ -586,6 +587,30 @@ int tn_ctx_create(int device, tn_ctx** out)
     }
 }
 
+int tn_ctx_set_autotune(tn_ctx* c, int on)
+{
+    if (!c) return TN_ERR_ARG;
+    c->autotune = on != 0;
+    return TN_OK;
+}
+
+int tn_ctx_tune_report(tn_ctx* c, char* buf, int len)
+{
+    if (!c || !buf || len <= 0) return TN_ERR_ARG;
+    std::string out;
+    static const char* tn_[] = {"f64", "f32", "bf16"};
+    for (const auto& kv : c->tune) {
+        const auto& k = kv.first;
+        char line[192];
+        snprintf(line, sizeof(line), "%s->%s %c%c M=%ld N=%ld K=%ld batch=%ld acc=%ld : tile %s ksplit %d\n",
+                 tn_[k[0]], tn_[k[1]], k[2] ? 'K' : 'T', k[3] ? 'K' : 'T', k[4], k[5], k[6], k[7], k[8],
+                 gemm_tile_name(kv.second.first), kv.second.second);
+        out += line;
+    }
+    snprintf(buf, (size_t)len, "%s", out.c_str());
+    return (int)c->tune.size();
+}
+
 int tn_ctx_destroy(tn_ctx* c)
 {
     if (!c) return TN_OK;
@@ -593,8 +618,9 @@ int tn_ctx_destroy(tn_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     tn::DBuf* bufs[] = {&c->T1, &c->T2, &c->xlo, &c->part, &c->krylov, &c->lanczos_small, &c->svd_X,
                         &c->svd_V, &c->svd_U, &c->svd_Vt, &c->svd_s, &c->svd_perm, &c->svd_pairs,
-                        &c->svd_flag, &c->svd_A, &c->spare, &c->pack, &c->b_L, &c->b_R, &c->b_Llo,
-                        &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M};
+                        &c->svd_flag, &c->svd_A, &c->spare, &c->pack, &c->qr_W, &c->qr_G, &c->qr_L,
+                        &c->qr_T, &c->qr_linv, &c->qr_Q, &c->tune_C, &c->splitk, &c->b_L, &c->b_R,
+                        &c->b_Llo, &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M};
     for (auto* b : bufs) b->release();
     c->b_mix_heff.buf.release();
     c->b_mix_envl.buf.release();
@@ -1222,7 +1248,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         g.M = D1p * d; g.N = d * D3p; g.K = Dmp; g.batch = 1;
         g.A = ch->A[i]; g.lda = Dmp; g.B = ch->A[j]; g.ldb = (long)d * D3p;
         g.C = ch->theta.p; g.ldc = (long)d * D3p; g.alpha = 1.0f;
-        TN_HIP(gemm(g, s));
+        gemm_run(c, g);
         two_site_W(d, Dw1, Dwm, Dw3, ch->W[i].data(), ch->W[j].data(), ch->W12[i]);
         HeffCall hc;
         hc.op.p = p2;
@@ -1348,7 +1374,7 @@ int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, double*
 
 // ---- diagnostics --------------------------------------------------------------
 int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, int K, int batch,
-                 const double* A, const double* B, double* C, int tile)
+                 const double* A, const double* B, double* C, int tile, int ksplit)
 {
     if (!c || !A || !B || !C) return TN_ERR_ARG;
     ABI_BEGIN
@@ -1378,10 +1404,32 @@ int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, i
     g.B = Bd; g.ldb = bkm ? K : N; g.sB = (long)K * N;
     g.C = cout.p; g.ldc = N; g.sC = (long)M * N;
     g.alpha = 1.0f; g.tile = tile;
-    TN_HIP(gemm(g, c->stream));
+    gemm_split(c, g, ksplit);
     TN_HIP(convert(tout, cout.p, T_F64, c64.p, nC, c->stream));
     d2h(c, C, c64.p, nC * 8);
     for (auto* b : {&a64, &b64, &alo, &blo, &cout, &c64}) b->release();
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_copy_bench(tn_ctx* c, int64_t n, double* ms)
+{
+    if (!c || n <= 0) return TN_ERR_ARG;
+    ABI_BEGIN
+    tn::DBuf a, b;
+    a.ensure((size_t)n * 8);
+    b.ensure((size_t)n * 8);
+    TN_HIP(fill_value(a.as<double>(), 1.0, n, c->stream));
+    TN_HIP(scale_copy(1.0, a.as<double>(), b.as<double>(), n, c->stream));  // warm
+    TN_HIP(hipEventRecord(c->ev[6], c->stream));
+    TN_HIP(scale_copy(1.0, a.as<double>(), b.as<double>(), n, c->stream));
+    TN_HIP(hipEventRecord(c->ev[7], c->stream));
+    TN_HIP(hipEventSynchronize(c->ev[7]));
+    float t = 0;
+    TN_HIP(hipEventElapsedTime(&t, c->ev[6], c->ev[7]));
+    if (ms) *ms = t;
+    a.release();
+    b.release();
     return TN_OK;
     ABI_END(c)
 }

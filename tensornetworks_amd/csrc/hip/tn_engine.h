@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -73,6 +75,11 @@ struct tn_ctx {
         svd_perm, svd_pairs, svd_flag, svd_A, spare, pack, qr_W, qr_G, qr_L, qr_T, qr_linv, qr_Q;
     tn::Scalars sc;
     hipEvent_t ev[8] = {};
+    // GEMM autotuning (off unless requested): best (tile code, k-split) per
+    // problem shape, measured on first use
+    bool autotune = false;
+    std::map<std::array<long, 9>, std::pair<int, int>> tune;
+    tn::DBuf tune_C, splitk;
     // seam state (tn_site_bind)
     int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
     std::vector<double> b_W;
@@ -82,6 +89,13 @@ struct tn_ctx {
 };
 
 namespace tn {
+
+// GEMM on the context's stream.  With g.tile == 0 and ctx->autotune set, the
+// tile and k-split come from the per-shape table (tuned on first use by timing
+// every candidate into scratch memory); otherwise the built-in choice.
+void gemm_run(tn_ctx* c, const GemmDesc& g);
+// the same GEMM with an explicit k-split (ks > 1 needs batch == 1, K % (64 ks) == 0)
+void gemm_split(tn_ctx* c, const GemmDesc& g, int ks);
 
 // --- effective Hamiltonian operator (internal padded layout) -------------
 // x, y: [D1p][p][D2p]   L: [D1p][Dw1][D1p]   R: [Dw2][D2p][D2p]
@@ -120,6 +134,8 @@ struct LanczosResult {
     int n_matvec = 0;
     bool converged = false;
 };
+// lowest two eigenvalues of tridiag(b, a, b) (k x k) and the eigenvector z of the lowest
+void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z);
 // generic operator callback: y = A x (device fp64)
 typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
 // converged when |H x - theta x| < tol or < tol_rel*|theta| (tol_rel = 0: off)

@@ -52,6 +52,85 @@ void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes)
 //  HEFF : T1[a'][n][w'][b] -> T2[w][a'][m][b]      (sum over w', n)
 //  ENVR : X [a'][n][w'][b] -> Y [a'][w][m][b]      (sum over w', n)
 //  ENVL : X [a][w][n][b']  -> Y [a][m][w'][b']     (sum over w , n)
+// ---------------------------------------------------------------------------
+// GEMM dispatch with k-split and per-shape autotuning
+void gemm_split(tn_ctx* c, const GemmDesc& g, int ks)
+{
+    if (ks <= 1) {
+        TN_HIP(gemm(g, c->stream));
+        return;
+    }
+    TN_REQUIRE(g.batch == 1 && g.K % (64 * ks) == 0, TN_ERR_ARG, "gemm_split: bad k-split");
+    const int tacc = g.tin == T_F64 ? T_F64 : T_F32;
+    const size_t es = tacc == T_F64 ? 8 : 4;
+    c->splitk.ensure((size_t)ks * g.M * g.N * es);
+    GemmDesc p = g;
+    const int Kc = g.K / ks;
+    p.K = Kc;
+    p.batch = ks;
+    p.sA = g.a_kmajor ? Kc : (long)Kc * g.lda;
+    p.sB = g.b_kmajor ? Kc : (long)Kc * g.ldb;
+    p.tout = tacc;
+    p.C = c->splitk.p;
+    p.ldc = g.N;
+    p.sC = (long)g.M * g.N;
+    p.accumulate = 0;
+    TN_HIP(gemm(p, c->stream));
+    TN_HIP(splitk_reduce(tacc, c->splitk.p, ks, g.M, g.N, g.tout, g.C, g.ldc, g.accumulate, c->stream));
+}
+
+static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
+{
+    hipStream_t s = c->stream;
+    static const int bm[] = {128, 128, 64, 64, 32, 64, 32}, bn[] = {128, 64, 128, 64, 64, 32, 32};
+    const int nt = gemm_num_tiles();
+    const size_t es = dtype_size(g.tout);
+    const size_t span = (size_t)(g.batch - 1) * g.sC + (size_t)(g.M - 1) * g.ldc + g.N;
+    c->tune_C.ensure(span * es);
+    GemmDesc t = g;
+    t.C = c->tune_C.p;
+    hipEvent_t e0, e1;
+    TN_HIP(hipEventCreate(&e0));
+    TN_HIP(hipEventCreate(&e1));
+    std::pair<int, int> best{0, 1};
+    float best_ms = 1e30f;
+    for (int code = 1; code <= nt && code <= 7; code++) {
+        if (g.M % bm[code - 1] || g.N % bn[code - 1]) continue;
+        for (int ks : {1, 2, 4}) {
+            if (ks > 1 && (g.batch != 1 || g.K % (64 * ks))) continue;
+            t.tile = code;
+            gemm_split(c, t, ks);  // warm
+            TN_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 3; r++) gemm_split(c, t, ks);
+            TN_HIP(hipEventRecord(e1, s));
+            TN_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            TN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = {code, ks};
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return best;
+}
+
+void gemm_run(tn_ctx* c, const GemmDesc& g)
+{
+    if (g.tile != 0 || !c->autotune) {
+        TN_HIP(gemm(g, c->stream));
+        return;
+    }
+    const std::array<long, 9> key{g.tin, g.tout, g.a_kmajor, g.b_kmajor, g.M, g.N, g.K, g.batch, g.accumulate};
+    auto it = c->tune.find(key);
+    if (it == c->tune.end()) it = c->tune.emplace(key, tune_gemm(c, g)).first;
+    GemmDesc t = g;
+    t.tile = it->second.first;
+    gemm_split(c, t, it->second.second);
+}
+
 void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
                hipStream_t s)
 {
@@ -177,13 +256,13 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         c->T2.ensure(n2 * 8);
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
         g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R; g1.C = c->T1.p;
-        TN_HIP(gemm(g1, s));
+        gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
         TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
                     op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
         g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L; g3.B = c->T2.p; g3.C = y;
-        TN_HIP(gemm(g3, s));
+        gemm_run(c, g3);
     } else {
         const int tl = lo_type(op.precision);
         const size_t es = dtype_size(tl);
@@ -193,13 +272,13 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         TN_HIP(convert(T_F64, x, tl, c->xlo.p, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
         g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = op.Rlo; g1.C = c->T1.p;
-        TN_HIP(gemm(g1, s));
+        gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
         TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
                     op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
         g3.tin = tl; g3.tout = T_F64; g3.A = op.Llo; g3.B = c->T2.p; g3.C = y;
-        TN_HIP(gemm(g3, s));
+        gemm_run(c, g3);
     }
     if (timed) TN_HIP(hipEventRecord(c->ev[4], s));
 }
@@ -253,11 +332,11 @@ void env_left(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, co
         c->T1.ensure(nX * 8);
         c->T2.ensure(nY * 8);
         e1.tin = T_F64; e1.tout = T_F64; e1.A = Lin; e1.B = A; e1.C = c->T1.p;
-        TN_HIP(gemm(e1, s));
+        gemm_run(c, e1);
         TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)Dw1 * p * D2p, c->T2.p, (long)p * Dw2 * D2p, D1p,
                     D2p, mix.t, s));
         e3.tin = T_F64; e3.tout = T_F64; e3.A = A; e3.B = c->T2.p; e3.C = Lout;
-        TN_HIP(gemm(e3, s));
+        gemm_run(c, e3);
     } else {
         const int tl = lo_type(prec);
         const size_t es = dtype_size(tl);
@@ -266,11 +345,11 @@ void env_left(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, co
         c->T2.ensure(nY * es);
         TN_HIP(convert(T_F64, A, tl, c->xlo.p, nA, s));
         e1.tin = tl; e1.tout = T_F32; e1.A = Lin_lo; e1.B = c->xlo.p; e1.C = c->T1.p;
-        TN_HIP(gemm(e1, s));
+        gemm_run(c, e1);
         TN_HIP(wmix(T_F32, tl, c->T1.p, (long)Dw1 * p * D2p, c->T2.p, (long)p * Dw2 * D2p, D1p,
                     D2p, mix.t, s));
         e3.tin = tl; e3.tout = T_F64; e3.A = c->xlo.p; e3.B = c->T2.p; e3.C = Lout;
-        TN_HIP(gemm(e3, s));
+        gemm_run(c, e3);
     }
 }
 
@@ -295,11 +374,11 @@ void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, c
         c->T1.ensure(nX * 8);
         c->T2.ensure(nY * 8);
         f1.tin = T_F64; f1.tout = T_F64; f1.A = A; f1.B = Rin; f1.C = c->T1.p;
-        TN_HIP(gemm(f1, s));
+        gemm_run(c, f1);
         TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)Dw1 * p * D2p, D1p,
                     D2p, mix.t, s));
         f3.tin = T_F64; f3.tout = T_F64; f3.A = A; f3.B = c->T2.p; f3.C = Rout;
-        TN_HIP(gemm(f3, s));
+        gemm_run(c, f3);
     } else {
         const int tl = lo_type(prec);
         const size_t es = dtype_size(tl);
@@ -308,48 +387,153 @@ void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, c
         c->T2.ensure(nY * es);
         TN_HIP(convert(T_F64, A, tl, c->xlo.p, nA, s));
         f1.tin = tl; f1.tout = T_F32; f1.A = c->xlo.p; f1.B = Rin_lo; f1.C = c->T1.p;
-        TN_HIP(gemm(f1, s));
+        gemm_run(c, f1);
         TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)Dw1 * p * D2p, D1p,
                     D2p, mix.t, s));
         f3.tin = tl; f3.tout = T_F64; f3.A = c->xlo.p; f3.B = c->T2.p; f3.C = Rout;
-        TN_HIP(gemm(f3, s));
+        gemm_run(c, f3);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Lanczos with full (two-pass classical Gram-Schmidt) re-orthogonalisation.
-// Krylov vectors, tridiagonal solve, Ritz vector and residual stay in FP64 on
-// the device; the host reads 4 scalars per iteration for loop control.
+// Host tridiagonal Ritz solve (k <= a few hundred): the two lowest eigenvalues
+// of T = tridiag(b, a, b) by Sturm-sequence bisection, the eigenvector of the
+// lowest by inverse iteration with a partially pivoted LU of T - t0 I.
+namespace {
+int sturm_count(int k, const double* a, const double* b2, double x)
+{
+    int cnt = 0;
+    double q = a[0] - x;
+    if (q < 0) cnt++;
+    for (int i = 1; i < k; i++) {
+        if (q == 0.0) q = 1e-300;
+        q = a[i] - x - b2[i - 1] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+}  // namespace
+
+void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z)
+{
+    std::vector<double> b2(std::max(k, 1));
+    for (int i = 0; i + 1 < k; i++) b2[i] = b[i] * b[i];
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < k; i++) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < k - 1 ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    const double span = hi - lo + 1e-300;
+    lo -= 1e-10 * span + 1e-300;
+    hi += 1e-10 * span + 1e-300;
+    auto eig = [&](int m) {
+        double l = lo, h = hi;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (l + h);
+            if (mid <= l || mid >= h) break;
+            if (sturm_count(k, a, b2.data(), mid) > m) h = mid;
+            else l = mid;
+        }
+        return 0.5 * (l + h);
+    };
+    *t0 = eig(0);
+    *t1 = k > 1 ? eig(1) : *t0;
+    if (k == 1) {
+        z[0] = 1.0;
+        return;
+    }
+    double scale = 0.0;
+    for (int i = 0; i < k; i++) scale = std::max(scale, std::fabs(a[i]) + (i < k - 1 ? std::fabs(b[i]) : 0.0));
+    const double pert = (scale > 0 ? scale : 1.0) * 1e-14;
+    std::vector<double> dd(k), du(k), dl(k), du2(k), rhs(k), zz(k);
+    std::vector<unsigned char> piv(k);
+    for (int i = 0; i < k; i++) zz[i] = 1.0 / std::sqrt((double)k) * (1.0 + 0.01 * ((i * 7919) % 13) / 13.0);
+    for (int iter = 0; iter < 3; iter++) {
+        for (int i = 0; i < k; i++) {
+            dd[i] = a[i] - *t0;
+            du2[i] = 0.0;
+            if (i < k - 1) du[i] = dl[i] = b[i];
+        }
+        for (int i = 0; i < k - 1; i++) {
+            if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+                piv[i] = 0;
+                if (dd[i] == 0.0) dd[i] = pert;
+                const double f = dl[i] / dd[i];
+                dl[i] = f;
+                dd[i + 1] -= f * du[i];
+            } else {
+                piv[i] = 1;
+                const double f = dd[i] / dl[i];
+                dd[i] = dl[i];
+                dl[i] = f;
+                const double tmp = du[i];
+                du[i] = dd[i + 1];
+                dd[i + 1] = tmp - f * dd[i + 1];
+                if (i < k - 2) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
+            }
+        }
+        if (dd[k - 1] == 0.0) dd[k - 1] = pert;
+        for (int i = 0; i < k; i++) rhs[i] = zz[i];
+        for (int i = 0; i < k - 1; i++) {
+            if (piv[i] == 0) {
+                rhs[i + 1] -= dl[i] * rhs[i];
+            } else {
+                const double t = rhs[i];
+                rhs[i] = rhs[i + 1];
+                rhs[i + 1] = t - dl[i] * rhs[i + 1];
+            }
+        }
+        rhs[k - 1] /= dd[k - 1];
+        rhs[k - 2] = (rhs[k - 2] - du[k - 2] * rhs[k - 1]) / dd[k - 2];
+        for (int i = k - 3; i >= 0; i--) rhs[i] = (rhs[i] - du[i] * rhs[i + 1] - du2[i] * rhs[i + 2]) / dd[i];
+        double nr = 0.0;
+        for (int i = 0; i < k; i++) nr += rhs[i] * rhs[i];
+        nr = std::sqrt(nr);
+        for (int i = 0; i < k; i++) zz[i] = rhs[i] / nr;
+    }
+    for (int i = 0; i < k; i++) z[i] = zz[i];
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos with full re-orthogonalisation: the three-term step removes the
+// large components of H v_j (alpha, beta), then one classical Gram-Schmidt
+// pass against every Krylov vector removes the rounding residue, which is
+// small, so a single pass keeps the basis orthonormal to working precision
+// (a single pass on the raw H v_j would not: its error grows by |H v_j|/beta
+// per step).  Krylov vectors, Ritz vector and residual stay in FP64 on the
+// device; per iteration the host reads the coefficients and the norm (one
+// sync) and solves the tridiagonal problem itself.
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
                       int max_krylov, int max_restarts, double tol_rel)
 {
     hipStream_t s = c->stream;
     LanczosResult res;
-    int kmax = std::max(2, std::min(max_krylov, 500));
+    const int kmax = std::max(2, std::min(max_krylov, 500));
     c->krylov.ensure((size_t)(kmax + 1) * N * 8);
     double* V = c->krylov.as<double>();
     c->lanczos_small.ensure(sizeof(double) * (4 * 512 + 64));
-    double* alpha = c->lanczos_small.as<double>();
-    double* beta = alpha + 512;
-    double* h = beta + 512;
-    double* z = h + 512;
-    double* out = z + 512;
-    long pcap = mdot_part_size(N, kmax + 1);
+    double* h = c->lanczos_small.as<double>();  // [0, 512): coefficients, [512, 1024): Ritz coefficients
+    double* zd = h + 512;
+    double* nrm = h + 1024;
+    const long pcap = mdot_part_size(N, kmax + 1);
     c->part.ensure(pcap * 8);
     double* part = c->part.as<double>();
-    double* hs = c->sc.h;
+    std::vector<double> al(kmax), be(kmax), z(kmax), hh(kmax + 4);
 
-    auto norm_to = [&](const double* v, double* dst) {
-        TN_HIP(mdot(v, N, 1, v, N, dst, part, (int)pcap, s));
-        TN_HIP(dsqrt_scalar(dst, dst, s));
+    auto norm_host = [&](const double* v) {
+        TN_HIP(mdot(v, N, 1, v, N, nrm, part, (int)pcap, s));
+        double v2 = 0;
+        d2h(c, &v2, nrm, 8);
+        return std::sqrt(v2);
     };
 
     for (int restart = 0; restart <= max_restarts; restart++) {
-        // v0 = x / |x|
-        norm_to(x, out + 8);
-        TN_HIP(hipMemcpyAsync(hs, out + 8, 8, hipMemcpyDeviceToHost, s));
-        sync(c);
-        if (!(hs[0] > 0.0) || !std::isfinite(hs[0])) {
+        double nx = norm_host(x);
+        if (!(nx > 0.0) || !std::isfinite(nx)) {
             // zero or non-finite start: seeded pseudo-random start vector
             std::vector<double> r(N);
             uint64_t st = 0x5eedULL + restart;
@@ -362,10 +546,9 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
                 r[i] = (double)(zz >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
             }
             h2d(c, x, r.data(), N * 8);
-            norm_to(x, out + 8);
+            nx = norm_host(x);
         }
-        TN_HIP(hipMemcpyAsync(V, x, N * 8, hipMemcpyDeviceToDevice, s));
-        TN_HIP(scale_by_inv(V, out + 8, N, s));
+        TN_HIP(scale_copy(1.0 / nx, x, V, N, s));
         int k = 0;
         bool conv = false;
         for (int j = 0; j < kmax; j++) {
@@ -373,32 +556,35 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
             double* w = V + (long)(j + 1) * N;
             fn(c, op, vj, w);
             res.n_matvec++;
-            TN_HIP(mdot(V, N, j + 1, w, N, h, part, (int)pcap, s));
-            TN_HIP(dcopy_scalar(h + j, alpha + j, s));
-            TN_HIP(gemv_acc(V, N, j + 1, h, -1.0, 1.0, w, N, s));
-            TN_HIP(mdot(V, N, j + 1, w, N, h, part, (int)pcap, s));
-            TN_HIP(gemv_acc(V, N, j + 1, h, -1.0, 1.0, w, N, s));
-            norm_to(w, beta + j);
-            TN_HIP(tridiag_ritz(alpha, beta, j + 1, z, out, s));
-            TN_HIP(hipMemcpyAsync(hs, out, 4 * 8, hipMemcpyDeviceToHost, s));
-            sync(c);
+            // three-term step with the exact alpha = v_j.w (h[0] on the device),
+            // then one classical Gram-Schmidt pass against every Krylov vector
+            // (h[1..j+1]) removes what rounding left; h[j+2] = |w|^2
+            TN_HIP(mdot(vj, N, 1, w, N, h, part, (int)pcap, s));
+            TN_HIP(three_term(w, vj, j > 0 ? V + (long)(j - 1) * N : nullptr, h, j > 0 ? be[j - 1] : 0.0, N, s));
+            TN_HIP(mdot(V, N, j + 1, w, N, h + 1, part, (int)pcap, s));
+            TN_HIP(gemv_acc(V, N, j + 1, h + 1, -1.0, 1.0, w, N, s));
+            TN_HIP(mdot(w, N, 1, w, N, h + j + 2, part, (int)pcap, s));
+            d2h(c, hh.data(), h, (size_t)(j + 3) * 8);
+            const double alpha = hh[0] + hh[j + 1];
+            const double nw2 = hh[j + 2];
+            al[j] = alpha;
+            be[j] = std::sqrt(std::max(nw2, 0.0));
+            tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, &res.e1, z.data());
             k = j + 1;
-            res.e0 = hs[0];
-            res.e1 = hs[1];
-            res.residual = hs[2];
-            const double b = hs[3];
+            res.residual = std::fabs(be[j] * z[j]);
             if (!std::isfinite(res.e0)) throw TnError{TN_ERR_STATE, "Lanczos: non-finite Ritz value"};
             if (res.residual < tol || res.residual < tol_rel * std::fabs(res.e0) ||
-                b < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
+                be[j] < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
                 conv = true;
                 break;
             }
-            if (j + 1 < kmax) TN_HIP(scale_by_inv(w, beta + j, N, s));
+            if (j + 1 < kmax) TN_HIP(scale_copy(1.0 / be[j], w, w, N, s));
         }
         // Ritz vector x = sum_i z_i v_i, normalised
-        TN_HIP(gemv_acc(V, N, k, z, 1.0, 0.0, x, N, s));
-        norm_to(x, out + 8);
-        TN_HIP(scale_by_inv(x, out + 8, N, s));
+        h2d(c, zd, z.data(), (size_t)k * 8);
+        TN_HIP(gemv_acc(V, N, k, zd, 1.0, 0.0, x, N, s));
+        const double nr = norm_host(x);
+        TN_HIP(scale_copy(1.0 / nr, x, x, N, s));
         if (conv) {
             res.converged = true;
             break;
@@ -555,7 +741,7 @@ static void qr_gram(tn_ctx* c, const double* A, long lda, int m, int n, double* 
     g.tin = T_F64; g.tout = T_F64; g.a_kmajor = false; g.b_kmajor = false;
     g.M = n; g.N = n; g.K = m; g.batch = 1;
     g.A = A; g.lda = lda; g.B = A; g.ldb = lda; g.C = G; g.ldc = n; g.alpha = 1.0f;
-    TN_HIP(gemm(g, c->stream));
+    gemm_run(c, g);
 }
 
 // G (destroyed) = L L^T; L and Linv lower triangular n x n.  A pivot <= 0
@@ -577,14 +763,14 @@ static void qr_factor(tn_ctx* c, double* G, int n, int nl, double shift, double*
         p.A = G + (size_t)r0 * n + k0; p.lda = n;
         p.B = Linv + (size_t)k0 * n + k0; p.ldb = n;
         p.C = L + (size_t)r0 * n + k0; p.ldc = n; p.alpha = 1.0f;
-        TN_HIP(gemm(p, s));
+        gemm_run(c, p);
         GemmDesc t{};  // G[r0:, r0:] -= Lp Lp^T
         t.tin = T_F64; t.tout = T_F64; t.a_kmajor = true; t.b_kmajor = true;
         t.M = rem; t.N = rem; t.K = nbk; t.batch = 1;
         t.A = L + (size_t)r0 * n + k0; t.lda = n;
         t.B = L + (size_t)r0 * n + k0; t.ldb = n;
         t.C = G + (size_t)r0 * n + r0; t.ldc = n; t.alpha = -1.0f; t.accumulate = 1;
-        TN_HIP(gemm(t, s));
+        gemm_run(c, t);
     }
     // recursive doubling: at the start of level b every aligned b-block (and
     // the shorter tail block) of Linv holds the inverse of that block of L
@@ -602,14 +788,14 @@ static void qr_factor(tn_ctx* c, double* G, int n, int nl, double shift, double*
             x.A = L + (o + b) * n + o; x.lda = n; x.sA = 2L * b * n + 2L * b;
             x.B = Linv + o * n + o; x.ldb = n; x.sB = 2L * b * n + 2L * b;
             x.C = X; x.ldc = b; x.sC = (long)b * b; x.alpha = 1.0f;
-            TN_HIP(gemm(x, s));
+            gemm_run(c, x);
             GemmDesc y{};  // Linv21 = -L22^-1 X
             y.tin = T_F64; y.tout = T_F64; y.a_kmajor = true; y.b_kmajor = false;
             y.M = s2; y.N = b; y.K = s2; y.batch = np;
             y.A = Linv + (o + b) * n + o + b; y.lda = n; y.sA = 2L * b * n + 2L * b;
             y.B = X; y.ldb = b; y.sB = (long)b * b;
             y.C = Linv + (o + b) * n + o; y.ldc = n; y.sC = 2L * b * n + 2L * b; y.alpha = -1.0f;
-            TN_HIP(gemm(y, s));
+            gemm_run(c, y);
         }
     }
 }
@@ -621,7 +807,7 @@ static void qr_apply(tn_ctx* c, const double* A, long lda, int m, int n, const d
     q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = true;
     q.M = m; q.N = n; q.K = n; q.batch = 1;
     q.A = A; q.lda = lda; q.B = Linv; q.ldb = n; q.C = Q; q.ldc = n; q.alpha = 1.0f;
-    TN_HIP(gemm(q, c->stream));
+    gemm_run(c, q);
 }
 
 // Lacc <- Lacc Lk  (A = Q_prev Lacc^T = Q Lk^T Lacc^T)
@@ -631,15 +817,17 @@ static void qr_accumulate(tn_ctx* c, double* Lacc, const double* Lk, int n)
     g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
     g.M = n; g.N = n; g.K = n; g.batch = 1;
     g.A = Lacc; g.lda = n; g.B = Lk; g.ldb = n; g.C = c->qr_T.p; g.ldc = n; g.alpha = 1.0f;
-    TN_HIP(gemm(g, c->stream));
+    gemm_run(c, g);
     TN_HIP(hipMemcpyAsync(Lacc, c->qr_T.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, c->stream));
 }
 
-// Shifted CholeskyQR: pass 1 factors A^T A directly and, if a pivot breaks
-// down (cond(A) beyond ~1e8), again with the shift 11 (m n + n (n+1)) u ||A||_F^2
-// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42
-// (2020) A477), which makes cond(Q1) moderate; passes 2 (and 3 when needed)
-// restore orthogonality.  The result is accepted when max|Q^T Q - I| <= tol.
+// Shifted CholeskyQR: every pass factors X^T X directly and, if a pivot
+// breaks down (cond(X) beyond ~1e8), again with the shift
+// 11 (m n + n (n+1)) u ||X||_F^2 (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+// Yanagisawa, SIAM J. Sci. Comput. 42 (2020) A477), which keeps cond of the
+// pass's output moderate.  At least two passes run; from the third on the Gram
+// matrix of the current Q is checked first and the result accepted when
+// max|Q^T Q - I| <= tol.  At most five passes.
 bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* Lacc,
                    double tol, int* shifted)
 {
@@ -659,41 +847,41 @@ bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, d
     double* Q1 = c->qr_Q.as<double>();
     int* flag = c->svd_flag.as<int>();
     double* dev = c->svd_s.as<double>();
+    const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * std::ldexp(1.0, -53);
+    const int max_pass = 5;
     if (shifted) *shifted = 0;
-    // pass 1: A -> Q1, Lacc
-    TN_HIP(set_int(flag, 0, s));
-    qr_gram(c, A, lda, m, n, G);
-    qr_factor(c, G, n, nl, 0.0, Lacc, Linv, flag);
-    int hflag = 0;
-    d2h(c, &hflag, flag, sizeof(int));
-    if (hflag) {
-        const double u = std::ldexp(1.0, -53);
-        const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * u;
+    const double* X = A;
+    long ldx = lda;
+    for (int pass = 1;; pass++) {
+        qr_gram(c, X, ldx, m, n, G);
+        if (pass >= 3) {
+            TN_HIP(eye_deviation(G, n, nl, dev, c->part.as<double>(), s));
+            double h = 0;
+            d2h(c, &h, dev, 8);
+            if (std::isfinite(h) && h <= tol) {
+                if (X != Q) TN_HIP(hipMemcpyAsync(Q, X, (size_t)m * n * 8, hipMemcpyDeviceToDevice, s));
+                return true;
+            }
+            if (pass > max_pass) return false;
+        }
+        double* Lp = pass == 1 ? Lacc : Lk;
         TN_HIP(set_int(flag, 0, s));
-        qr_gram(c, A, lda, m, n, G);
-        qr_factor(c, G, n, nl, shift, Lacc, Linv, flag);
-        if (shifted) *shifted = 1;
-    }
-    qr_apply(c, A, lda, m, n, Linv, Q1);
-    // pass 2: Q1 -> Q
-    qr_gram(c, Q1, n, m, n, G);
-    qr_factor(c, G, n, nl, 0.0, Lk, Linv, flag);
-    qr_apply(c, Q1, n, m, n, Linv, Q);
-    qr_accumulate(c, Lacc, Lk, n);
-    for (int pass = 3;; pass++) {
-        qr_gram(c, Q, n, m, n, G);
-        TN_HIP(eye_deviation(G, n, nl, dev, c->part.as<double>(), s));
-        double h = 0;
-        d2h(c, &h, dev, 8);
+        qr_factor(c, G, n, nl, 0.0, Lp, Linv, flag);
+        int hflag = 0;
         d2h(c, &hflag, flag, sizeof(int));
-        if (hflag || !std::isfinite(h)) return false;
-        if (h <= tol) return true;
-        if (pass > 3) return false;
-        // pass 3 on the Gram matrix just formed
-        qr_factor(c, G, n, nl, 0.0, Lk, Linv, flag);
-        qr_apply(c, Q, n, m, n, Linv, Q1);
-        TN_HIP(hipMemcpyAsync(Q, Q1, (size_t)m * n * 8, hipMemcpyDeviceToDevice, s));
-        qr_accumulate(c, Lacc, Lk, n);
+        if (hflag) {
+            TN_HIP(set_int(flag, 0, s));
+            qr_gram(c, X, ldx, m, n, G);
+            qr_factor(c, G, n, nl, shift, Lp, Linv, flag);
+            d2h(c, &hflag, flag, sizeof(int));
+            if (hflag) return false;
+            if (shifted) *shifted = 1;
+        }
+        double* Y = X == Q1 ? Q : Q1;
+        qr_apply(c, X, ldx, m, n, Linv, Y);
+        if (pass > 1) qr_accumulate(c, Lacc, Lk, n);
+        X = Y;
+        ldx = n;
     }
 }
 

@@ -31,6 +31,10 @@ struct GemmDesc {
     int tile;  // 0 = automatic, else forced tile code (autotuner)
 };
 hipError_t gemm(const GemmDesc& g, hipStream_t s);
+// split-K epilogue: C[i*ldc + j] = (accumulate ? C : 0) + sum_{z<ks} W[z][i][j]
+// (W: ks x M x N of type tacc, T_F64 or T_F32)
+hipError_t splitk_reduce(int tacc, const void* W, int ks, int M, int N, int tout, void* C, long ldc, int accumulate,
+                         hipStream_t s);
 int gemm_num_tiles();
 const char* gemm_tile_name(int code);
 
@@ -74,11 +78,12 @@ hipError_t absmax(const double* x, long n, double* out, double* part, int part_c
 // v[n] *= 1/den[0] (den on device)
 hipError_t scale_by_inv(double* v, const double* den, long n, hipStream_t s);
 hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStream_t s);
+// w -= alpha[0] v + beta u (u may be null); alpha is a device scalar
+hipError_t three_term(double* w, const double* v, const double* u, const double* alpha, double beta, long n,
+                      hipStream_t s);
+// y = a x (y is not read; x and y may be the same buffer)
+hipError_t scale_copy(double a, const double* x, double* y, long n, hipStream_t s);
 
-// tridiagonal Ritz solve on the device:
-// out[0]=theta0, out[1]=theta1, out[2]=residual=|beta[k-1]*z[k-1]|, out[3]=beta[k-1]; z (k)
-hipError_t tridiag_ritz(const double* alpha, const double* beta, int k, double* z,
-                        double* out, hipStream_t s);
 // alpha[j] = h[j]   (copy a device scalar)
 hipError_t dcopy_scalar(const double* src, double* dst, hipStream_t s);
 // dst[0] = sqrt(src[0])

@@ -8,7 +8,6 @@
 // * wmix_k     : applies the sparse MPO tensor W[w][w'][m][n] between the GEMMs
 //                (coalesced along the trailing bond index).
 // * mdot/gemv  : Lanczos re-orthogonalisation (fp64, deterministic reductions).
-// * tridiag_k  : Sturm multisection + inverse iteration for the Ritz pair.
 // * jacobi_k   : one block pair of a one-sided block-Jacobi SVD (fp64).
 #include "tn_kernels.h"
 
@@ -355,6 +354,54 @@ hipError_t gemm(const GemmDesc& g, hipStream_t s)
     return gemm_layout<false, true>(g, s);
 }
 
+// split-K reduction: C[i*ldc + j] (+)= sum_z W[z*M*N + i*N + j]
+template <typename TA, typename TO>
+__global__ void __launch_bounds__(256) splitk_reduce_k(const TA* __restrict__ W, int ks, int M, int N,
+                                                       TO* __restrict__ C, long ldc, int accumulate)
+{
+    const long MN = (long)M * N;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < MN; e += (long)gridDim.x * 256) {
+        const long i = e / N, j = e - i * N;
+        if constexpr (std::is_same<TA, double>::value) {
+            double v = 0.0;
+            for (int z = 0; z < ks; z++) v += W[z * MN + e];
+            TO* cp = C + i * ldc + j;
+            if (accumulate) v += to_d(*cp);
+            *cp = cvt_from_d<TO>(v);
+        } else {
+            float v = 0.0f;
+            for (int z = 0; z < ks; z++) v += W[z * MN + e];
+            TO* cp = C + i * ldc + j;
+            if constexpr (std::is_same<TO, double>::value) {
+                double dv = (double)v;
+                if (accumulate) dv += *cp;
+                *cp = dv;
+            } else {
+                if (accumulate) v += to_f(*cp);
+                *cp = cvt_from_f<TO>(v);
+            }
+        }
+    }
+}
+
+hipError_t splitk_reduce(int tacc, const void* W, int ks, int M, int N, int tout, void* C, long ldc, int accumulate,
+                         hipStream_t s)
+{
+    long blocks = ((long)M * N + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+#define SK_CASE(TAc, TOc, TAt, TOt)                                                                  \
+    if (tacc == TAc && tout == TOc) {                                                                \
+        splitk_reduce_k<TAt, TOt><<<blocks, 256, 0, s>>>((const TAt*)W, ks, M, N, (TOt*)C, ldc, accumulate); \
+        return hipGetLastError();                                                                    \
+    }
+    SK_CASE(T_F64, T_F64, double, double)
+    SK_CASE(T_F32, T_F32, float, float)
+    SK_CASE(T_F32, T_F64, float, double)
+    SK_CASE(T_F32, T_BF16, float, bf16)
+#undef SK_CASE
+    return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 // MPO mixing
 template <typename TI, typename TO>
@@ -666,6 +713,38 @@ hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStre
     return hipGetLastError();
 }
 
+// w -= alpha[0] v + beta u   (the three-term Lanczos step; alpha on the device)
+__global__ void three_term_k(double* __restrict__ w, const double* __restrict__ v, const double* __restrict__ u,
+                             const double* __restrict__ alpha, double beta, long n)
+{
+    const double a = alpha[0];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        double r = w[i] - a * v[i];
+        if (u) r -= beta * u[i];
+        w[i] = r;
+    }
+}
+hipError_t three_term(double* w, const double* v, const double* u, const double* alpha, double beta, long n,
+                      hipStream_t s)
+{
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    three_term_k<<<blocks, 256, 0, s>>>(w, v, u, alpha, beta, n);
+    return hipGetLastError();
+}
+
+__global__ void scale_copy_k(double a, const double* x, double* y, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = a * x[i];
+}
+hipError_t scale_copy(double a, const double* x, double* y, long n, hipStream_t s)
+{
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    scale_copy_k<<<blocks, 256, 0, s>>>(a, x, y, n);
+    return hipGetLastError();
+}
+
 __global__ void dcopy_scalar_k(const double* src, double* dst)
 {
     if (threadIdx.x == 0) dst[0] = src[0];
@@ -686,152 +765,6 @@ hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// tridiagonal Ritz solve: one wavefront.  Sturm multisection gives the two
-// lowest eigenvalues; inverse iteration (LU with partial pivoting, one lane)
-// gives the Ritz coefficients of the lowest.
-#define TD_MAXK 512
-
-__device__ int sturm_count_d(int k, const double* a, const double* b2, double x)
-{
-    int cnt = 0;
-    double q = a[0] - x;
-    if (q < 0) cnt++;
-    for (int i = 1; i < k; i++) {
-        if (q == 0.0) q = 1e-300;
-        q = a[i] - x - b2[i - 1] / q;
-        if (q < 0) cnt++;
-    }
-    return cnt;
-}
-
-__device__ double multisect(int k, const double* a, const double* b2, double lo, double hi,
-                            int idx, int lane, double* sh_lo, double* sh_hi)
-{
-    for (int it = 0; it < 16; it++) {
-        const double step = (hi - lo) / 65.0;
-        const double x = lo + step * (lane + 1);
-        const int above = sturm_count_d(k, a, b2, x) > idx;  // x > lambda_idx
-        // smallest lane with 'above'
-        unsigned long long mask = __ballot(above);
-        int first = mask ? __ffsll((long long)mask) - 1 : 64;
-        double nlo = (first == 0) ? lo : lo + step * first;
-        double nhi = (first == 64) ? hi : lo + step * (first + 1);
-        lo = nlo;
-        hi = nhi;
-        if (!(hi - lo > 0.0)) break;
-    }
-    (void)sh_lo;
-    (void)sh_hi;
-    return 0.5 * (lo + hi);
-}
-
-__global__ void __launch_bounds__(64) tridiag_k(const double* __restrict__ alpha,
-                                                const double* __restrict__ beta, int k,
-                                                double* __restrict__ z, double* __restrict__ out)
-{
-    __shared__ double a[TD_MAXK], b[TD_MAXK], b2[TD_MAXK];
-    __shared__ double s_dd[TD_MAXK], s_du[TD_MAXK], s_dl[TD_MAXK], s_du2[TD_MAXK];
-    __shared__ double s_rhs[TD_MAXK], s_zz[TD_MAXK];
-    __shared__ unsigned char s_piv[TD_MAXK];
-    __shared__ double th[2];
-    const int lane = threadIdx.x;
-    for (int i = lane; i < k; i += 64) {
-        a[i] = alpha[i];
-        b[i] = beta[i];
-        b2[i] = beta[i] * beta[i];
-    }
-    __syncthreads();
-    // Gershgorin bounds (all lanes compute the same)
-    double lo = a[0], hi = a[0];
-    for (int i = 0; i < k; i++) {
-        double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i < k - 1 ? fabs(b[i]) : 0.0);
-        lo = fmin(lo, a[i] - r);
-        hi = fmax(hi, a[i] + r);
-    }
-    const double span = hi - lo + 1e-300;
-    lo -= 1e-10 * span + 1e-300;
-    hi += 1e-10 * span + 1e-300;
-    double t0 = multisect(k, a, b2, lo, hi, 0, lane, nullptr, nullptr);
-    double t1 = (k > 1) ? multisect(k, a, b2, lo, hi, 1, lane, nullptr, nullptr) : t0;
-    if (lane == 0) {
-        th[0] = t0;
-        th[1] = t1;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        // inverse iteration for the eigenvector at t0 (work arrays in LDS)
-        double* dd = s_dd; double* du = s_du; double* dl = s_dl; double* du2 = s_du2;
-        double* rhs = s_rhs; double* zz = s_zz; unsigned char* piv = s_piv;
-        double scale = 0.0;
-        for (int i = 0; i < k; i++) scale = fmax(scale, fabs(a[i]) + (i < k - 1 ? fabs(b[i]) : 0.0));
-        const double pert = (scale > 0 ? scale : 1.0) * 1e-14;
-        for (int i = 0; i < k; i++) zz[i] = 1.0 / sqrt((double)k) * (1.0 + 0.01 * ((i * 7919) % 13) / 13.0);
-        if (k == 1) zz[0] = 1.0;
-        for (int iter = 0; iter < 3 && k > 1; iter++) {
-            for (int i = 0; i < k; i++) {
-                dd[i] = a[i] - t0;
-                du2[i] = 0.0;
-                if (i < k - 1) {
-                    du[i] = b[i];
-                    dl[i] = b[i];
-                }
-            }
-            for (int i = 0; i < k - 1; i++) {
-                if (fabs(dd[i]) >= fabs(dl[i])) {
-                    piv[i] = 0;
-                    if (dd[i] == 0.0) dd[i] = pert;
-                    double f = dl[i] / dd[i];
-                    dl[i] = f;
-                    dd[i + 1] -= f * du[i];
-                } else {
-                    piv[i] = 1;
-                    double f = dd[i] / dl[i];
-                    dd[i] = dl[i];
-                    dl[i] = f;
-                    double tmp = du[i];
-                    du[i] = dd[i + 1];
-                    dd[i + 1] = tmp - f * dd[i + 1];
-                    if (i < k - 2) {
-                        du2[i] = du[i + 1];
-                        du[i + 1] = -f * du[i + 1];
-                    }
-                }
-            }
-            if (dd[k - 1] == 0.0) dd[k - 1] = pert;
-            for (int i = 0; i < k; i++) rhs[i] = zz[i];
-            for (int i = 0; i < k - 1; i++) {
-                if (piv[i] == 0) rhs[i + 1] -= dl[i] * rhs[i];
-                else {
-                    double t = rhs[i];
-                    rhs[i] = rhs[i + 1];
-                    rhs[i + 1] = t - dl[i] * rhs[i + 1];
-                }
-            }
-            rhs[k - 1] /= dd[k - 1];
-            rhs[k - 2] = (rhs[k - 2] - du[k - 2] * rhs[k - 1]) / dd[k - 2];
-            for (int i = k - 3; i >= 0; i--)
-                rhs[i] = (rhs[i] - du[i] * rhs[i + 1] - du2[i] * rhs[i + 2]) / dd[i];
-            double nr = 0.0;
-            for (int i = 0; i < k; i++) nr += rhs[i] * rhs[i];
-            nr = sqrt(nr);
-            for (int i = 0; i < k; i++) zz[i] = rhs[i] / nr;
-        }
-        for (int i = 0; i < k; i++) z[i] = zz[i];
-        out[0] = th[0];
-        out[1] = th[1];
-        out[2] = fabs(b[k - 1] * zz[k - 1]);
-        out[3] = b[k - 1];
-    }
-}
-
-hipError_t tridiag_ritz(const double* alpha, const double* beta, int k, double* z, double* out,
-                        hipStream_t s)
-{
-    if (k < 1 || k > TD_MAXK) return hipErrorInvalidValue;
-    tridiag_k<<<1, 64, 0, s>>>(alpha, beta, k, z, out);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // one-sided block Jacobi SVD: one workgroup per column-block pair (2 x 16
 // columns).  The 32x32 Gram matrix is diagonalised in LDS by cyclic parallel
@@ -1101,67 +1034,85 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 // symmetric positive definite G in LDS (one wavefront) and writes L_kk (lower)
 // and its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
 #define CB 64
-__global__ void __launch_bounds__(256) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
-                                                   double* __restrict__ L, long ldl,
-                                                   double* __restrict__ Linv, long ldi,
-                                                   int* __restrict__ flag)
+// One wavefront, everything in LDS.  Left-looking (Crout) Cholesky: step j
+// has lane r >= j form A[r][j] - sum_{k<j} L[r][k] L[j][k] (L[j][k] is a
+// broadcast read, L[r][k] conflict-free across lanes), then lane j takes the
+// square root and the others divide.  The inverse follows row by row with the
+// same access pattern.  Dot products use eight partial sums so the LDS reads
+// pipeline.  (A register-resident variant needs the steps unrolled, and the
+// unrolled triangle does not fit the register file.)
+__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
+                                                  double* __restrict__ L, long ldl,
+                                                  double* __restrict__ Linv, long ldi,
+                                                  int* __restrict__ flag)
 {
-    __shared__ double A[CB][CB + 1];
-    __shared__ double X[CB][CB + 1];
+    __shared__ double Ls[CB][CB + 1];
+    __shared__ double Xs[CB][CB + 1];
     __shared__ int bad;
-    const int t = threadIdx.x;
-    if (t == 0) bad = 0;
-    // a block narrower than CB (the last one) is completed with the identity
-    for (int e = t; e < CB * CB; e += 256) {
-        const int i = e / CB, j = e % CB;
-        A[i][j] = (i < nbk && j < nbk) ? G[(long)(k0 + i) * ldg + k0 + j] : (i == j ? 1.0 : 0.0);
+    const int r = threadIdx.x;
+    if (r == 0) bad = 0;
+    // coalesced load; a block narrower than CB (the last one) is completed
+    // with the identity
+    for (int e = r; e < CB * CB; e += 64) {
+        const int i = e / CB, c = e % CB;
+        Ls[i][c] = (i < nbk && c < nbk) ? G[(long)(k0 + i) * ldg + k0 + c] : (i == c ? 1.0 : 0.0);
     }
     __syncthreads();
-    // right-looking Cholesky on the lower triangle, one column per step
     for (int j = 0; j < CB; j++) {
-        if (t == 0) {
-            const double dj = A[j][j];
-            if (!(dj > 0.0) || !isfinite(dj)) {
-                bad = 1;
-                A[j][j] = 1.0;
-            } else {
-                A[j][j] = sqrt(dj);
+        double sp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (r >= j) {
+            int k = 0;
+            for (; k + 7 < j; k += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) sp[u] += Ls[r][k + u] * Ls[j][k + u];
             }
+            for (; k < j; k++) sp[0] += Ls[r][k] * Ls[j][k];
+        }
+        const double v = Ls[r][j] - (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])));
+        __syncthreads();
+        if (r == j) {
+            double d = v;
+            if (!(d > 0.0) || !isfinite(d)) {
+                bad = 1;
+                d = 1.0;
+            } else {
+                d = sqrt(d);
+            }
+            Ls[j][j] = d;
         }
         __syncthreads();
-        if (t > j && t < CB) A[t][j] /= A[j][j];
-        __syncthreads();
-        const int rem = CB - 1 - j;
-        for (int e = t; e < rem * rem; e += 256) {
-            const int r = j + 1 + e / rem, cc = j + 1 + e % rem;
-            if (cc <= r) A[r][cc] -= A[r][j] * A[cc][j];
-        }
+        if (r > j) Ls[r][j] = v / Ls[j][j];
         __syncthreads();
     }
-    // inverse of the lower factor: thread c < CB solves L x = e_c
-    if (t < CB) {
-        const int c = t;
-        for (int i = 0; i < CB; i++) {
-            double v = (i == c) ? 1.0 : 0.0;
-            for (int kk = c; kk < i; kk++) v -= A[i][kk] * X[kk][c];
-            X[i][c] = (i < c) ? 0.0 : v / A[i][i];
+    // X = L^-1 row by row: lane c forms X[i][c]; X[k][c] = 0 for k < c, so
+    // every lane runs the same k range
+    for (int i = 0; i < CB; i++) {
+        double sp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int k = 0;
+        for (; k + 7 < i; k += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) sp[u] += Ls[i][k + u] * Xs[k + u][r];
+        }
+        for (; k < i; k++) sp[0] += Ls[i][k] * Xs[k][r];
+        const double v = (i == r ? 1.0 : 0.0) - (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])));
+        Xs[i][r] = (r <= i) ? v / Ls[i][i] : 0.0;
+        __syncthreads();
+    }
+    for (int e = r; e < CB * CB; e += 64) {
+        const int i = e / CB, c = e % CB;
+        if (i < nbk && c < nbk) {
+            L[(long)(k0 + i) * ldl + k0 + c] = (c <= i) ? Ls[i][c] : 0.0;
+            Linv[(long)(k0 + i) * ldi + k0 + c] = Xs[i][c];
         }
     }
-    __syncthreads();
-    for (int e = t; e < CB * CB; e += 256) {
-        const int i = e / CB, j = e % CB;
-        if (i >= nbk || j >= nbk) continue;
-        L[(long)(k0 + i) * ldl + k0 + j] = (j <= i) ? A[i][j] : 0.0;
-        Linv[(long)(k0 + i) * ldi + k0 + j] = X[i][j];
-    }
-    if (t == 0 && bad) flag[0] = 1;
+    if (r == 0 && bad) flag[0] = 1;
 }
 
 hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
                      int* flag, hipStream_t s)
 {
     if (nbk <= 0 || nbk > CB) return hipErrorInvalidValue;
-    chol_diag_k<<<1, 256, 0, s>>>(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag);
+    chol_diag_k<<<1, 64, 0, s>>>(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag);
     return hipGetLastError();
 }
 

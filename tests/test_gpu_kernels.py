@@ -65,6 +65,27 @@ def test_gemm_every_tile(ctx, tin, tile):
     assert err < (1e-15 if tin == 0 else 2e-6), (tile, err)
 
 
+@pytest.mark.parametrize("ks", [2, 4])
+@pytest.mark.parametrize("tin,tout", [(0, 0), (1, 1), (1, 0), (2, 1), (2, 0)])
+@pytest.mark.parametrize("akm,bkm", [(True, True), (True, False), (False, False)])
+def test_gemm_split_k(ctx, tin, tout, akm, bkm, ks):
+    # K split into ks partial products reduced by a second kernel: the same
+    # product as the plain GEMM to the operand precision
+    M, N, K = 96, 160, 512
+    A = rand((1, M, K), 5)
+    B = rand((1, K, N), 6)
+    ref = ROUND[tin](A) @ ROUND[tin](B)
+    Ain = A if akm else np.ascontiguousarray(A.transpose(0, 2, 1))
+    Bin = np.ascontiguousarray(B.transpose(0, 2, 1)) if bkm else B
+    C = ctx.gemm_test(Ain, Bin, tin, tout, akm, bkm, M, N, K, 1, 0, ks)
+    # summation-order rounding: ~sqrt(K) u |C|, |C| ~ sqrt(K)/3
+    err = np.abs(C - ref).max() / np.sqrt(K)
+    assert err < (1e-14 if tin == 0 else 2e-6), (tin, tout, ks, err)
+    # accumulate=0 semantics: a second call gives the same result
+    C2 = ctx.gemm_test(Ain, Bin, tin, tout, akm, bkm, M, N, K, 1, 0, ks)
+    assert np.array_equal(C, C2)
+
+
 def sym_env(Dw, D, seed):
     """Environment whose every w-slice is symmetric (reference layout)."""
     F = rand((Dw, D, D), seed)
