@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--gauge", default="qr", choices=["qr", "svd"])
     p.add_argument("--no-fp64-compare", action="store_true")
     p.add_argument("--no-autotune", action="store_true", help="built-in GEMM tile choice instead of per-shape tuning")
+    p.add_argument("--tune-db", default=os.path.join(ROOT, "tensornetworks_amd", "tuning", "mi355x_gemm.txt"),
+                   help="saved GEMM tuning table to start from (shapes not in it are tuned on first use)")
+    p.add_argument("--tune-save", default=None, help="write the final tuning table here")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--out", default=None)
@@ -97,6 +100,10 @@ def main():
     ctx = tn.Context(0)
     if not a.no_autotune:
         ctx.set_autotune(True)
+        if a.tune_db and os.path.exists(a.tune_db):
+            n = ctx.tune_load(a.tune_db)
+            print(f"# loaded {n} tuned GEMM shapes from {os.path.relpath(a.tune_db, ROOT)}", file=sys.stderr,
+                  flush=True)
     Ws = xxz_mpo(a.L, a.S, Dz=a.Dz)
     ch = tn.Chain(ctx, Ws, a.D)
     ch.initialize(1, a.seed)
@@ -123,10 +130,12 @@ def main():
         s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
         print(f"# heff bench site {site} dims {ch.dims()[0][site]}x{ch.dims()[1][site]}: stage1 {s1:.4f} ms "
               f"stage3 {s3:.4f} ms mix {mix:.4f} ms", file=sys.stderr, flush=True)
+        if not a.no_autotune:
+            print("# tuned GEMM shapes:\n" + ctx.tune_report(), file=sys.stderr, flush=True)
         n = 64 << 20  # 64 Mi doubles = 512 MiB read + 512 MiB written
         ms = ctx.copy_bench(n)
         print(f"# calibration copy (scale_copy_k): {n * 8} bytes read, {n * 8} bytes written, {ms:.3f} ms "
-              f"({2 * n * 8 / ms / 1e9:.0f} GB/s)", file=sys.stderr, flush=True)
+              f"({2 * n * 8 / ms / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
         return
     for w in range(a.warmup):
         ch.sweep(0, o_mixed)
@@ -191,8 +200,28 @@ def main():
                     "ms_per_matvec": st64.heff_ms / max(st64.n_matvec, 1)}
         print(f"# fp64 {wall64:.2f}s E/L={E64 / a.L:.12f} |dE|/L={abs(E_mixed - E64) / a.L:.3e}",
               file=sys.stderr, flush=True)
+    stage1_tile = None
     if not a.no_autotune:
-        print("# tuned GEMM shapes:\n" + ctx.tune_report(), file=sys.stderr, flush=True)
+        report = ctx.tune_report()
+        print("# tuned GEMM shapes:\n" + report, file=sys.stderr, flush=True)
+        if a.tune_save:
+            ctx.tune_save(a.tune_save)
+        # tile of the bulk-site stage-1 GEMM: x (D*d x D) times R^T (Dw*D x D)
+        tname = {"fp32": "f32->f32", "fp64": "f64->f64"}.get(a.precision, "bf16->f32")
+        key = f"{tname} KK M={a.D * d} N={5 * a.D} K={a.D} "
+        for line in report.splitlines():
+            if line.startswith(key):
+                stage1_tile = line.split(": tile ")[1]
+    # HBM bytes per launch of the stage-1 GEMM from the committed byte-counter
+    # summary, used only when it describes the same kernel configuration
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "heff_stage1_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            tr = json.load(f)
+        if (tr.get("precision") == a.precision and tr.get("D") == a.D and tr.get("d") == d
+                and tr.get("tile") == stage1_tile):
+            traffic = {"bytes": tr["bytes_per_launch"], "source": "profiles/heff_stage1_traffic.json"}
     value = st.heff_flops / max(st.heff_ms, 1e-12) / 1e9
     achieved = f1 / (s1 * 1e-3) / 1e12
     hp = "bf16" if a.precision == "mixed" else a.precision
@@ -228,7 +257,8 @@ def main():
         "bf16_then_fp32": res_bf16,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[hp], "unit": "TFLOP/s",
-                     "frac": achieved / PEAKS[hp], "traffic": None,
+                     "frac": achieved / PEAKS[hp], "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None, "stage1_tile": stage1_tile,
                      "kernel": "Heff stage-1 GEMM (psi x R, NT) at the bulk site",
                      "per_launch_flops": f1, "avg_launch_ms": s1,
                      "stage3_ms": s3, "stage3_flops": f3, "mpo_stage_ms": mix,

@@ -51,6 +51,11 @@ int tn_ctx_set_autotune(tn_ctx* ctx, int on);
 /* Text table of the tuned shapes ("f32->f64 KT M=.. N=.. K=.. : tile 128x64
  * ksplit 2" per line) into buf (len bytes); returns the number of shapes. */
 int tn_ctx_tune_report(tn_ctx* ctx, char* buf, int len);
+/* Tuning table persistence: one line per shape, "tin tout akm bkm M N K batch
+ * accumulate tile ksplit" (integers).  load merges into the context's table
+ * (returns the number of entries read, < 0 on error); save writes it. */
+int tn_ctx_tune_load(tn_ctx* ctx, const char* path);
+int tn_ctx_tune_save(tn_ctx* ctx, const char* path);
 const char* tn_last_error(const tn_ctx* ctx);
 int tn_version(void);
 /* number of visible HIP devices (0 when none or when the runtime fails) */

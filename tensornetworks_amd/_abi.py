@@ -93,6 +93,8 @@ def _load():
         "tn_cholqr_test": (i, [v, i, i, _dp, ctypes.c_double, _dp, _dp, _ip]),
         "tn_ctx_set_autotune": (i, [v, i]),
         "tn_copy_bench": (i, [v, ctypes.c_int64, _dp]),
+        "tn_ctx_tune_load": (i, [v, ctypes.c_char_p]),
+        "tn_ctx_tune_save": (i, [v, ctypes.c_char_p]),
         "tn_ctx_tune_report": (i, [v, ctypes.c_char_p, i]),
         "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
         "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
@@ -168,6 +170,13 @@ class Context:
     def set_autotune(self, on=True):
         """Per-shape GEMM tile / k-split autotuning for this context."""
         self.check(lib.tn_ctx_set_autotune(self.h, int(bool(on))), "tn_ctx_set_autotune")
+
+    def tune_load(self, path):
+        """Merge a saved tuning table; returns the number of entries."""
+        return self.check(lib.tn_ctx_tune_load(self.h, os.fsencode(path)), "tn_ctx_tune_load")
+
+    def tune_save(self, path):
+        return self.check(lib.tn_ctx_tune_save(self.h, os.fsencode(path)), "tn_ctx_tune_save")
 
     def copy_bench(self, n):
         """Duration (ms) of one plain device copy of n doubles."""

@@ -611,6 +611,44 @@ int tn_ctx_tune_report(tn_ctx* c, char* buf, int len)
     return (int)c->tune.size();
 }
 
+int tn_ctx_tune_load(tn_ctx* c, const char* path)
+{
+    if (!c || !path) return TN_ERR_ARG;
+    FILE* f = fopen(path, "r");
+    if (!f) {
+        c->err = std::string("tn_ctx_tune_load: cannot open ") + path;
+        return TN_ERR_ARG;
+    }
+    int n = 0;
+    long v[11];
+    while (fscanf(f, "%ld %ld %ld %ld %ld %ld %ld %ld %ld %ld %ld", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                  &v[7], &v[8], &v[9], &v[10]) == 11) {
+        if (v[9] < 1 || v[9] > gemm_num_tiles() || (v[10] != 1 && v[10] != 2 && v[10] != 4)) continue;
+        const std::array<long, 9> key{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
+        c->tune[key] = {(int)v[9], (int)v[10]};
+        n++;
+    }
+    fclose(f);
+    return n;
+}
+
+int tn_ctx_tune_save(tn_ctx* c, const char* path)
+{
+    if (!c || !path) return TN_ERR_ARG;
+    FILE* f = fopen(path, "w");
+    if (!f) {
+        c->err = std::string("tn_ctx_tune_save: cannot open ") + path;
+        return TN_ERR_ARG;
+    }
+    for (const auto& kv : c->tune) {
+        const auto& k = kv.first;
+        fprintf(f, "%ld %ld %ld %ld %ld %ld %ld %ld %ld %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], k[6], k[7], k[8],
+                kv.second.first, kv.second.second);
+    }
+    fclose(f);
+    return (int)c->tune.size();
+}
+
 int tn_ctx_destroy(tn_ctx* c)
 {
     if (!c) return TN_OK;

@@ -1051,11 +1051,23 @@ __global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, 
     __shared__ int bad;
     const int r = threadIdx.x;
     if (r == 0) bad = 0;
-    // coalesced load; a block narrower than CB (the last one) is completed
-    // with the identity
-    for (int e = r; e < CB * CB; e += 64) {
-        const int i = e / CB, c = e % CB;
-        Ls[i][c] = (i < nbk && c < nbk) ? G[(long)(k0 + i) * ldg + k0 + c] : (i == c ? 1.0 : 0.0);
+    // coalesced load, 16 independent loads in flight per lane before the LDS
+    // stores (a loop of load-then-store would pay one memory latency per
+    // row); a block narrower than CB (the last one) is completed with the
+    // identity
+#pragma unroll
+    for (int e0 = 0; e0 < CB * CB; e0 += 64 * 16) {
+        double tmp[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = e0 + u * 64 + r, i = e / CB, c = e % CB;
+            tmp[u] = (i < nbk && c < nbk) ? G[(long)(k0 + i) * ldg + k0 + c] : (i == c ? 1.0 : 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = e0 + u * 64 + r;
+            Ls[e / CB][e % CB] = tmp[u];
+        }
     }
     __syncthreads();
     for (int j = 0; j < CB; j++) {
