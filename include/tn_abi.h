@@ -204,6 +204,23 @@ int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, doub
 int tn_gemm_test(tn_ctx* ctx, int tin, int tout, int a_kmajor, int b_kmajor, int M, int N, int K,
                  int batch, const double* A, const double* B, double* C, int tile, int ksplit);
 
+/* ---- batched chains (no reference counterpart: the reference sweeps one
+ * MPS at a time) ----
+ * A batch advances many chains of one context in lockstep: pass dir (0 left
+ * to right, 1 right to left) gives round t to site t (or L-1-t) of every chain
+ * with L > t, and every stage of a round is one grouped launch over the
+ * chains (grouped persistent GEMMs with per-shape tiles, batched Lanczos
+ * vector kernels, batched CholeskyQR).  The per-site algorithm is that of
+ * tn_chain_sweep with gauge = 1 (QR) and FP32 contractions: opts must have
+ * precision = env_precision = TN_FP32 and max_krylov <= the batch's.  The
+ * chains stay owned by the caller, need their caches loaded (FP32 or FP64
+ * environments) and must outlive the batch.  energies (optional, n doubles)
+ * receives each chain's last Ritz value. */
+typedef struct tn_batch tn_batch;
+int tn_batch_create(tn_ctx* ctx, int n, tn_chain* const* chains, int max_krylov, tn_batch** batch);
+int tn_batch_destroy(tn_batch* batch);
+int tn_batch_sweep(tn_batch* batch, int dir, const tn_sweep_opts* opts, tn_sweep_stats* stats, double* energies);
+
 /* Plain device copy of n doubles (read n*8 bytes, write n*8 bytes) by the
  * library's streaming kernel, once, into freshly allocated buffers; *ms = its
  * duration.  A byte-counter run uses it as the known-size calibration. */

@@ -9,6 +9,7 @@ from ._abi import (  # noqa: F401
     TN_BF16,
     TN_FP32,
     TN_FP64,
+    Batch,
     Chain,
     Context,
     SweepOpts,
@@ -21,7 +22,7 @@ from .models import xxz_mpo, mpo_square, identity_mpo  # noqa: F401
 from .dmrg import FindVariationalGroundState, ScheduleLine  # noqa: F401
 
 __all__ = [
-    "Context", "Chain", "SweepOpts", "SweepStats", "TnError", "lib", "LIB_PATH",
+    "Context", "Chain", "Batch", "SweepOpts", "SweepStats", "TnError", "lib", "LIB_PATH",
     "TN_FP64", "TN_FP32", "TN_BF16", "svcompress_rank", "xxz_mpo", "mpo_square",
     "identity_mpo", "FindVariationalGroundState", "ScheduleLine",
 ]

@@ -94,6 +94,9 @@ def _load():
         "tn_ctx_set_autotune": (i, [v, i]),
         "tn_copy_bench": (i, [v, ctypes.c_int64, _dp]),
         "tn_ctx_tune_load": (i, [v, ctypes.c_char_p]),
+        "tn_batch_create": (i, [v, i, ctypes.POINTER(ctypes.c_void_p), i, vp]),
+        "tn_batch_destroy": (i, [v]),
+        "tn_batch_sweep": (i, [v, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
         "tn_ctx_tune_save": (i, [v, ctypes.c_char_p]),
         "tn_ctx_tune_report": (i, [v, ctypes.c_char_p, i]),
         "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
@@ -364,3 +367,36 @@ class Chain:
         self.ctx.check(lib.tn_chain_expectation(self.h, IP(dw1), IP(dw2), P(wcat), ctypes.byref(v)),
                        "tn_chain_expectation")
         return v.value
+
+
+class Batch:
+    """Many chains of one context swept in lockstep (tn_batch_*): every stage
+    of a round is one grouped launch over the chains.  FP32 contractions, QR
+    gauge; the chains must have their caches loaded and outlive the batch."""
+
+    def __init__(self, ctx, chains, max_krylov=32):
+        self.ctx = ctx
+        self.chains = list(chains)
+        arr = (ctypes.c_void_p * len(self.chains))(*[c.h.value for c in self.chains])
+        h = ctypes.c_void_p()
+        ctx.check(lib.tn_batch_create(ctx.h, len(self.chains), arr, int(max_krylov), ctypes.byref(h)),
+                  "tn_batch_create")
+        self.h = h
+        ctx._chains.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if getattr(self.ctx, "h", None):
+                lib.tn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def sweep(self, direction, opts, stats=None):
+        """One pass of every chain; returns each chain's last Ritz value."""
+        e = np.zeros(len(self.chains))
+        self.ctx.check(lib.tn_batch_sweep(self.h, direction, ctypes.byref(opts),
+                                          ctypes.byref(stats) if stats is not None else None, P(e)),
+                       "tn_batch_sweep")
+        return e

@@ -1,0 +1,50 @@
+// tn_chain.h -- the device-resident chain and the single-site steps shared by
+// the one-chain sweep (tn_chain.hip) and the lockstep batch (tn_batch.hip).
+#pragma once
+#include "tn_engine.h"
+
+using tn::DBuf;
+using tn::DevMix;
+
+struct tn_chain {
+    tn_ctx* ctx = nullptr;
+    int L = 0, d = 0, Dcap = 0, Dcapp = 0, Dwmax = 0;
+    std::vector<int> dw1, dw2;
+    std::vector<std::vector<double>> W;
+    std::vector<int> D1, D2;
+    size_t site_cap = 0, env_cap = 0;
+    DBuf sites_buf, spare_buf, env_buf, bnd_buf;
+    std::vector<double*> A, Lenv, Renv;
+    // reduced-precision copies of the environments, one set per precision
+    // (index TN_FP32 / TN_BF16); each set is refreshed whenever an FP64
+    // environment changes
+    struct LoSet {
+        bool active = false;
+        DBuf buf;
+        std::vector<void*> L, R;
+        void *bL = nullptr, *bR = nullptr;
+    } lo[3];
+    double *bL = nullptr, *bR = nullptr;  // boundary environments, fp64
+    std::vector<DevMix> mixH, mixL, mixR;
+    // bonds b = 0..L-2
+    std::vector<std::vector<double>> lam;
+    std::vector<double> maxDelta, compErr;
+    std::vector<double> Emin, IterDE, GapE;
+    double* spare = nullptr;
+    // two-site scratch
+    DBuf theta, w12buf;
+    std::vector<std::vector<double>> W12;
+    std::vector<DevMix> mix2H;
+};
+
+namespace tn {
+double now_ms();
+void ensure_lo(tn_chain* ch, int prec);
+const void* lo_left(tn_chain* ch, int prec, int i);
+const void* lo_right(tn_chain* ch, int prec, int i);
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile);
+void rescale_site(tn_chain* ch, int i);
+void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge);
+void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_stats* st);
+void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1);
+}  // namespace tn

@@ -14,42 +14,12 @@
 #include <cstdio>
 #include <cstring>
 
-#include "tn_engine.h"
+#include "tn_chain.h"
 
 using namespace tn;
 
-struct tn_chain {
-    tn_ctx* ctx = nullptr;
-    int L = 0, d = 0, Dcap = 0, Dcapp = 0, Dwmax = 0;
-    std::vector<int> dw1, dw2;
-    std::vector<std::vector<double>> W;
-    std::vector<int> D1, D2;
-    size_t site_cap = 0, env_cap = 0;
-    DBuf sites_buf, spare_buf, env_buf, bnd_buf;
-    std::vector<double*> A, Lenv, Renv;
-    // reduced-precision copies of the environments, one set per precision
-    // (index TN_FP32 / TN_BF16); each set is refreshed whenever an FP64
-    // environment changes
-    struct LoSet {
-        bool active = false;
-        DBuf buf;
-        std::vector<void*> L, R;
-        void *bL = nullptr, *bR = nullptr;
-    } lo[3];
-    double *bL = nullptr, *bR = nullptr;  // boundary environments, fp64
-    std::vector<DevMix> mixH, mixL, mixR;
-    // bonds b = 0..L-2
-    std::vector<std::vector<double>> lam;
-    std::vector<double> maxDelta, compErr;
-    std::vector<double> Emin, IterDE, GapE;
-    double* spare = nullptr;
-    // two-site scratch
-    DBuf theta, w12buf;
-    std::vector<std::vector<double>> W12;
-    std::vector<DevMix> mix2H;
-};
 
-namespace {
+namespace tn {
 
 int fail(tn_ctx* c, const TnError& e)
 {
@@ -548,7 +518,7 @@ void chain_reserve(tn_chain* ch, int Dcap)
     sync(c);
 }
 
-}  // namespace
+}  // namespace tn
 
 // ===========================================================================
 // C ABI

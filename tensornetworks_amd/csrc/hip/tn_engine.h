@@ -96,6 +96,9 @@ namespace tn {
 void gemm_run(tn_ctx* c, const GemmDesc& g);
 // the same GEMM with an explicit k-split (ks > 1 needs batch == 1, K % (64 ks) == 0)
 void gemm_split(tn_ctx* c, const GemmDesc& g, int ks);
+// (tile code, k-split) gemm_run would use for g (tuning it first when
+// autotuning is on and the shape is new)
+std::pair<int, int> gemm_choice(tn_ctx* c, const GemmDesc& g);
 
 // --- effective Hamiltonian operator (internal padded layout) -------------
 // x, y: [D1p][p][D2p]   L: [D1p][Dw1][D1p]   R: [Dw2][D2p][D2p]
@@ -135,7 +138,8 @@ struct LanczosResult {
     bool converged = false;
 };
 // lowest two eigenvalues of tridiag(b, a, b) (k x k) and the eigenvector z of the lowest
-void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z);
+// (t1 only when want_t1; else it is set to t0)
+void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z, bool want_t1 = true);
 // generic operator callback: y = A x (device fp64)
 typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
 // converged when |H x - theta x| < tol or < tol_rel*|theta| (tol_rel = 0: off)

@@ -117,6 +117,16 @@ static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
     return best;
 }
 
+std::pair<int, int> gemm_choice(tn_ctx* c, const GemmDesc& g)
+{
+    if (g.tile != 0) return {g.tile, 1};
+    const std::array<long, 9> key{g.tin, g.tout, g.a_kmajor, g.b_kmajor, g.M, g.N, g.K, g.batch, g.accumulate};
+    auto it = c->tune.find(key);
+    if (it != c->tune.end()) return it->second;
+    if (!c->autotune) return {gemm_default_tile(g.M, g.N, g.batch), 1};
+    return c->tune.emplace(key, tune_gemm(c, g)).first->second;
+}
+
 void gemm_run(tn_ctx* c, const GemmDesc& g)
 {
     if (g.tile != 0 || !c->autotune) {
@@ -414,7 +424,7 @@ int sturm_count(int k, const double* a, const double* b2, double x)
 }
 }  // namespace
 
-void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z)
+void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z, bool want_t1)
 {
     std::vector<double> b2(std::max(k, 1));
     for (int i = 0; i + 1 < k; i++) b2[i] = b[i] * b[i];
@@ -438,7 +448,7 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
         return 0.5 * (l + h);
     };
     *t0 = eig(0);
-    *t1 = k > 1 ? eig(1) : *t0;
+    if (t1) *t1 = (want_t1 && k > 1) ? eig(1) : *t0;
     if (k == 1) {
         z[0] = 1.0;
         return;
@@ -569,12 +579,15 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
             const double nw2 = hh[j + 2];
             al[j] = alpha;
             be[j] = std::sqrt(std::max(nw2, 0.0));
-            tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, &res.e1, z.data());
+            tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, nullptr, z.data(), false);
             k = j + 1;
             res.residual = std::fabs(be[j] * z[j]);
             if (!std::isfinite(res.e0)) throw TnError{TN_ERR_STATE, "Lanczos: non-finite Ritz value"};
-            if (res.residual < tol || res.residual < tol_rel * std::fabs(res.e0) ||
-                be[j] < 1e-14 * std::max(1.0, std::fabs(res.e0))) {
+            const bool done = res.residual < tol || res.residual < tol_rel * std::fabs(res.e0) ||
+                              be[j] < 1e-14 * std::max(1.0, std::fabs(res.e0));
+            if (done || j + 1 == kmax)  // the second Ritz value is only reported
+                tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, &res.e1, z.data(), true);
+            if (done) {
                 conv = true;
                 break;
             }

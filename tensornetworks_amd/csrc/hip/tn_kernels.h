@@ -31,6 +31,28 @@ struct GemmDesc {
     int tile;  // 0 = automatic, else forced tile code (autotuner)
 };
 hipError_t gemm(const GemmDesc& g, hipStream_t s);
+
+// One problem of a grouped GEMM (device-resident descriptor): C = alpha A B
+// (+ C), layouts and types shared by the whole group.
+struct GemmProb {
+    const void* A;
+    const void* B;
+    void* C;
+    long lda, ldb, ldc;
+    int M, N, K;
+    float alpha;
+    int accumulate;
+};
+// One persistent launch over all output tiles of nprob problems; off (device,
+// nprob+1) is the prefix sum of per-problem tile counts (M/BM)*(N/BN) for the
+// tile code; bk the K step (gemm_k_step).  M, N of every problem must be
+// multiples of the tile, K of bk.
+hipError_t gemm_grouped(int tin, int tout, bool akm, bool bkm, int tile, int bk, const GemmProb* probs,
+                        const int* off, int nprob, int total_tiles, hipStream_t s);
+int gemm_tile_bm(int code);
+int gemm_tile_bn(int code);
+int gemm_default_tile(int M, int N, int batch);
+int gemm_k_step(int tin, int K);
 // split-K epilogue: C[i*ldc + j] = (accumulate ? C : 0) + sum_{z<ks} W[z][i][j]
 // (W: ks x M x N of type tacc, T_F64 or T_F32)
 hipError_t splitk_reduce(int tacc, const void* W, int ks, int M, int N, int tout, void* C, long ldc, int accumulate,
@@ -116,6 +138,71 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
                      int* flag, hipStream_t s);
 hipError_t shift_diag(double* G, long ldg, int n, double factor, hipStream_t s);
+
+// --- batched (one job per chain) kernels ---------------------------------
+struct CholJob {
+    const double* G;
+    long ldg;
+    int k0, nbk;
+    double* L;
+    long ldl;
+    double* Linv;
+    long ldi;
+    int* flag;
+};
+hipError_t chol_diag_batch(const CholJob* jobs, int n, hipStream_t s);
+struct DotJob {  // out[i] = V_i . w, i < k
+    const double* V;
+    long ldv;
+    int k;
+    const double* w;
+    long n;
+    double* out;
+};
+// part: bmdot_part_size(n, kmax, nmax) doubles; every job has k <= kmax, n <= nmax
+hipError_t bmdot(const DotJob* jobs, int n, int kmax, long nmax, double* part, hipStream_t s);
+long bmdot_part_size(int n, int kmax, long nmax);
+struct GemvJob {  // w = beta w + cscale sum_i coef[i] V_i
+    const double* V;
+    long ldv;
+    int k;
+    const double* coef;
+    double cscale, beta;
+    double* w;
+    long n;
+};
+hipError_t bgemv(const GemvJob* jobs, int n, long nmax, hipStream_t s);
+struct ThreeJob {  // w -= alpha[0] v + beta u
+    double* w;
+    const double* v;
+    const double* u;
+    const double* alpha;
+    double beta;
+    long n;
+};
+hipError_t bthree(const ThreeJob* jobs, int n, long nmax, hipStream_t s);
+struct ScaleJob {  // y = a x
+    double a;
+    const double* x;
+    double* y;
+    long n;
+};
+hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s);
+struct ConvJob {  // y = (float) x
+    const double* x;
+    float* y;
+    long n;
+};
+hipError_t bconvert(const ConvJob* jobs, int n, long nmax, hipStream_t s);
+struct MixJob {  // FP32 in
+    const void* in;
+    long isx;
+    void* out;
+    long osx;
+    int X, Bn;
+    MixTable t;
+};
+hipError_t bwmix(int tout, const MixJob* jobs, int n, int Xmax, int Bnmax, hipStream_t s);
 hipError_t add_diag(double* G, long ldg, int i0, int i1, double v, hipStream_t s);
 hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* part, hipStream_t s);
 

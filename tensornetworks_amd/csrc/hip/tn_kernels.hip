@@ -161,9 +161,13 @@ __device__ __forceinline__ void tile_sstore(const u32x4 (&rg)[R], TI* __restrict
     }
 }
 
-// 256 threads = 4 waves in a 2x2 arrangement; wave tile (BM/2) x (BN/2).
+// One BM x BN output tile: 256 threads = 4 waves in a 2x2 arrangement, wave
+// tile (BM/2) x (BN/2); operand tiles staged through LDS (As, Bs) with the
+// next K-slice prefetched into registers.
 template <typename TI, typename TO, int BM, int BN, int BK, bool AKM, bool BKM>
-__global__ void __launch_bounds__(256) gemm_k(GemmParams p)
+__device__ __forceinline__ void gemm_tile(const TI* __restrict__ A, const TI* __restrict__ B, TO* __restrict__ C,
+                                          long lda, long ldb, long ldc, int K, float alpha, int accumulate,
+                                          int m0, int n0, TI* __restrict__ As, TI* __restrict__ Bs)
 {
     typedef MT<TI> T;
     typedef typename T::frag_t frag_t;
@@ -176,23 +180,8 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
     static_assert(FM >= 1 && FN >= 1, "tile");
     static_assert(BK % T::MK == 0, "BK");
 
-    __shared__ __attribute__((aligned(16))) TI As[BM * LDK];
-    __shared__ __attribute__((aligned(16))) TI Bs[BN * LDK];
-
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
-    const int ntn = p.N / BN, ntm = p.M / BM;
-    const int nblk = ntn * ntm;
-    int bid = blockIdx.x;
-    // consecutive block ids land on different XCDs; give each XCD a contiguous
-    // range of output tiles so tiles sharing A rows share an L2
-    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
-    const int tm = bid / ntn, tn = bid - tm * ntn;
-    const int m0 = tm * BM, n0 = tn * BN;
-    const long bz = blockIdx.y;
-    const TI* __restrict__ A = (const TI*)p.A + bz * p.sA;
-    const TI* __restrict__ B = (const TI*)p.B + bz * p.sB;
-    TO* __restrict__ C = (TO*)p.C + bz * p.sC;
 
     u32x4 ra[RA], rb[RB];
     acc_t acc[FM][FN];
@@ -203,16 +192,16 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
 
-    const int nk = p.K / BK;
-    tile_gload<TI, BM, BK, AKM>(ra, A, p.lda, m0, 0, tid);
-    tile_gload<TI, BN, BK, BKM>(rb, B, p.ldb, n0, 0, tid);
+    const int nk = K / BK;
+    tile_gload<TI, BM, BK, AKM>(ra, A, lda, m0, 0, tid);
+    tile_gload<TI, BN, BK, BKM>(rb, B, ldb, n0, 0, tid);
     tile_sstore<TI, BM, BK, AKM>(ra, As, tid);
     tile_sstore<TI, BN, BK, BKM>(rb, Bs, tid);
     __syncthreads();
     for (int t = 0; t < nk; t++) {
         if (t + 1 < nk) {
-            tile_gload<TI, BM, BK, AKM>(ra, A, p.lda, m0, (t + 1) * BK, tid);
-            tile_gload<TI, BN, BK, BKM>(rb, B, p.ldb, n0, (t + 1) * BK, tid);
+            tile_gload<TI, BM, BK, AKM>(ra, A, lda, m0, (t + 1) * BK, tid);
+            tile_gload<TI, BN, BK, BKM>(rb, B, ldb, n0, (t + 1) * BK, tid);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += T::MK) {
@@ -243,17 +232,64 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
             for (int r = 0; r < 4; r++) {
                 const long row = m0 + wm0 + i * 16 + T::crow(lane, r);
                 const long col = n0 + wn0 + j * 16 + (lane & 15);
-                TO* cp = C + row * p.ldc + col;
+                TO* cp = C + row * ldc + col;
                 if constexpr (std::is_same<TO, double>::value) {
-                    double v = (double)acc[i][j][r] * (double)p.alpha;
-                    if (p.accumulate) v += to_d(*cp);
+                    double v = (double)acc[i][j][r] * (double)alpha;
+                    if (accumulate) v += to_d(*cp);
                     *cp = cvt_from_d<TO>(v);
                 } else {
-                    float v = (float)acc[i][j][r] * p.alpha;
-                    if (p.accumulate) v += to_f(*cp);
+                    float v = (float)acc[i][j][r] * alpha;
+                    if (accumulate) v += to_f(*cp);
                     *cp = cvt_from_f<TO>(v);
                 }
             }
+}
+
+template <typename TI, typename TO, int BM, int BN, int BK, bool AKM, bool BKM>
+__global__ void __launch_bounds__(256) gemm_k(GemmParams p)
+{
+    constexpr int LDK = BK + MT<TI>::PAD;
+    __shared__ __attribute__((aligned(16))) TI As[BM * LDK];
+    __shared__ __attribute__((aligned(16))) TI Bs[BN * LDK];
+    const int ntn = p.N / BN, ntm = p.M / BM;
+    const int nblk = ntn * ntm;
+    int bid = blockIdx.x;
+    // consecutive block ids land on different XCDs; give each XCD a contiguous
+    // range of output tiles so tiles sharing A rows share an L2
+    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    const int tm = bid / ntn, tn = bid - tm * ntn;
+    const long bz = blockIdx.y;
+    gemm_tile<TI, TO, BM, BN, BK, AKM, BKM>((const TI*)p.A + bz * p.sA, (const TI*)p.B + bz * p.sB,
+                                            (TO*)p.C + bz * p.sC, p.lda, p.ldb, p.ldc, p.K, p.alpha,
+                                            p.accumulate, tm * BM, tn * BN, As, Bs);
+}
+
+// Grouped GEMM: one persistent launch over the output tiles of many problems
+// (problem q owns tiles [off[q], off[q+1]), row-major over its N/BN columns).
+// Workgroups stride over the tile list; the owning problem is found by a
+// binary search of off.
+template <typename TI, typename TO, int BM, int BN, int BK, bool AKM, bool BKM>
+__global__ void __launch_bounds__(256) gemm_grouped_k(const GemmProb* __restrict__ probs,
+                                                      const int* __restrict__ off, int nprob)
+{
+    constexpr int LDK = BK + MT<TI>::PAD;
+    __shared__ __attribute__((aligned(16))) TI As[BM * LDK];
+    __shared__ __attribute__((aligned(16))) TI Bs[BN * LDK];
+    const int total = off[nprob];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GemmProb& q = probs[lo];
+        const int local = t - off[lo], ntn = q.N / BN;
+        const int tm = local / ntn, tn = local - tm * ntn;
+        __syncthreads();  // LDS of the previous tile is free
+        gemm_tile<TI, TO, BM, BN, BK, AKM, BKM>((const TI*)q.A, (const TI*)q.B, (TO*)q.C, q.lda, q.ldb, q.ldc, q.K,
+                                                q.alpha, q.accumulate, tm * BM, tn * BN, As, Bs);
+    }
 }
 
 // tile menu (code -> BM x BN)
@@ -281,6 +317,72 @@ static hipError_t launch_tile(int code, const GemmParams& p, int batch, hipStrea
     default: gemm_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
     }
     return hipGetLastError();
+}
+
+template <typename TI, typename TO, int BK, bool AKM, bool BKM>
+static hipError_t launch_grouped(int code, const GemmProb* probs, const int* off, int nprob, int grid,
+                                 hipStream_t s)
+{
+    switch (code) {
+    case 0: gemm_grouped_k<TI, TO, 128, 128, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 1: gemm_grouped_k<TI, TO, 128, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 2: gemm_grouped_k<TI, TO, 64, 128, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 3: gemm_grouped_k<TI, TO, 64, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 4: gemm_grouped_k<TI, TO, 32, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 5: gemm_grouped_k<TI, TO, 64, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    default: gemm_grouped_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename TI, typename TO, bool AKM, bool BKM>
+static hipError_t grouped_typed(int code, int bk, const GemmProb* probs, const int* off, int nprob, int grid,
+                                hipStream_t s)
+{
+    if constexpr (std::is_same<TI, double>::value) {
+        return launch_grouped<TI, TO, 16, AKM, BKM>(code, probs, off, nprob, grid, s);
+    } else if constexpr (std::is_same<TI, float>::value) {
+        if (bk == 32) return launch_grouped<TI, TO, 32, AKM, BKM>(code, probs, off, nprob, grid, s);
+        return launch_grouped<TI, TO, 16, AKM, BKM>(code, probs, off, nprob, grid, s);
+    } else {
+        if (bk == 64) return launch_grouped<TI, TO, 64, AKM, BKM>(code, probs, off, nprob, grid, s);
+        return launch_grouped<TI, TO, 32, AKM, BKM>(code, probs, off, nprob, grid, s);
+    }
+}
+
+template <bool AKM, bool BKM>
+static hipError_t grouped_layout(int tin, int tout, int code, int bk, const GemmProb* probs, const int* off,
+                                 int nprob, int grid, hipStream_t s)
+{
+    if (tin == T_F64 && tout == T_F64) return grouped_typed<double, double, AKM, BKM>(code, bk, probs, off, nprob, grid, s);
+    if (tin == T_F32 && tout == T_F32) return grouped_typed<float, float, AKM, BKM>(code, bk, probs, off, nprob, grid, s);
+    if (tin == T_F32 && tout == T_F64) return grouped_typed<float, double, AKM, BKM>(code, bk, probs, off, nprob, grid, s);
+    if (tin == T_BF16 && tout == T_F32) return grouped_typed<bf16, float, AKM, BKM>(code, bk, probs, off, nprob, grid, s);
+    if (tin == T_BF16 && tout == T_F64) return grouped_typed<bf16, double, AKM, BKM>(code, bk, probs, off, nprob, grid, s);
+    return hipErrorInvalidValue;
+}
+
+static int choose_tile(int M, int N, int batch);
+int gemm_tile_bm(int code) { return (code >= 1 && code <= kNumTiles) ? kTiles[code - 1].bm : 0; }
+int gemm_tile_bn(int code) { return (code >= 1 && code <= kNumTiles) ? kTiles[code - 1].bn : 0; }
+int gemm_default_tile(int M, int N, int batch) { return choose_tile(M, N, batch) + 1; }
+int gemm_k_step(int tin, int K)
+{
+    if (tin == T_F64) return 16;
+    if (tin == T_F32) return K % 32 == 0 ? 32 : 16;
+    return K % 64 == 0 ? 64 : 32;
+}
+
+hipError_t gemm_grouped(int tin, int tout, bool akm, bool bkm, int tile, int bk, const GemmProb* probs,
+                        const int* off, int nprob, int total_tiles, hipStream_t s)
+{
+    if (nprob <= 0 || total_tiles <= 0) return hipSuccess;
+    if (tile < 1 || tile > kNumTiles) return hipErrorInvalidValue;
+    const int grid = total_tiles < 2048 ? total_tiles : 2048;
+    if (akm && bkm) return grouped_layout<true, true>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
+    if (akm && !bkm) return grouped_layout<true, false>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
+    if (!akm && !bkm) return grouped_layout<false, false>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
+    return grouped_layout<false, true>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
 }
 
 // Pick the largest tile that divides the problem and still gives >= ~1.5
@@ -1041,14 +1143,11 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 // same access pattern.  Dot products use eight partial sums so the LDS reads
 // pipeline.  (A register-resident variant needs the steps unrolled, and the
 // unrolled triangle does not fit the register file.)
-__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
-                                                  double* __restrict__ L, long ldl,
-                                                  double* __restrict__ Linv, long ldi,
-                                                  int* __restrict__ flag)
+__device__ __forceinline__ void chol_block(const double* __restrict__ G, long ldg, int k0, int nbk,
+                                           double* __restrict__ L, long ldl, double* __restrict__ Linv, long ldi,
+                                           int* __restrict__ flag, double (*Ls)[CB + 1], double (*Xs)[CB + 1],
+                                           int& bad)
 {
-    __shared__ double Ls[CB][CB + 1];
-    __shared__ double Xs[CB][CB + 1];
-    __shared__ int bad;
     const int r = threadIdx.x;
     if (r == 0) bad = 0;
     // coalesced load, 16 independent loads in flight per lane before the LDS
@@ -1120,6 +1219,34 @@ __global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, 
     if (r == 0 && bad) flag[0] = 1;
 }
 
+__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
+                                                  double* __restrict__ L, long ldl,
+                                                  double* __restrict__ Linv, long ldi,
+                                                  int* __restrict__ flag)
+{
+    __shared__ double Ls[CB][CB + 1];
+    __shared__ double Xs[CB][CB + 1];
+    __shared__ int bad;
+    chol_block(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag, Ls, Xs, bad);
+}
+
+// one diagonal block per workgroup, one job per chain
+__global__ void __launch_bounds__(64) chol_diag_batch_k(const CholJob* __restrict__ jobs)
+{
+    __shared__ double Ls[CB][CB + 1];
+    __shared__ double Xs[CB][CB + 1];
+    __shared__ int bad;
+    const CholJob& j = jobs[blockIdx.x];
+    chol_block(j.G, j.ldg, j.k0, j.nbk, j.L, j.ldl, j.Linv, j.ldi, j.flag, Ls, Xs, bad);
+}
+
+hipError_t chol_diag_batch(const CholJob* jobs, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    chol_diag_batch_k<<<n, 64, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
 hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
                      int* flag, hipStream_t s)
 {
@@ -1189,6 +1316,172 @@ hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* 
     if (G_ > 1024) G_ = 1024;
     eye_dev_k<<<G_, 256, 0, s>>>(G, ldg, n, part);
     absmax2_k<<<1, 256, 0, s>>>(part, (int)G_, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched vector kernels: one job per problem (chain), problem index = last
+// grid dimension.
+#define BV_EPT 8
+#define BV_BLOCK 256
+#define BV_CHUNK (BV_BLOCK * BV_EPT)
+
+// stage 1: part[(q * kmax + i) * G + b] = dot of chunk b of V_i and w
+__global__ void __launch_bounds__(BV_BLOCK) bmdot1_k(const DotJob* __restrict__ jobs, int kmax, int G,
+                                                     double* __restrict__ part)
+{
+    __shared__ double red[BV_BLOCK / 64];
+    const DotJob& j = jobs[blockIdx.y];
+    const int b = blockIdx.x;
+    double* pq = part + (size_t)blockIdx.y * kmax * G;
+    const long base = (long)b * BV_CHUNK + threadIdx.x;
+    double wr[BV_EPT];
+#pragma unroll
+    for (int e = 0; e < BV_EPT; e++) {
+        const long idx = base + (long)e * BV_BLOCK;
+        wr[e] = idx < j.n ? j.w[idx] : 0.0;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i < j.k; i++) {
+        const double* vi = j.V + (long)i * j.ldv;
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < BV_EPT; e++) {
+            const long idx = base + (long)e * BV_BLOCK;
+            if (idx < j.n) acc += vi[idx] * wr[e];
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) red[wid] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) pq[(size_t)i * G + b] = (red[0] + red[1]) + (red[2] + red[3]);
+        __syncthreads();
+    }
+}
+// stage 2: out[i] = sum_b part, one workgroup per (problem, row)
+__global__ void __launch_bounds__(256) bmdot2_k(const DotJob* __restrict__ jobs, int kmax, int G,
+                                                const double* __restrict__ part)
+{
+    __shared__ double red[4];
+    const DotJob& j = jobs[blockIdx.y];
+    const int i = blockIdx.x;
+    if (i >= j.k) return;
+    const long nb = (j.n + BV_CHUNK - 1) / BV_CHUNK;
+    const double* pq = part + ((size_t)blockIdx.y * kmax + i) * G;
+    double acc = 0.0;
+    for (long b = threadIdx.x; b < nb; b += 256) acc += pq[b];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) j.out[i] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+hipError_t bmdot(const DotJob* jobs, int n, int kmax, long nmax, double* part, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int G = (int)((nmax + BV_CHUNK - 1) / BV_CHUNK);
+    bmdot1_k<<<dim3(G, n), BV_BLOCK, 0, s>>>(jobs, kmax, G, part);
+    bmdot2_k<<<dim3(kmax, n), 256, 0, s>>>(jobs, kmax, G, part);
+    return hipGetLastError();
+}
+long bmdot_part_size(int n, int kmax, long nmax) { return (long)n * kmax * ((nmax + BV_CHUNK - 1) / BV_CHUNK); }
+
+// w = beta w + cscale sum_i coef[i] V_i
+__global__ void __launch_bounds__(256) bgemv_k(const GemvJob* __restrict__ jobs)
+{
+    __shared__ double cf[MD_MAXK];
+    const GemvJob& j = jobs[blockIdx.y];
+    for (int i = threadIdx.x; i < j.k; i += 256) cf[i] = j.coef[i] * j.cscale;
+    __syncthreads();
+    for (long idx = blockIdx.x * 256L + threadIdx.x; idx < j.n; idx += (long)gridDim.x * 256) {
+        double acc = (j.beta == 0.0) ? 0.0 : j.beta * j.w[idx];
+        for (int i = 0; i < j.k; i++) acc += cf[i] * j.V[(long)i * j.ldv + idx];
+        j.w[idx] = acc;
+    }
+}
+hipError_t bgemv(const GemvJob* jobs, int n, long nmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (nmax + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    bgemv_k<<<dim3(blocks, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+// w -= alpha[0] v + beta u
+__global__ void __launch_bounds__(256) bthree_k(const ThreeJob* __restrict__ jobs)
+{
+    const ThreeJob& j = jobs[blockIdx.y];
+    const double a = j.alpha[0];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < j.n; i += (long)gridDim.x * 256) {
+        double r = j.w[i] - a * j.v[i];
+        if (j.u) r -= j.beta * j.u[i];
+        j.w[i] = r;
+    }
+}
+hipError_t bthree(const ThreeJob* jobs, int n, long nmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (nmax + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    bthree_k<<<dim3(blocks, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+// y = a x
+__global__ void __launch_bounds__(256) bscale_k(const ScaleJob* __restrict__ jobs)
+{
+    const ScaleJob& j = jobs[blockIdx.y];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < j.n; i += (long)gridDim.x * 256) j.y[i] = j.a * j.x[i];
+}
+hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (nmax + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    bscale_k<<<dim3(blocks, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+// FP64 -> FP32 copy
+__global__ void __launch_bounds__(256) bconvert_k(const ConvJob* __restrict__ jobs)
+{
+    const ConvJob& j = jobs[blockIdx.y];
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < j.n; i += (long)gridDim.x * 256) j.y[i] = (float)j.x[i];
+}
+hipError_t bconvert(const ConvJob* jobs, int n, long nmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (nmax + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    bconvert_k<<<dim3(blocks, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+// MPO mixing (FP32 in, TO out) for many problems: grid (Bn blocks, X, problem)
+template <typename TO>
+__global__ void __launch_bounds__(256) bwmix_k(const MixJob* __restrict__ jobs)
+{
+    const MixJob& j = jobs[blockIdx.z];
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const long x = blockIdx.y;
+    if (b >= j.Bn || x >= j.X) return;
+    const float* ip = (const float*)j.in + x * j.isx + b;
+    TO* op = (TO*)j.out + x * j.osx + b;
+    for (int o = 0; o < j.t.nout; o++) {
+        const int e0 = j.t.optr[o], e1 = j.t.optr[o + 1];
+        float acc = 0.0f;
+        for (int e = e0; e < e1; e++) acc += j.t.evalf[e] * ip[j.t.eoff[e]];
+        op[j.t.ooff[o]] = cvt_from_f<TO>(acc);
+    }
+}
+hipError_t bwmix(int tout, const MixJob* jobs, int n, int Xmax, int Bnmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    dim3 grid((Bnmax + 255) / 256, Xmax, n);
+    if (tout == T_F32) bwmix_k<float><<<grid, 256, 0, s>>>(jobs);
+    else if (tout == T_F64) bwmix_k<double><<<grid, 256, 0, s>>>(jobs);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

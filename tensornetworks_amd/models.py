@@ -84,3 +84,16 @@ def random_xxz_couplings(n, seed):
     """Batched workload couplings: J~U(0.5,1.5), Jz~U(0,2), hz~U(-0.5,0.5)."""
     rng = np.random.default_rng(seed)
     return [(rng.uniform(0.5, 1.5), rng.uniform(0.0, 2.0), rng.uniform(-0.5, 0.5)) for _ in range(n)]
+
+
+def batched_workload(n, seed, L_range=(16, 128), D_range=(32, 256)):
+    """Chains of the batched workload: per chain (L, D, J, Jz, hz) with
+    L uniform in L_range, D uniform in D_range, J~U(0.5,1.5), Jz~U(0,2),
+    hz~U(-0.5,0.5), all from one seed (S=1/2 XXZ, open ends)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = int(rng.integers(L_range[0], L_range[1] + 1))
+        D = int(rng.integers(D_range[0], D_range[1] + 1))
+        out.append((L, D, rng.uniform(0.5, 1.5), rng.uniform(0.0, 2.0), rng.uniform(-0.5, 0.5)))
+    return out
