@@ -134,6 +134,12 @@ typedef struct tn_sweep_opts {
                             to the SVD when it is not accurate */
     int refine_precision;  /* operator precision of the refinement (FP64 or FP32),
                               used when refine=1 and it is finer than precision */
+    int eig_method;      /* 0 = Lanczos; 1 = Davidson (basis max_krylov, at most
+                            max_krylov*(1+max_restarts) iterations), an alternative to
+                            the reference's SparseEigenSolver back ends */
+    int precond;         /* Davidson correction: 0 none, 1 diagonal (Heff diagonal),
+                            2 learned (network set by tn_ctx_set_mlp) */
+    double precond_delta;  /* safeguard of the diagonal shift |diag - theta| >= delta */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {
@@ -220,6 +226,37 @@ typedef struct tn_batch tn_batch;
 int tn_batch_create(tn_ctx* ctx, int n, tn_chain* const* chains, int max_krylov, tn_batch** batch);
 int tn_batch_destroy(tn_batch* batch);
 int tn_batch_sweep(tn_batch* batch, int dir, const tn_sweep_opts* opts, tn_sweep_stats* stats, double* energies);
+
+/* ---- learned Davidson preconditioner (no reference counterpart) ----
+ * Network 4 -> 64 -> 64 -> 1 (tanh, FP32): params TN_MLP_NP floats
+ * (W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3), norm 8 floats
+ * (mu[4], inv_sd[4]) standardising the raw features
+ * x = [asinh((diag - theta)/delta), (theta - mean(diag))/sd(diag), log(lam_l + 1e-10),
+ * log(lam_r + 1e-10)] (diagonal statistics over the logical elements).
+ * The Davidson correction is t = g * r / (theta - diag) with g the output
+ * clamped to [-16, 16]. */
+#define TN_MLP_NP 4545
+/* Load the network used by eig_method = 1, precond = 2 on this context. */
+int tn_ctx_set_mlp(tn_ctx* ctx, const float* params, const float* norm);
+/* g[i] = network(X[i]) for raw features X (n x 4). */
+int tn_mlp_forward(tn_ctx* ctx, const float* params, const float* norm, int64_t n, const double* X, double* g);
+/* Mean loss (1/n) sum (g_i a_i - t_i)^2 and its gradient (FP32 kernels). */
+int tn_mlp_loss_grad(tn_ctx* ctx, const float* params, const float* norm, int64_t n, const double* X,
+                     const double* a, const double* t, double* loss, float* grad);
+/* Adam training on the device: `steps` minibatches of `batch` samples drawn
+ * (seeded) from the n samples; params updated in place.  loss[0] / loss[1]:
+ * mean loss over all samples before / after. */
+int tn_mlp_train(tn_ctx* ctx, float* params, const float* norm, int64_t n, const double* X, const double* a,
+                 const double* t, int steps, int batch, double lr, uint64_t seed, double* loss);
+/* Training samples from the local problem of site `site` (current
+ * environments, FP64 operator): the exact local ground state u0 is found by
+ * Lanczos, then `iters` Davidson iterations with the diagonal correction run
+ * from the current site tensor; after each, nsamp elements (seeded) are
+ * recorded as raw features X (4), a = r/(theta - diag) and the exact
+ * correction t = u0/(u.u0) - u, both divided by |r|.  Arrays hold
+ * iters*nsamp entries; *nout = entries written.  The chain is unchanged. */
+int tn_chain_precond_samples(tn_chain* ch, int site, int iters, int nsamp, uint64_t seed, double delta,
+                             double* X, double* a, double* t, int* nout);
 
 /* Plain device copy of n doubles (read n*8 bytes, write n*8 bytes) by the
  * library's streaming kernel, once, into freshly allocated buffers; *ms = its

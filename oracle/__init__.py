@@ -70,6 +70,9 @@ def _load():
         "orc_mps_norm_status": (None, [v, ctypes.c_char_p]),
         "orc_mps_get_bond_sv": (i, [v, i, _dp]),
         "orc_rng_uniform": (d, [ctypes.POINTER(u64)]),
+        "orc_mlp_features": (None, [l, _dp, d, _dp, _dp, d, _dp]),
+        "orc_mlp_forward": (None, [_dp, _dp, l, _dp, _dp]),
+        "orc_mlp_loss_grad": (d, [_dp, _dp, l, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -300,3 +303,33 @@ class OracleMPS:
         sv = np.zeros(4096)
         n = lib.orc_mps_get_bond_sv(self.h, b, P(sv))
         return sv[:n].copy()
+
+
+MLP_H = 64
+MLP_NP = MLP_H * 4 + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1
+
+
+def mlp_features(diag, theta, lam_l, lam_r, delta=1e-2):
+    diag = np.ascontiguousarray(diag, dtype=np.float64)
+    X = np.zeros((diag.size, 4))
+    lib.orc_mlp_features(diag.size, P(diag), float(theta), P(np.ascontiguousarray(lam_l, dtype=np.float64)),
+                         P(np.ascontiguousarray(lam_r, dtype=np.float64)), float(delta), P(X))
+    return X
+
+
+def mlp_forward(params, norm, X):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    g = np.zeros(X.shape[0])
+    lib.orc_mlp_forward(P(np.ascontiguousarray(params, dtype=np.float64)),
+                        P(np.ascontiguousarray(norm, dtype=np.float64)), X.shape[0], P(X), P(g))
+    return g
+
+
+def mlp_loss_grad(params, norm, X, a, t):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    grad = np.zeros(MLP_NP)
+    loss = lib.orc_mlp_loss_grad(P(np.ascontiguousarray(params, dtype=np.float64)),
+                                 P(np.ascontiguousarray(norm, dtype=np.float64)), X.shape[0], P(X),
+                                 P(np.ascontiguousarray(a, dtype=np.float64)),
+                                 P(np.ascontiguousarray(t, dtype=np.float64)), P(grad))
+    return loss, grad

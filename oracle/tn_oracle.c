@@ -1551,3 +1551,98 @@ double orc_mps_sweep2(orc_mps* s, int dir, int Dmax, double epsSV, double epsEig
     }
     return maxTrunc;
 }
+
+
+/* ---- learned Davidson preconditioner: the network's definition ---------- */
+void orc_mlp_features(long n, const double* diag, double theta, const double* lam_l, const double* lam_r,
+                      double delta, double* X)
+{
+    /* theta enters relative to the spread of the diagonal (an intensive
+     * quantity; theta itself grows with the chain length) */
+    double mu = 0.0, sq = 0.0;
+    for (long i = 0; i < n; i++) {
+        mu += diag[i];
+        sq += diag[i] * diag[i];
+    }
+    mu = n > 0 ? mu / (double)n : 0.0;
+    double sd = n > 0 ? sqrt(fmax(sq / (double)n - mu * mu, 0.0)) : 0.0;
+    if (!(sd > 1e-12)) sd = 1.0;
+    for (long i = 0; i < n; i++) {
+        X[4 * i + 0] = asinh((diag[i] - theta) / delta);
+        X[4 * i + 1] = (theta - mu) / sd;
+        X[4 * i + 2] = log(lam_l[i] + 1e-10);
+        X[4 * i + 3] = log(lam_r[i] + 1e-10);
+    }
+}
+
+/* one sample: forward, optionally backward (dy = dloss/dy, grad accumulated) */
+static double mlp_one(const double* P, const double* norm, const double* x, double dy_scale_a, double t,
+                      double* grad, double* g_out)
+{
+    const int H = ORC_MLP_H;
+    const double* W1 = P;
+    const double* b1 = W1 + H * 4;
+    const double* W2 = b1 + H;
+    const double* b2 = W2 + H * H;
+    const double* w3 = b2 + H;
+    const double b3 = w3[H];
+    double xh[4], h1[ORC_MLP_H], h2[ORC_MLP_H];
+    for (int k = 0; k < 4; k++) xh[k] = (x[k] - norm[k]) * norm[4 + k];
+    for (int j = 0; j < H; j++) {
+        double z = b1[j];
+        for (int k = 0; k < 4; k++) z += W1[j * 4 + k] * xh[k];
+        h1[j] = tanh(z);
+    }
+    for (int j = 0; j < H; j++) {
+        double z = b2[j];
+        for (int k = 0; k < H; k++) z += W2[j * H + k] * h1[k];
+        h2[j] = tanh(z);
+    }
+    double y = b3;
+    for (int j = 0; j < H; j++) y += w3[j] * h2[j];
+    if (g_out) *g_out = y;
+    if (!grad) return 0.0;
+    /* loss term (y a - t)^2; dy = 2 (y a - t) a (the 1/n is applied by the caller) */
+    const double a = dy_scale_a, res = y * a - t, dy = 2.0 * res * a;
+    double* gW1 = grad;
+    double* gb1 = gW1 + H * 4;
+    double* gW2 = gb1 + H;
+    double* gb2 = gW2 + H * H;
+    double* gw3 = gb2 + H;
+    double d2[ORC_MLP_H], d1[ORC_MLP_H];
+    gw3[H] += dy;
+    for (int j = 0; j < H; j++) {
+        gw3[j] += dy * h2[j];
+        d2[j] = dy * w3[j] * (1.0 - h2[j] * h2[j]);
+    }
+    for (int j = 0; j < H; j++) {
+        gb2[j] += d2[j];
+        for (int k = 0; k < H; k++) gW2[j * H + k] += d2[j] * h1[k];
+    }
+    for (int k = 0; k < H; k++) {
+        double s = 0.0;
+        for (int j = 0; j < H; j++) s += W2[j * H + k] * d2[j];
+        d1[k] = s * (1.0 - h1[k] * h1[k]);
+    }
+    for (int j = 0; j < H; j++) {
+        gb1[j] += d1[j];
+        for (int k = 0; k < 4; k++) gW1[j * 4 + k] += d1[j] * xh[k];
+    }
+    return res * res;
+}
+
+void orc_mlp_forward(const double* params, const double* norm, long n, const double* X, double* g)
+{
+    for (long i = 0; i < n; i++) mlp_one(params, norm, X + 4 * i, 0.0, 0.0, NULL, g + i);
+}
+
+double orc_mlp_loss_grad(const double* params, const double* norm, long n, const double* X, const double* a,
+                         const double* t, double* grad)
+{
+    for (int p = 0; p < ORC_MLP_NP; p++) grad[p] = 0.0;
+    double loss = 0.0;
+    for (long i = 0; i < n; i++) loss += mlp_one(params, norm, X + 4 * i, a[i], t[i], grad, NULL);
+    const double inv = n > 0 ? 1.0 / (double)n : 0.0;
+    for (int p = 0; p < ORC_MLP_NP; p++) grad[p] *= inv;
+    return loss * inv;
+}

@@ -146,6 +146,26 @@ int orc_mps_get_bond_sv(const orc_mps* s, int b, double* sv);
 double orc_mps_sweep2(orc_mps* s, int dir, int Dmax, double epsSV, double epsEig);
 double orc_mps_last_energy(const orc_mps* s);
 
+/* ---- learned Davidson preconditioner (no reference counterpart) -------
+ * The network maps per-element features of a local eigenproblem to a factor
+ * g on the Davidson diagonal correction: t_i = g_i * s0_i * r_i with
+ * s0_i = -1/den_i, den_i = diag_i - theta (|den_i| < delta -> delta).
+ * Features (raw): x = [asinh((diag_i - theta)/delta), (theta - mean(diag))/sd(diag),
+ * log(lam_left[a] + 1e-10), log(lam_right[b] + 1e-10)] (mean and sd over the n
+ * given elements), standardised by
+ * norm = [mu0..mu3, inv_sd0..inv_sd3].  Network 4 -> 64 -> 64 -> 1, tanh,
+ * params (ORC_MLP_NP) = W1[64][4], b1[64], W2[64][64], b2[64], w3[64], b3.
+ * Loss over n samples: (1/n) sum (g_i a_i - t_i)^2 (a_i = s0_i r_i, t_i the
+ * exact correction).  Pinned by central finite differences of the loss
+ * (tests/test_oracle.py) and used as the checker of the GPU kernels. */
+#define ORC_MLP_H 64
+#define ORC_MLP_NP (ORC_MLP_H * 4 + ORC_MLP_H + ORC_MLP_H * ORC_MLP_H + ORC_MLP_H + ORC_MLP_H + 1)
+void orc_mlp_features(long n, const double* diag, double theta, const double* lam_l, const double* lam_r,
+                      double delta, double* X);
+void orc_mlp_forward(const double* params, const double* norm, long n, const double* X, double* g);
+double orc_mlp_loss_grad(const double* params, const double* norm, long n, const double* X, const double* a,
+                         const double* t, double* grad);
+
 /* ---- seeded generator shared with the GPU host code ------------------- */
 uint64_t orc_rng_next(uint64_t* state);
 double orc_rng_uniform(uint64_t* state); /* U(-1,1) */

@@ -9,6 +9,7 @@ from ._abi import (  # noqa: F401
     TN_BF16,
     TN_FP32,
     TN_FP64,
+    MLP_NP,
     Batch,
     Chain,
     Context,
@@ -23,6 +24,6 @@ from .dmrg import FindVariationalGroundState, ScheduleLine  # noqa: F401
 
 __all__ = [
     "Context", "Chain", "Batch", "SweepOpts", "SweepStats", "TnError", "lib", "LIB_PATH",
-    "TN_FP64", "TN_FP32", "TN_BF16", "svcompress_rank", "xxz_mpo", "mpo_square",
+    "TN_FP64", "TN_FP32", "TN_BF16", "MLP_NP", "svcompress_rank", "xxz_mpo", "mpo_square",
     "identity_mpo", "FindVariationalGroundState", "ScheduleLine",
 ]

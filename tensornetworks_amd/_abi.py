@@ -13,6 +13,7 @@ TN_OK, TN_WARN_NOT_CONVERGED = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class TnError(RuntimeError):
@@ -33,6 +34,9 @@ class SweepOpts(ctypes.Structure):
         ("gemm_tile", ctypes.c_int),
         ("gauge", ctypes.c_int),
         ("refine_precision", ctypes.c_int),
+        ("eig_method", ctypes.c_int),
+        ("precond", ctypes.c_int),
+        ("precond_delta", ctypes.c_double),
     ]
 
     @classmethod
@@ -94,6 +98,11 @@ def _load():
         "tn_ctx_set_autotune": (i, [v, i]),
         "tn_copy_bench": (i, [v, ctypes.c_int64, _dp]),
         "tn_ctx_tune_load": (i, [v, ctypes.c_char_p]),
+        "tn_ctx_set_mlp": (i, [v, _fp, _fp]),
+        "tn_mlp_forward": (i, [v, _fp, _fp, ctypes.c_int64, _dp, _dp]),
+        "tn_mlp_loss_grad": (i, [v, _fp, _fp, ctypes.c_int64, _dp, _dp, _dp, _dp, _fp]),
+        "tn_mlp_train": (i, [v, _fp, _fp, ctypes.c_int64, _dp, _dp, _dp, i, i, ctypes.c_double, u64, _dp]),
+        "tn_chain_precond_samples": (i, [v, i, i, i, u64, ctypes.c_double, _dp, _dp, _dp, _ip]),
         "tn_batch_create": (i, [v, i, ctypes.POINTER(ctypes.c_void_p), i, vp]),
         "tn_batch_destroy": (i, [v]),
         "tn_batch_sweep": (i, [v, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
@@ -126,6 +135,15 @@ def _load():
 
 
 lib = _load()
+
+
+MLP_NP = 4545  # parameters of the 4 -> 64 -> 64 -> 1 preconditioner network
+
+
+def FP(a):
+    """numpy float32 C-contiguous array -> float*"""
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_fp)
 
 
 def P(a):
@@ -180,6 +198,43 @@ class Context:
 
     def tune_save(self, path):
         return self.check(lib.tn_ctx_tune_save(self.h, os.fsencode(path)), "tn_ctx_tune_save")
+
+    # -- learned Davidson preconditioner ---------------------------------------
+    def set_mlp(self, params, norm):
+        """Network used by SweepOpts(eig_method=1, precond=2) on this context."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        self.check(lib.tn_ctx_set_mlp(self.h, FP(p), FP(nm)), "tn_ctx_set_mlp")
+
+    def mlp_forward(self, params, norm, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        g = np.zeros(X.shape[0])
+        self.check(lib.tn_mlp_forward(self.h, FP(np.ascontiguousarray(params, dtype=np.float32)),
+                                      FP(np.ascontiguousarray(norm, dtype=np.float32)), X.shape[0], P(X), P(g)),
+                   "tn_mlp_forward")
+        return g
+
+    def mlp_loss_grad(self, params, norm, X, a, t):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        grad = np.zeros(MLP_NP, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        self.check(lib.tn_mlp_loss_grad(self.h, FP(np.ascontiguousarray(params, dtype=np.float32)),
+                                        FP(np.ascontiguousarray(norm, dtype=np.float32)), X.shape[0], P(X),
+                                        P(np.ascontiguousarray(a, dtype=np.float64)),
+                                        P(np.ascontiguousarray(t, dtype=np.float64)), ctypes.byref(loss), FP(grad)),
+                   "tn_mlp_loss_grad")
+        return loss.value, grad
+
+    def mlp_train(self, params, norm, X, a, t, steps, batch=4096, lr=1e-3, seed=0):
+        """Adam on the device; returns (new params, (loss before, loss after))."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        p = np.array(params, dtype=np.float32, copy=True)
+        loss = np.zeros(2)
+        self.check(lib.tn_mlp_train(self.h, FP(p), FP(np.ascontiguousarray(norm, dtype=np.float32)), X.shape[0],
+                                    P(X), P(np.ascontiguousarray(a, dtype=np.float64)),
+                                    P(np.ascontiguousarray(t, dtype=np.float64)), int(steps), int(batch), float(lr),
+                                    int(seed), P(loss)), "tn_mlp_train")
+        return p, (loss[0], loss[1])
 
     def copy_bench(self, n):
         """Duration (ms) of one plain device copy of n doubles."""
@@ -353,6 +408,18 @@ class Chain:
         if n < 0:
             raise TnError(n, "tn_chain_bond_sv")
         return buf[:n].copy()
+
+    def precond_samples(self, site, iters=4, nsamp=2048, seed=0, delta=1e-2):
+        """Training samples (X, a, t) of the learned preconditioner at a site."""
+        n = iters * nsamp
+        X = np.zeros((n, 4))
+        a = np.zeros(n)
+        t = np.zeros(n)
+        nout = ctypes.c_int(0)
+        self.ctx.check(lib.tn_chain_precond_samples(self.h, site, iters, nsamp, int(seed), float(delta), P(X), P(a),
+                                                    P(t), ctypes.byref(nout)), "tn_chain_precond_samples")
+        k = nout.value
+        return X[:k], a[:k], t[:k]
 
     def heff_bench(self, site, precision, reps=10):
         """Average device ms of (GEMM 1, MPO stage, GEMM 2) and flops per GEMM launch."""

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-I", INCLUDE,
 ]
 
-HIP_SOURCES = ["hip/tn_kernels.hip", "hip/tn_engine.hip", "hip/tn_chain.hip", "hip/tn_batch.hip"]
+HIP_SOURCES = ["hip/tn_kernels.hip", "hip/tn_engine.hip", "hip/tn_chain.hip", "hip/tn_batch.hip", "hip/tn_precond.hip"]
 HIP_HEADERS = ["hip/tn_kernels.h", "hip/tn_engine.h", "hip/tn_chain.h"]
 CXX_SOURCES = ["host/tensornetworks.cpp"]
 CXX_HEADERS = []

@@ -38,6 +38,14 @@ struct tn_chain {
 };
 
 namespace tn {
+// Heff apply callback (ApplyFn) of a chain site, with optional statistics
+struct HeffCall {
+    HeffOp op;
+    tn_sweep_stats* st;
+    int D1, D2;  // logical dims for the algorithmic flop count
+};
+void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y);
+const double* w_diag(tn_chain* ch, int i);
 double now_ms();
 void ensure_lo(tn_chain* ch, int prec);
 const void* lo_left(tn_chain* ch, int prec, int i);

@@ -155,12 +155,6 @@ void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_s
 }
 
 // --- Heff apply callback for the Lanczos driver ---------------------------
-struct HeffCall {
-    HeffOp op;
-    tn_sweep_stats* st;
-    int D1, D2;  // logical dims for the algorithmic flop count
-};
-
 void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y)
 {
     const HeffCall* hc = (const HeffCall*)vop;
@@ -198,6 +192,21 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
     return op;
 }
 
+// device copy of the diagonal blocks W[w][w'][m][m] of site i ([Dw1][Dw2][d])
+const double* w_diag(tn_chain* ch, int i)
+{
+    tn_ctx* c = ch->ctx;
+    const int d = ch->d, Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+    std::vector<double> wd((size_t)Dw1 * Dw2 * d);
+    const double* W = ch->W[i].data();
+    for (int w = 0; w < Dw1; w++)
+        for (int v = 0; v < Dw2; v++)
+            for (int m = 0; m < d; m++) wd[((size_t)w * Dw2 + v) * d + m] = W[(((size_t)w * Dw2 + v) * d + m) * d + m];
+    c->dav_wd.ensure(wd.size() * 8);
+    h2d(c, c->dav_wd.p, wd.data(), wd.size() * 8);
+    return c->dav_wd.as<double>();
+}
+
 // local eigensolve of site i; returns (e0, e1)
 void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1)
 {
@@ -212,6 +221,17 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     // a reduced-precision operator cannot resolve residuals below its own
     // rounding (~2^-8 for BF16, ~2^-23 for FP32 operands, relative to |theta|)
     // when a refinement follows, stop there and let the refinement finish
+    if (o.eig_method == 1) {
+        const double* Wd = w_diag(ch, i);
+        DavidsonResult dr = davidson(c, heff_cb, &hc, make_heff(ch, i, TN_FP64, o.gemm_tile), ch->D1[i], ch->D2[i],
+                                     Wd, ch->A[i],
+                                     o.eig_tol, o.max_krylov, o.max_krylov * (1 + o.max_restarts), o.precond,
+                                     o.precond_delta);
+        if (st) st->lanczos_ms += now_ms() - t0;
+        *e0 = dr.e0;
+        *e1 = dr.e1;
+        return;
+    }
     const bool refining = o.refine && o.refine_precision < o.precision;
     const double rel = refining ? (o.precision == TN_BF16 ? 4e-3 : 4e-7) : 0.0;
     LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel);
@@ -656,6 +676,9 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->gemm_tile = 0;
     o->gauge = 0;
     o->refine_precision = TN_FP64;
+    o->eig_method = 0;
+    o->precond = 1;
+    o->precond_delta = 1e-2;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)

@@ -80,6 +80,8 @@ struct tn_ctx {
     bool autotune = false;
     std::map<std::array<long, 9>, std::pair<int, int>> tune;
     tn::DBuf tune_C, splitk;
+    // Davidson workspace and the preconditioner network (params + norm, FP32)
+    tn::DBuf dav_W, dav_vec, dav_wd, mlp;
     // seam state (tn_site_bind)
     int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
     std::vector<double> b_W;
@@ -145,6 +147,22 @@ typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
 // converged when |H x - theta x| < tol or < tol_rel*|theta| (tol_rel = 0: off)
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
                       int max_krylov, int max_restarts, double tol_rel = 0.0);
+
+// --- Davidson with a diagonal / learned preconditioner (tn_precond.hip) ---
+struct DavidsonResult {
+    double e0 = 0, e1 = 0, residual = 0;
+    int n_matvec = 0, iters = 0;
+    bool converged = false;
+};
+// geo: the operator's shapes and FP64 environments (for the Heff diagonal),
+// D1 x D2 the logical bond dimensions;
+// Wd: device [Dw1][Dw2][p] diagonal of the W blocks; kind 0 none, 1 diagonal,
+// 2 network (ctx->mlp); converged when |H u - theta u| < tol.
+DavidsonResult davidson(tn_ctx* c, ApplyFn fn, const void* op, const HeffOp& geo, int D1, int D2, const double* Wd,
+                        double* x, double tol, int mmax, int max_iter, int kind, double delta);
+void heff_diag(tn_ctx* c, const HeffOp& op, const double* Wd, double* diag);
+void bond_weights(tn_ctx* c, const double* u, int p, int D1p, int D2p, double* laml, double* lamr);
+void sym_eig_small(int m, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z);
 
 // --- SVD -----------------------------------------------------------------
 // Thin SVD of a device matrix view A(i,j) = A[i*rs + j*cs], r x c.

@@ -1,0 +1,108 @@
+"""Learned Davidson preconditioner: the network's GPU kernels against the CPU
+oracle (forward values, loss gradient), device training, and the Davidson
+local solver against the reference energies."""
+import numpy as np
+import pytest
+
+import oracle as O
+import tensornetworks_amd as tn
+from tensornetworks_amd import precond
+from tensornetworks_amd.models import xxz_mpo
+
+pytestmark = pytest.mark.gpu
+
+
+def net(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(tn.MLP_NP) * scale).astype(np.float32)
+    norm = np.concatenate([rng.standard_normal(4) * 0.2, 0.5 + rng.random(4)]).astype(np.float32)
+    return p, norm
+
+
+def test_mlp_forward_matches_oracle(ctx):
+    p, norm = net(0)
+    X = np.random.default_rng(1).standard_normal((5000, 4)) * 2.0
+    g = ctx.mlp_forward(p, norm, X)
+    ref = O.mlp_forward(p.astype(np.float64), norm.astype(np.float64), X)
+    # FP32 network vs FP64 oracle
+    assert np.abs(g - ref).max() < 2e-5 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 1000, 4096])
+def test_mlp_gradient_matches_oracle(ctx, n):
+    p, norm = net(2)
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, 4))
+    a = rng.standard_normal(n)
+    t = rng.standard_normal(n)
+    loss, g = ctx.mlp_loss_grad(p, norm, X, a, t)
+    lref, gref = O.mlp_loss_grad(p.astype(np.float64), norm.astype(np.float64), X, a, t)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    # FP32 kernels vs FP64 oracle (itself pinned by finite differences)
+    assert np.linalg.norm(g - gref) < 1e-4 * np.linalg.norm(gref)
+
+
+def test_mlp_training_reduces_loss(ctx):
+    # targets from a smooth factor of the features: the device Adam fits it
+    rng = np.random.default_rng(3)
+    n = 20000
+    X = rng.standard_normal((n, 4))
+    a = rng.standard_normal(n)
+    gtrue = 1.0 + 0.5 * np.tanh(X[:, 0]) - 0.3 * X[:, 2] / (1.0 + X[:, 2] ** 2)
+    t = gtrue * a
+    p0 = precond.init_params(0)
+    norm = precond.fit_norm(X)
+    p, (l0, l1) = ctx.mlp_train(p0, norm, X, a, t, steps=600, batch=2048, lr=3e-3, seed=1)
+    assert l1 < 0.05 * l0, (l0, l1)
+    # the reported loss is the oracle's loss of the returned parameters
+    lref, _ = O.mlp_loss_grad(p.astype(np.float64), norm.astype(np.float64), X, a, t)
+    assert l1 == pytest.approx(lref, rel=1e-3)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_davidson_reference_energy(ctx, kind):
+    # PyTensorNetworks/test_Unittest1.py:22 : L=10 S=1/2 D=8 -> -0.473113399629092 per bond
+    ch = tn.Chain(ctx, xxz_mpo(10, 0.5), 8)
+    ch.initialize(1, 20261202)
+    ch.normalize(1)
+    o = tn.SweepOpts.default(eig_tol=1e-12, eig_method=1, precond=kind, max_krylov=40, max_restarts=4)
+    ch.load_caches(o)
+    for _ in range(10):
+        ch.sweep(0, o)
+        ch.sweep(1, o)
+    assert abs(ch.expectation() / 9 - (-0.473113399629092)) < 1e-11
+
+
+def test_identity_network_equals_diagonal_davidson(ctx):
+    # w3 = 0, b3 = 1 makes the learned correction the diagonal one exactly
+    p = precond.init_params(5)
+    norm = np.array([0, 0, 0, 0, 1, 1, 1, 1], dtype=np.float32)
+    ctx.set_mlp(p, norm)
+    res = {}
+    for kind in (1, 2):
+        ch = tn.Chain(ctx, xxz_mpo(12, 0.5, Jz=0.7, hz=0.1), 16)
+        ch.initialize(1, 99)
+        ch.normalize_qr(1)
+        o = tn.SweepOpts.default(eig_tol=1e-10, eig_method=1, precond=kind, max_krylov=30, max_restarts=2, gauge=1)
+        ch.load_caches(o)
+        st = tn.SweepStats()
+        for _ in range(2):
+            ch.sweep(0, o, st)
+            ch.sweep(1, o, st)
+        res[kind] = (st.n_matvec, ch.expectation())
+    assert res[1][0] == res[2][0]
+    assert res[1][1] == pytest.approx(res[2][1], abs=1e-12)
+
+
+def test_precond_samples_are_exact_corrections(ctx):
+    # with t the exact correction, u + t is the exact local ground state up to
+    # normalisation: the samples' a/t relation is consistent with g ~ 1 early on
+    ch = tn.Chain(ctx, xxz_mpo(10, 0.5, Jz=1.3), 16)
+    ch.initialize(1, 5)
+    ch.normalize_qr(1)
+    o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1)
+    ch.load_caches(o)
+    X, a, t = ch.precond_samples(0, iters=3, nsamp=500, seed=1)
+    assert X.shape == (1500, 4) and np.all(np.isfinite(X)) and np.all(np.isfinite(a)) and np.all(np.isfinite(t))
+    # the diagonal correction points the right way on average
+    assert np.sum(a * t) > 0

@@ -299,3 +299,52 @@ def test_increase_bond_dimensions_shapes():
     D1, D2 = m.dims()
     assert D1[0] == 1 and D2[-1] == 1
     assert max(D2) <= 6 and D2[0] == 2 and D1[-1] == 2
+
+
+# ---- learned preconditioner network (oracle definition) ----------------------
+def _mlp_setup(seed=0, n=40):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(O.MLP_NP) * 0.3
+    norm = np.concatenate([rng.standard_normal(4) * 0.2, 0.5 + rng.random(4)])
+    X = rng.standard_normal((n, 4))
+    a = rng.standard_normal(n)
+    t = rng.standard_normal(n)
+    return p, norm, X, a, t
+
+
+def test_mlp_gradient_matches_finite_differences():
+    # central differences of the loss, one parameter from every group
+    p, norm, X, a, t = _mlp_setup()
+    loss, g = O.mlp_loss_grad(p, norm, X, a, t)
+    H = O.MLP_H
+    offs = {"W1": 0, "b1": H * 4, "W2": H * 5, "b2": H * 5 + H * H, "w3": H * 6 + H * H, "b3": H * 7 + H * H}
+    rng = np.random.default_rng(1)
+    idx = [offs[k] + int(rng.integers(0, 4 if k == "W1" else (H * H if k == "W2" else (1 if k == "b3" else H))))
+           for k in offs] + [offs["W1"] + 3, offs["W2"] + H * H - 1]
+    eps = 1e-6
+    for k in idx:
+        pp = p.copy(); pp[k] += eps
+        pm = p.copy(); pm[k] -= eps
+        fd = (O.mlp_loss_grad(pp, norm, X, a, t)[0] - O.mlp_loss_grad(pm, norm, X, a, t)[0]) / (2 * eps)
+        assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(fd)), (k, fd, g[k])
+    # the loss is the mean of (g a - t)^2 with g the forward output
+    gout = O.mlp_forward(p, norm, X)
+    assert loss == pytest.approx(np.mean((gout * a - t) ** 2), rel=1e-12)
+
+
+def test_mlp_identity_network_is_plain_davidson():
+    # w3 = 0, b3 = 1: g = 1 for every input, the diagonal Davidson correction
+    p, norm, X, _, _ = _mlp_setup(2)
+    H = O.MLP_H
+    p[H * 6 + H * H: H * 7 + H * H] = 0.0
+    p[H * 7 + H * H] = 1.0
+    assert np.all(O.mlp_forward(p, norm, X) == 1.0)
+
+
+def test_mlp_features_definition():
+    diag = np.array([1.0, -2.0, 3.5])
+    X = O.mlp_features(diag, 0.5, np.array([1.0, 0.1, 0.0]), np.array([0.5, 0.5, 1e-3]), 1e-2)
+    assert np.allclose(X[:, 0], np.arcsinh((diag - 0.5) / 1e-2))
+    assert np.allclose(X[:, 1], (0.5 - diag.mean()) / diag.std())
+    assert np.allclose(X[:, 2], np.log(np.array([1.0, 0.1, 0.0]) + 1e-10))
+    assert np.allclose(X[:, 3], np.log(np.array([0.5, 0.5, 1e-3]) + 1e-10))
