@@ -59,6 +59,7 @@ def parse():
                         "precond: train the learned Davidson preconditioner on the device and compare local "
                         "solvers on the S=1/2 Heisenberg chain L=128, D=1024")
     p.add_argument("--chains", type=int, default=128, help="batch workload: number of chains")
+    p.add_argument("--precond-out", default=None, help="precond workload: write the trained network (.npz) here")
     p.add_argument("--heff-only", action="store_true",
                    help="set up the chain and time only the bulk-site Heff stages (for counter runs), "
                         "plus one plain device copy of known size that calibrates the byte counters")
@@ -236,15 +237,15 @@ def run_precond(a, tn, ctx):
     if a.out:
         with open(a.out, "w") as f:
             f.write(line + "\n")
-    precond.save(os.path.join(ROOT, "gpurun_out", "precond_mlp.npz") if os.path.isdir(os.path.join(ROOT, "gpurun_out"))
-                 else "precond_mlp.npz", params, norm)
+    if a.precond_out:
+        precond.save(a.precond_out, params, norm)
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is a design note "
-                                   "(DESIGN.md), not a launch path", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on exactly one GPU; sharding across GPUs is designed "
+                                   "but not launched", "n_gpus": a.gpus}))
         sys.exit(2)
     import tensornetworks_amd as tn
     from tensornetworks_amd.models import xxz_mpo
