@@ -269,8 +269,10 @@ int tn_copy_bench(tn_ctx* ctx, int64_t n, double* ms);
  * tn_sweep_opts.gauge = 1; reference MPSSite_Normalization.C:80-107 uses a
  * Householder QR for the same step).  Returns 0 when the factorisation met
  * max|Q^T Q - I| <= tol, 1 when it did not (Q, R still written), < 0 on
- * error.  *shifted = 1 when the first pass needed the shift. */
-int tn_cholqr_test(tn_ctx* ctx, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted);
+ * error.  *shifted = 1 when the first pass needed the shift.  triangular = 0
+ * allows the polar finish of the gauge move (R then a full matrix). */
+int tn_cholqr_test(tn_ctx* ctx, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted,
+                   int triangular);
 
 #ifdef __cplusplus
 }

@@ -94,7 +94,7 @@ def _load():
         "tn_svd": (i, [v, i, i, _dp, _dp, _dp, _dp]),
         "tn_svcompress_rank": (i, [i, _dp, i, d, _dp, _dp]),
         "tn_gemm_test": (i, [v, i, i, i, i, i, i, i, i, _dp, _dp, _dp, i, i]),
-        "tn_cholqr_test": (i, [v, i, i, _dp, ctypes.c_double, _dp, _dp, _ip]),
+        "tn_cholqr_test": (i, [v, i, i, _dp, ctypes.c_double, _dp, _dp, _ip, i]),
         "tn_ctx_set_autotune": (i, [v, i]),
         "tn_copy_bench": (i, [v, ctypes.c_int64, _dp]),
         "tn_ctx_tune_load": (i, [v, ctypes.c_char_p]),
@@ -298,14 +298,16 @@ class Context:
                                     P(C), tile, ksplit), "tn_gemm_test")
         return C.reshape(batch, M, N)
 
-    def cholqr_test(self, A, tol=1e-11):
-        """A = Q R on the device (shifted CholeskyQR); returns (Q, R, ok, shifted)."""
+    def cholqr_test(self, A, tol=1e-11, triangular=True):
+        """A = Q R on the device (shifted CholeskyQR); returns (Q, R, ok, shifted).
+        triangular=False allows the polar finish used by the gauge move."""
         A = np.ascontiguousarray(A, dtype=np.float64)
         m, n = A.shape
         Q = np.zeros((m, n))
         R = np.zeros((n, n))
         sh = ctypes.c_int(0)
-        rc = lib.tn_cholqr_test(self.h, m, n, P(A.ravel()), tol, P(Q.ravel()), P(R.ravel()), ctypes.byref(sh))
+        rc = lib.tn_cholqr_test(self.h, m, n, P(A.ravel()), tol, P(Q.ravel()), P(R.ravel()), ctypes.byref(sh),
+                                int(bool(triangular)))
         if rc < 0:
             self.check(rc, "tn_cholqr_test")
         return Q, R, rc == 0, sh.value

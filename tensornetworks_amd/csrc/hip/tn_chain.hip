@@ -326,7 +326,7 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
         const int m = D1p * d, n = D2p;
         c->spare.ensure((size_t)n * n * 8);
         double* Lacc = c->spare.as<double>();
-        if (!cholqr_device(c, ch->A[i], D2p, m, n, D2, ch->spare, Lacc, tol, &shifted)) return false;
+        if (!cholqr_device(c, ch->A[i], D2p, m, n, D2, ch->spare, Lacc, tol, &shifted, false)) return false;
         if (st) st->n_qr_shifted += shifted;
         // site i <- Q (already in ch->spare: swap buffers)
         std::swap(ch->A[i], ch->spare);
@@ -354,7 +354,7 @@ bool qr_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st)
     double* Lacc = c->spare.as<double>();
     c->svd_U.ensure((size_t)m * n * 8);
     double* Qt = c->svd_U.as<double>();
-    if (!cholqr_device(c, At, n, m, n, D1, Qt, Lacc, tol, &shifted)) return false;
+    if (!cholqr_device(c, At, n, m, n, D1, Qt, Lacc, tol, &shifted, false)) return false;
     if (st) st->n_qr_shifted += shifted;
     // site i <- Q^T
     TN_HIP(copy2(Qt, 1, n, D1p, d * D2p, ch->A[i], (long)d * D2p, 1, D1p, d * D2p, s));
@@ -1465,7 +1465,8 @@ int tn_copy_bench(tn_ctx* c, int64_t n, double* ms)
     ABI_END(c)
 }
 
-int tn_cholqr_test(tn_ctx* c, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted)
+int tn_cholqr_test(tn_ctx* c, int m, int n, const double* A, double tol, double* Q, double* R, int* shifted,
+                   int triangular)
 {
     if (!c || !A || !Q || !R) return TN_ERR_ARG;
     ABI_BEGIN
@@ -1482,7 +1483,8 @@ int tn_cholqr_test(tn_ctx* c, int m, int n, const double* A, double tol, double*
         for (int j = 0; j < n; j++) h[(size_t)i * np_ + j] = A[(size_t)i * n + j];
     h2d(c, a.p, h.data(), h.size() * 8);
     int sh = 0;
-    const bool ok = cholqr_device(c, a.as<double>(), np_, mp, np_, n, q.as<double>(), l.as<double>(), tol, &sh);
+    const bool ok =
+        cholqr_device(c, a.as<double>(), np_, mp, np_, n, q.as<double>(), l.as<double>(), tol, &sh, triangular != 0);
     d2h(c, h.data(), q.p, h.size() * 8);
     for (int i = 0; i < m; i++)
         for (int j = 0; j < n; j++) Q[(size_t)i * n + j] = h[(size_t)i * np_ + j];

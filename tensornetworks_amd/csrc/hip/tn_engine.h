@@ -178,9 +178,10 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int c_, doub
 // logical columns, zero padded columns, and Lacc (n x n row-major, lower)
 // satisfies A = Q Lacc^T.  Returns false if max|Q^T Q - I| > tol after the
 // passes (the caller falls back to SVD).  *shifted = 1 when the first pass
-// needed the shift.
+// needed the shift.  With triangular = false a first pass that is already
+// nearly orthogonal is finished by a polar correction (Lacc then full).
 bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q,
-                   double* Lacc, double tol, int* shifted = nullptr);
+                   double* Lacc, double tol, int* shifted = nullptr, bool triangular = true);
 
 void sync(tn_ctx* c);
 // stream-ordered copies that return when the data has arrived

@@ -834,6 +834,50 @@ static void qr_accumulate(tn_ctx* c, double* Lacc, const double* Lk, int n)
     TN_HIP(hipMemcpyAsync(Lacc, c->qr_T.p, (size_t)n * n * 8, hipMemcpyDeviceToDevice, c->stream));
 }
 
+// Second pass without a factorisation: with G = Q1^T Q1 = I + E and
+// |E|_F <= 1e-3, Q = Q1 G^{-1/2} and A = Q1 L1^T = Q (L1 G^{1/2})^T, the
+// inverse square root and the square root from their binomial series to E^3
+// (remainder below 35/128 |E|^4 <= 3e-13, so no Gram check follows).  The
+// factor is no longer triangular; a gauge move does not need it to be.
+// Returns false (nothing written) when |E| is too large for the series.
+static bool polar_finish(tn_ctx* c, const double* Q1, int m, int n, int nl, double* Q, double* Lacc)
+{
+    hipStream_t s = c->stream;
+    double* G = c->qr_G.as<double>();
+    c->qr_W.ensure((size_t)n * n * 8);
+    double* E = c->qr_T.as<double>();
+    double* E2 = c->qr_L.as<double>();
+    double* E3 = c->qr_linv.as<double>();
+    double* Xm = G;
+    double* Ym = c->qr_W.as<double>();
+    qr_gram(c, Q1, n, m, n, G);
+    TN_HIP(eye_sub(G, n, nl, E, s));
+    const long nn = (long)n * n;
+    const long pcap = mdot_part_size(nn, 1);
+    c->part.ensure(std::max<long>(pcap, 1024) * 8);
+    double* h = c->svd_s.as<double>();
+    TN_HIP(mdot(E, nn, 1, E, nn, h, c->part.as<double>(), (int)pcap, s));
+    double e2 = 0;
+    d2h(c, &e2, h, 8);
+    if (!(std::sqrt(e2) <= 1e-3)) return false;
+    GemmDesc g{};
+    g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
+    g.M = n; g.N = n; g.K = n; g.batch = 1; g.lda = n; g.ldb = n; g.ldc = n; g.alpha = 1.0f;
+    g.A = E; g.B = E; g.C = E2;
+    gemm_run(c, g);
+    g.A = E2; g.B = E; g.C = E3;
+    gemm_run(c, g);
+    const double ci[3] = {-0.5, 0.375, -0.3125}, cs[3] = {0.5, -0.125, 0.0625};
+    TN_HIP(series3(E, E2, E3, n, ci, cs, Xm, Ym, s));
+    GemmDesc q{};
+    q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = false;
+    q.M = m; q.N = n; q.K = n; q.batch = 1;
+    q.A = Q1; q.lda = n; q.B = Xm; q.ldb = n; q.C = Q; q.ldc = n; q.alpha = 1.0f;
+    gemm_run(c, q);
+    qr_accumulate(c, Lacc, Ym, n);
+    return true;
+}
+
 // Shifted CholeskyQR: every pass factors X^T X directly and, if a pivot
 // breaks down (cond(X) beyond ~1e8), again with the shift
 // 11 (m n + n (n+1)) u ||X||_F^2 (Fukaya, Kannan, Nakatsukasa, Yamamoto,
@@ -842,7 +886,7 @@ static void qr_accumulate(tn_ctx* c, double* Lacc, const double* Lk, int n)
 // matrix of the current Q is checked first and the result accepted when
 // max|Q^T Q - I| <= tol.  At most five passes.
 bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, double* Q, double* Lacc,
-                   double tol, int* shifted)
+                   double tol, int* shifted, bool triangular)
 {
     hipStream_t s = c->stream;
     TN_REQUIRE(m % 32 == 0 && n % 32 == 0 && m >= n && nl <= n, TN_ERR_ARG, "cholqr: bad shape");
@@ -895,6 +939,7 @@ bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, d
         if (pass > 1) qr_accumulate(c, Lacc, Lk, n);
         X = Y;
         ldx = n;
+        if (pass == 1 && !triangular && polar_finish(c, Q1, m, n, nl, Q, Lacc)) return true;
     }
 }
 

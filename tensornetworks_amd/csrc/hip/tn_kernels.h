@@ -138,6 +138,11 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long ldl, double* Linv, long ldi,
                      int* flag, hipStream_t s);
 hipError_t shift_diag(double* G, long ldg, int n, double factor, hipStream_t s);
+// E = G - I on the leading nl x nl block of an n x n matrix, zero elsewhere
+hipError_t eye_sub(const double* G, int n, int nl, double* E, hipStream_t s);
+// X = I + c1 E + c2 E2 + c3 E3, Y = I + d1 E + d2 E2 + d3 E3 (n x n)
+hipError_t series3(const double* E, const double* E2, const double* E3, int n, const double c[3], const double d[3],
+                   double* X, double* Y, hipStream_t s);
 
 // --- batched (one job per chain) kernels ---------------------------------
 struct CholJob {

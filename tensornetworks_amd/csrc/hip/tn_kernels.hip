@@ -1255,6 +1255,46 @@ hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long
     return hipGetLastError();
 }
 
+// E = G - I on the leading nl x nl block, zero elsewhere (n x n, ld n)
+__global__ void __launch_bounds__(256) eye_sub_k(const double* __restrict__ G, int n, int nl, double* __restrict__ E)
+{
+    const long nn = (long)n * n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += (long)gridDim.x * 256) {
+        const int i = (int)(e / n), j = (int)(e - (long)i * n);
+        E[e] = (i < nl && j < nl) ? G[e] - (i == j ? 1.0 : 0.0) : 0.0;
+    }
+}
+hipError_t eye_sub(const double* G, int n, int nl, double* E, hipStream_t s)
+{
+    long blocks = ((long)n * n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    eye_sub_k<<<blocks, 256, 0, s>>>(G, n, nl, E);
+    return hipGetLastError();
+}
+
+// X = I + c1 E + c2 E2 + c3 E3 and Y = I + d1 E + d2 E2 + d3 E3 (n x n)
+__global__ void __launch_bounds__(256) series3_k(const double* __restrict__ E, const double* __restrict__ E2,
+                                                 const double* __restrict__ E3, int n, double c1, double c2,
+                                                 double c3, double d1, double d2, double d3,
+                                                 double* __restrict__ X, double* __restrict__ Y)
+{
+    const long nn = (long)n * n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += (long)gridDim.x * 256) {
+        const int i = (int)(e / n), j = (int)(e - (long)i * n);
+        const double id = i == j ? 1.0 : 0.0, a = E[e], b = E2[e], c = E3[e];
+        X[e] = id + c1 * a + c2 * b + c3 * c;
+        Y[e] = id + d1 * a + d2 * b + d3 * c;
+    }
+}
+hipError_t series3(const double* E, const double* E2, const double* E3, int n, const double c[3], const double d[3],
+                   double* X, double* Y, hipStream_t s)
+{
+    long blocks = ((long)n * n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    series3_k<<<blocks, 256, 0, s>>>(E, E2, E3, n, c[0], c[1], c[2], d[0], d[1], d[2], X, Y);
+    return hipGetLastError();
+}
+
 // G[i][i] += factor * trace(G[0:n, 0:n]) for i < n  (the shift of shifted
 // CholeskyQR; trace(A^T A) = ||A||_F^2 bounds ||A||_2^2 from above)
 __global__ void __launch_bounds__(256) shift_diag_k(double* __restrict__ G, long ldg, int n, double factor)

@@ -228,3 +228,19 @@ def test_cholqr_rank_deficient(ctx):
     assert ok and shifted
     assert np.abs(Q.T @ Q - np.eye(128)).max() < 1e-11
     assert np.abs(Q @ R - A).max() < 1e-12
+
+
+@pytest.mark.parametrize("cond", [1.0, 1e3, 1e6, 1e12])
+def test_cholqr_polar_finish(ctx, cond):
+    # the gauge move's mode: any A = Q R with orthonormal Q (R need not be
+    # triangular); well-conditioned inputs take the polar finish, the others
+    # the further CholeskyQR passes
+    rng = np.random.default_rng(int(np.log10(cond)) + 11)
+    m, n = 1100, 700
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = (U * np.logspace(0, -np.log10(cond), n)) @ V.T
+    Q, R, ok, _ = ctx.cholqr_test(A, triangular=False)
+    assert ok
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-11
+    assert np.abs(Q @ R - A).max() < 1e-12 * np.abs(A).max() * np.sqrt(n)
