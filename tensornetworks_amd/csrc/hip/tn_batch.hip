@@ -83,7 +83,7 @@ void gemm_group(tn_batch* b, const std::vector<GemmDesc>& list)
                        g.a_kmajor == list[0].a_kmajor && g.b_kmajor == list[0].b_kmajor,
                    TN_ERR_ARG, "gemm_group: mixed problem types");
         int tile = gemm_choice(c, g).first;
-        if (g.M % gemm_tile_bm(tile) || g.N % gemm_tile_bn(tile)) tile = gemm_default_tile(g.M, g.N, 1);
+        if (!gemm_tile_ok(tile, g.tin, g.M, g.N, g.K)) tile = gemm_default_tile(g.M, g.N, 1);
         groups[{tile, gemm_k_step(g.tin, g.K)}].push_back(q);
     }
     for (const auto& kv : groups) {
@@ -93,7 +93,7 @@ void gemm_group(tn_batch* b, const std::vector<GemmDesc>& list)
         std::vector<int> off{0};
         for (int q : kv.second) {
             const GemmDesc& g = list[q];
-            TN_REQUIRE(g.K % bk == 0, TN_ERR_ARG, "gemm_group: K not a multiple of the step");
+            TN_REQUIRE(gemm_tile_ok(tile, g.tin, g.M, g.N, g.K), TN_ERR_ARG, "gemm_group: tile does not fit");
             probs.push_back(GemmProb{g.A, g.B, g.C, g.lda, g.ldb, g.ldc, g.M, g.N, g.K, g.alpha, g.accumulate});
             off.push_back(off.back() + (g.M / bm) * (g.N / bn));
         }

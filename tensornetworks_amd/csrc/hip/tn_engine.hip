@@ -82,7 +82,6 @@ void gemm_split(tn_ctx* c, const GemmDesc& g, int ks)
 static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
 {
     hipStream_t s = c->stream;
-    static const int bm[] = {128, 128, 64, 64, 32, 64, 32}, bn[] = {128, 64, 128, 64, 64, 32, 32};
     const int nt = gemm_num_tiles();
     const size_t es = dtype_size(g.tout);
     const size_t span = (size_t)(g.batch - 1) * g.sC + (size_t)(g.M - 1) * g.ldc + g.N;
@@ -94,8 +93,8 @@ static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
     TN_HIP(hipEventCreate(&e1));
     std::pair<int, int> best{0, 1};
     float best_ms = 1e30f;
-    for (int code = 1; code <= nt && code <= 7; code++) {
-        if (g.M % bm[code - 1] || g.N % bn[code - 1]) continue;
+    for (int code = 1; code <= nt; code++) {
+        if (!gemm_tile_ok(code, g.tin, g.M, g.N, g.K)) continue;
         for (int ks : {1, 2, 4}) {
             if (ks > 1 && (g.batch != 1 || g.K % (64 * ks))) continue;
             t.tile = code;

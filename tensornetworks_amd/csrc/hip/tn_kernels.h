@@ -50,6 +50,8 @@ struct GemmProb {
 hipError_t gemm_grouped(int tin, int tout, bool akm, bool bkm, int tile, int bk, const GemmProb* probs,
                         const int* off, int nprob, int total_tiles, hipStream_t s);
 int gemm_tile_bm(int code);
+// tile code usable for an M x N x K problem of input type tin
+bool gemm_tile_ok(int code, int tin, int M, int N, int K);
 int gemm_tile_bn(int code);
 int gemm_default_tile(int M, int N, int batch);
 int gemm_k_step(int tin, int K);

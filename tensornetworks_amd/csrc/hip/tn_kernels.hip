@@ -292,11 +292,14 @@ __global__ void __launch_bounds__(256) gemm_grouped_k(const GemmProb* __restrict
     }
 }
 
-// tile menu (code -> BM x BN)
-struct TileCfg { int bm, bn; const char* name; };
+// tile menu (code -> BM x BN; "deep" tiles stage twice the K slice per LDS
+// round, halving the barriers per flop at twice the LDS footprint)
+struct TileCfg { int bm, bn; bool deep; const char* name; };
 static const TileCfg kTiles[] = {
-    {128, 128, "128x128"}, {128, 64, "128x64"}, {64, 128, "64x128"},
-    {64, 64, "64x64"},     {32, 64, "32x64"},   {64, 32, "64x32"},   {32, 32, "32x32"},
+    {128, 128, false, "128x128"}, {128, 64, false, "128x64"}, {64, 128, false, "64x128"},
+    {64, 64, false, "64x64"},     {32, 64, false, "32x64"},   {64, 32, false, "64x32"},
+    {32, 32, false, "32x32"},     {128, 128, true, "128x128d"}, {128, 64, true, "128x64d"},
+    {64, 128, true, "64x128d"},
 };
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int gemm_num_tiles() { return kNumTiles; }
@@ -314,7 +317,10 @@ static hipError_t launch_tile(int code, const GemmParams& p, int batch, hipStrea
     case 3: gemm_k<TI, TO, 64, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
     case 4: gemm_k<TI, TO, 32, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
     case 5: gemm_k<TI, TO, 64, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
-    default: gemm_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 6: gemm_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 7: gemm_k<TI, TO, 128, 128, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    case 8: gemm_k<TI, TO, 128, 64, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
+    default: gemm_k<TI, TO, 64, 128, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(p); break;
     }
     return hipGetLastError();
 }
@@ -330,7 +336,10 @@ static hipError_t launch_grouped(int code, const GemmProb* probs, const int* off
     case 3: gemm_grouped_k<TI, TO, 64, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
     case 4: gemm_grouped_k<TI, TO, 32, 64, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
     case 5: gemm_grouped_k<TI, TO, 64, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
-    default: gemm_grouped_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 6: gemm_grouped_k<TI, TO, 32, 32, BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 7: gemm_grouped_k<TI, TO, 128, 128, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    case 8: gemm_grouped_k<TI, TO, 128, 64, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
+    default: gemm_grouped_k<TI, TO, 64, 128, 2 * BK, AKM, BKM><<<grid, 256, 0, s>>>(probs, off, nprob); break;
     }
     return hipGetLastError();
 }
@@ -363,6 +372,14 @@ static hipError_t grouped_layout(int tin, int tout, int code, int bk, const Gemm
 }
 
 static int choose_tile(int M, int N, int batch);
+bool gemm_tile_ok(int code, int tin, int M, int N, int K)
+{
+    if (code < 1 || code > kNumTiles) return false;
+    const TileCfg& t = kTiles[code - 1];
+    if (M % t.bm || N % t.bn) return false;
+    const int bk = gemm_k_step(tin, K);
+    return K % (t.deep ? 2 * bk : bk) == 0;
+}
 int gemm_tile_bm(int code) { return (code >= 1 && code <= kNumTiles) ? kTiles[code - 1].bm : 0; }
 int gemm_tile_bn(int code) { return (code >= 1 && code <= kNumTiles) ? kTiles[code - 1].bn : 0; }
 int gemm_default_tile(int M, int N, int batch) { return choose_tile(M, N, batch) + 1; }
@@ -392,7 +409,7 @@ static int choose_tile(int M, int N, int batch)
     int best = -1;
     long best_blocks = -1;
     for (int c = 0; c < kNumTiles; c++) {
-        if (M % kTiles[c].bm || N % kTiles[c].bn) continue;
+        if (kTiles[c].deep || M % kTiles[c].bm || N % kTiles[c].bn) continue;
         long blocks = (long)(M / kTiles[c].bm) * (N / kTiles[c].bn) * batch;
         if (blocks >= 384) return c;
         if (blocks > best_blocks) { best_blocks = blocks; best = c; }
@@ -412,6 +429,7 @@ static hipError_t gemm_typed(const GemmDesc& g, hipStream_t s)
     int code = g.tile > 0 ? g.tile - 1 : choose_tile(g.M, g.N, g.batch);
     if (code < 0 || code >= kNumTiles || g.M % kTiles[code].bm || g.N % kTiles[code].bn)
         return hipErrorInvalidValue;
+    if (kTiles[code].deep && g.K % (2 * gemm_k_step(g.tin, g.K))) return hipErrorInvalidValue;
     if constexpr (std::is_same<TI, double>::value) {
         if (g.K % 16) return hipErrorInvalidValue;
         return launch_tile<TI, TO, 16, AKM, BKM>(code, p, g.batch, s);

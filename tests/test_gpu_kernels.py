@@ -53,10 +53,10 @@ def test_gemm_matches_rounded_fp64(ctx, tin, tout, akm, bkm):
     assert err < tol, (tin, tout, akm, bkm, err)
 
 
-@pytest.mark.parametrize("tile", range(1, 8))
-@pytest.mark.parametrize("tin", [0, 2])
+@pytest.mark.parametrize("tile", range(1, 11))
+@pytest.mark.parametrize("tin", [0, 1, 2])
 def test_gemm_every_tile(ctx, tin, tile):
-    M, N, K = 128, 128, 64
+    M, N, K = 128, 128, 256
     A = rand((1, M, K), 3)
     B = rand((1, K, N), 4)
     ref = ROUND[tin](A) @ ROUND[tin](B)
