@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--max-krylov", type=int, default=24)
     p.add_argument("--max-restarts", type=int, default=0)
     p.add_argument("--gauge", default="qr", choices=["qr", "svd"])
+    p.add_argument("--no-skip-identity", action="store_true",
+                   help="contract identity environment channels like the others instead of copying them")
     p.add_argument("--no-fp64-compare", action="store_true")
     p.add_argument("--no-autotune", action="store_true", help="built-in GEMM tile choice instead of per-shape tuning")
     p.add_argument("--tune-db", default=os.path.join(ROOT, "tensornetworks_amd", "tuning", "mi355x_gemm.txt"),
@@ -282,7 +284,8 @@ def main():
         ch.normalize_qr(1)
     else:
         ch.normalize(1)
-    common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, gauge=gauge)
+    common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, gauge=gauge,
+                  skip_identity=0 if a.no_skip_identity else 1)
     if a.precision == "mixed":
         o_mixed = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
                                        refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
@@ -296,7 +299,7 @@ def main():
     print(f"# setup {setup_s:.1f}s", file=sys.stderr, flush=True)
     if a.heff_only:
         site = a.L // 2
-        s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
+        s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10, skip_identity=not a.no_skip_identity)
         print(f"# heff bench site {site} dims {ch.dims()[0][site]}x{ch.dims()[1][site]}: stage1 {s1:.4f} ms "
               f"stage3 {s3:.4f} ms mix {mix:.4f} ms", file=sys.stderr, flush=True)
         if not a.no_autotune:
@@ -328,7 +331,7 @@ def main():
           f"(fp64 {st.n_matvec_fp64}) qr={st.n_qr} shifted={st.n_qr_shifted} fallback={st.n_qr_fallback}",
           file=sys.stderr, flush=True)
     site = a.L // 2
-    s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10)
+    s1, mix, s3, f1, f3 = ch.heff_bench(site, prec, reps=10, skip_identity=not a.no_skip_identity)
     print(f"# heff bench site {site}: stage1 {s1:.3f} ms stage3 {s3:.3f} ms mix {mix:.3f} ms",
           file=sys.stderr, flush=True)
     def timed_from_saved(opts):
@@ -375,9 +378,11 @@ def main():
         print("# tuned GEMM shapes:\n" + report, file=sys.stderr, flush=True)
         if a.tune_save:
             ctx.tune_save(a.tune_save)
-        # tile of the bulk-site stage-1 GEMM: x (D*d x D) times R^T (Dw*D x D)
+        # tile of the bulk-site stage-1 GEMM: x (D*d x D) times R^T (nR*D x D),
+        # nR = right channels contracted (Dw, or Dw - 1 with the identity copied)
         tname = {"fp32": "f32->f32", "fp64": "f64->f64"}.get(a.precision, "bf16->f32")
-        key = f"{tname} KK M={a.D * d} N={5 * a.D} K={a.D} "
+        nR = int(round(f1 / (2.0 * d * a.D ** 3)))
+        key = f"{tname} KK M={a.D * d} N={nR * a.D} K={a.D} "
         for line in report.splitlines():
             if line.startswith(key):
                 stage1_tile = line.split(": tile ")[1]
@@ -414,13 +419,16 @@ def main():
                                + "), FP64 Krylov/Ritz/gauge/energy",
                    "precision": a.precision,
                    "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov,
-                   "max_restarts": a.max_restarts, "gauge": a.gauge, "autotune": not a.no_autotune},
+                   "max_restarts": a.max_restarts, "gauge": a.gauge, "autotune": not a.no_autotune,
+                   "skip_identity": not a.no_skip_identity},
         "sweep_s": wall / a.steps,
         "E_per_site": E_mixed / a.L,
         "breakdown_ms": {"heff": st.heff_ms, "heff_gemm": st.heff_gemm_ms, "lanczos": st.lanczos_ms,
                          "env": st.env_ms, "gauge": st.svd_ms, "wall": st.wall_ms},
         "n_qr": st.n_qr, "n_qr_fallback": st.n_qr_fallback, "n_qr_shifted": st.n_qr_shifted,
         "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64,
+        "heff_gemm_tflops_launched": st.heff_gemm_flops / max(st.heff_gemm_ms, 1e-12) / 1e9,
+        "n_ident_left": st.n_ident_left, "n_ident_right": st.n_ident_right,
         "ms_per_matvec": st.heff_ms / max(st.n_matvec, 1),
         "fp64": res_fp64,
         "bf16_then_fp32": res_bf16,

@@ -140,6 +140,11 @@ typedef struct tn_sweep_opts {
     int precond;         /* Davidson correction: 0 none, 1 diagonal (Heff diagonal),
                             2 learned (network set by tn_ctx_set_mlp) */
     double precond_delta;  /* safeguard of the diagonal shift |diag - theta| >= delta */
+    int skip_identity;   /* 1 = an environment channel equal to the identity (the
+                            "nothing placed yet" / "all placed" MPO channel of an
+                            orthonormal block) is applied as a copy instead of a GEMM
+                            slice; checked per local solve against the identity
+                            (1e-12 for FP64 environments, 1e-4 for FP32 ones) */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {
@@ -150,13 +155,16 @@ typedef struct tn_sweep_stats {
     double heff_flops;         /* algorithmic flops of all Heff applies */
     double heff_ms;            /* device time of all Heff applies (HIP events) */
     double heff_gemm_ms;       /* device time of the two GEMM stages only */
-    double heff_gemm_flops;    /* algorithmic flops of the two GEMM stages */
+    double heff_gemm_flops;    /* flops of the two GEMM stages as launched (identity
+                                  environment channels excluded) */
     double env_flops, env_ms;
     double svd_ms, lanczos_ms, wall_ms;
     double energy;             /* last site eigenvalue */
     int64_t n_qr;              /* QR gauge moves */
     int64_t n_qr_fallback;     /* QR gauge moves that fell back to the SVD */
     int64_t n_qr_shifted;      /* QR gauge moves whose first pass needed the shift */
+    int64_t n_ident_left;      /* local solves whose left environment had an identity channel */
+    int64_t n_ident_right;     /* ... right environment */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);
@@ -194,10 +202,12 @@ int tn_chain_sweep2(tn_chain* chain, int dir, int Dmax, double eps_sv,
 /* Timing of `reps` Heff applications at one site of the chain in the given
  * precision (HIP events on the library stream): average device time of the
  * first GEMM stage, the MPO stage and the second GEMM stage, in ms, and the
- * algorithmic flops of one GEMM stage launch (2 d Dw D1 D2 max(D1,D2) form).
+ * flops of one launch of each GEMM stage as launched.  skip_identity = 1 uses
+ * the operator a sweep with tn_sweep_opts.skip_identity = 1 would build
+ * (identity environment channels found with env_precision's tolerance).
  * The site tensor is left unchanged. */
-int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, double* s1_ms,
-                        double* mix_ms, double* s3_ms, double* s1_flops, double* s3_flops);
+int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, int skip_identity, int env_precision,
+                        double* s1_ms, double* mix_ms, double* s3_ms, double* s1_flops, double* s3_flops);
 
 /* One batched GEMM through the device kernel, for numerics tests of the
  * matrix-core path: C[b] = A[b] B[b] with operands rounded to tin

@@ -37,6 +37,7 @@ class SweepOpts(ctypes.Structure):
         ("eig_method", ctypes.c_int),
         ("precond", ctypes.c_int),
         ("precond_delta", ctypes.c_double),
+        ("skip_identity", ctypes.c_int),
     ]
 
     @classmethod
@@ -67,6 +68,8 @@ class SweepStats(ctypes.Structure):
         ("n_qr", ctypes.c_int64),
         ("n_qr_fallback", ctypes.c_int64),
         ("n_qr_shifted", ctypes.c_int64),
+        ("n_ident_left", ctypes.c_int64),
+        ("n_ident_right", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -108,7 +111,7 @@ def _load():
         "tn_batch_sweep": (i, [v, i, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
         "tn_ctx_tune_save": (i, [v, ctypes.c_char_p]),
         "tn_ctx_tune_report": (i, [v, ctypes.c_char_p, i]),
-        "tn_chain_heff_bench": (i, [v, i, i, i, _dp, _dp, _dp, _dp, _dp]),
+        "tn_chain_heff_bench": (i, [v, i, i, i, i, i, _dp, _dp, _dp, _dp, _dp]),
         "tn_chain_create": (i, [v, i, i, _ip, _ip, _dp, i, i, vp]),
         "tn_chain_destroy": (i, [v]),
         "tn_chain_initialize": (i, [v, i, u64]),
@@ -423,10 +426,16 @@ class Chain:
         k = nout.value
         return X[:k], a[:k], t[:k]
 
-    def heff_bench(self, site, precision, reps=10):
-        """Average device ms of (GEMM 1, MPO stage, GEMM 2) and flops per GEMM launch."""
+    def heff_bench(self, site, precision, reps=10, skip_identity=True, env_precision=None):
+        """Average device ms of (GEMM 1, MPO stage, GEMM 2) and flops per GEMM
+        launch, for the operator a sweep builds (identity environment channels
+        applied as copies when skip_identity; env_precision defaults to the
+        operator precision, FP32 for BF16)."""
+        if env_precision is None:
+            env_precision = TN_FP32 if precision == TN_BF16 else precision
         vals = [ctypes.c_double() for _ in range(5)]
-        self.ctx.check(lib.tn_chain_heff_bench(self.h, site, precision, reps, *[ctypes.byref(x) for x in vals]),
+        self.ctx.check(lib.tn_chain_heff_bench(self.h, site, precision, reps, int(bool(skip_identity)),
+                                               env_precision, *[ctypes.byref(x) for x in vals]),
                        "tn_chain_heff_bench")
         return [x.value for x in vals]
 

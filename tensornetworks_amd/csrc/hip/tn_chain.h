@@ -50,7 +50,18 @@ double now_ms();
 void ensure_lo(tn_chain* ch, int prec);
 const void* lo_left(tn_chain* ch, int prec, int i);
 const void* lo_right(tn_chain* ch, int prec, int i);
-HeffOp make_heff(tn_chain* ch, int i, int prec, int tile);
+// operator of site i; wl/wr: identity environment channels (see HeffOp), -1 none
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl = -1, int wr = -1);
+// identity channels of the environments of site i (first or last channel
+// within tol of the identity on the logical block), -1 where there is none
+void env_identity(tn_chain* ch, int i, double tol, int* wl, int* wr);
+// largest deviation from the identity (elementwise, logical block) accepted for
+// an environment channel applied as the identity: environments contracted in
+// FP64, and in FP32 (whose rounding, accumulated along the chain, reaches ~1e-5
+// at D = 1024 while the sites themselves are orthonormal to FP64 accuracy)
+constexpr double kIdentTolFp64 = 1e-12;
+constexpr double kIdentTolLow = 1e-4;
+inline double ident_tol(int env_prec) { return env_prec == TN_FP64 ? kIdentTolFp64 : kIdentTolLow; }
 void rescale_site(tn_chain* ch, int i);
 void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge);
 void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_stats* st);

@@ -163,17 +163,44 @@ void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y)
     if (timed) {
         heff_timing_read(c, &hc->st->heff_gemm_ms, &hc->st->heff_ms);
         hc->st->heff_flops += hc->op.flops_alg(hc->D1, hc->D2);
-        {
-            const HeffOp& o = hc->op;
-            const double D1 = hc->D1, D2 = hc->D2;
-            hc->st->heff_gemm_flops += 2.0 * D1 * o.p * o.Dw2 * D2 * D2 + 2.0 * D1 * o.p * D2 * o.Dw1 * D1;
-        }
+        hc->st->heff_gemm_flops += hc->op.flops_gemm(hc->D1, hc->D2);
         hc->st->n_matvec++;
         if (hc->op.precision == TN_FP64) hc->st->n_matvec_fp64++;
     }
 }
 
-HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
+// Identity channels of the environments of site i: for the first and the last
+// channel of L (layout [D1p][Dw1][D1p]) and of R ([Dw2][D2p][D2p]), the largest
+// deviation from the identity on the logical block (the padding is zero by
+// construction).  A channel within tol of the identity is applied as one.
+void env_identity(tn_chain* ch, int i, double tol, int* wl, int* wr)
+{
+    tn_ctx* c = ch->ctx;
+    hipStream_t s = c->stream;
+    const int D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+    const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+    *wl = *wr = -1;
+    if (Dw1 < 2 && Dw2 < 2) return;
+    const double* L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
+    const double* R = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
+    c->ident.ensure(4 * 8);
+    c->part.ensure(1024 * 8);
+    double* dv = c->ident.as<double>();
+    double* part = c->part.as<double>();
+    const int chs[2] = {0, -1};
+    for (int k = 0; k < 2; k++) {
+        const int wlk = chs[k] < 0 ? Dw1 - 1 : 0, wrk = chs[k] < 0 ? Dw2 - 1 : 0;
+        TN_HIP(eye_deviation(L + (long)wlk * D1p, (long)Dw1 * D1p, D1, dv + k, part, s));
+        TN_HIP(eye_deviation(R + (long)wrk * D2p * D2p, D2p, D2, dv + 2 + k, part, s));
+    }
+    double h[4];
+    d2h(c, h, dv, sizeof(h));
+    auto ok = [&](double v) { return std::isfinite(v) && v <= tol; };
+    if (Dw1 >= 2) *wl = ok(h[0]) ? 0 : (ok(h[1]) ? Dw1 - 1 : -1);
+    if (Dw2 >= 2) *wr = ok(h[2]) ? 0 : (ok(h[3]) ? Dw2 - 1 : -1);
+}
+
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl, int wr)
 {
     HeffOp op;
     op.p = ch->d;
@@ -186,7 +213,10 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile)
     op.precision = prec;
     op.Llo = lo_left(ch, prec, i);
     op.Rlo = lo_right(ch, prec, i);
-    mix_build(ch->mixH[i], MIX_HEFF, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream);
+    op.wl_id = wl;
+    op.wr_id = wr;
+    mix_build(ch->mixH[i], MIX_HEFF, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream,
+              op.wl_id, op.wr_id);
     op.mix = &ch->mixH[i];
     op.tile = tile;
     return op;
@@ -211,8 +241,14 @@ const double* w_diag(tn_chain* ch, int i)
 void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1)
 {
     tn_ctx* c = ch->ctx;
+    int wl = -1, wr = -1;
+    if (o.skip_identity) env_identity(ch, i, ident_tol(o.env_precision), &wl, &wr);
+    if (st) {
+        st->n_ident_left += wl >= 0;
+        st->n_ident_right += wr >= 0;
+    }
     HeffCall hc;
-    hc.op = make_heff(ch, i, o.precision, o.gemm_tile);
+    hc.op = make_heff(ch, i, o.precision, o.gemm_tile, wl, wr);
     hc.st = st;
     hc.D1 = ch->D1[i];
     hc.D2 = ch->D2[i];
@@ -223,7 +259,8 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     // when a refinement follows, stop there and let the refinement finish
     if (o.eig_method == 1) {
         const double* Wd = w_diag(ch, i);
-        DavidsonResult dr = davidson(c, heff_cb, &hc, make_heff(ch, i, TN_FP64, o.gemm_tile), ch->D1[i], ch->D2[i],
+        DavidsonResult dr = davidson(c, heff_cb, &hc, make_heff(ch, i, TN_FP64, o.gemm_tile, wl, wr), ch->D1[i],
+                                     ch->D2[i],
                                      Wd, ch->A[i],
                                      o.eig_tol, o.max_krylov, o.max_krylov * (1 + o.max_restarts), o.precond,
                                      o.precond_delta);
@@ -238,7 +275,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     if (refining) {
         // polish the reduced-precision Ritz vector with a more accurate operator
         HeffCall hr = hc;
-        hr.op = make_heff(ch, i, o.refine_precision, o.gemm_tile);
+        hr.op = make_heff(ch, i, o.refine_precision, o.gemm_tile, wl, wr);
         LanczosResult r2 = lanczos(c, heff_cb, &hr, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts);
         r.e0 = r2.e0;
         r.e1 = r2.e1;
@@ -648,7 +685,8 @@ int tn_ctx_destroy(tn_ctx* c)
                         &c->svd_V, &c->svd_U, &c->svd_Vt, &c->svd_s, &c->svd_perm, &c->svd_pairs,
                         &c->svd_flag, &c->svd_A, &c->spare, &c->pack, &c->qr_W, &c->qr_G, &c->qr_L,
                         &c->qr_T, &c->qr_linv, &c->qr_Q, &c->tune_C, &c->splitk, &c->b_L, &c->b_R,
-                        &c->b_Llo, &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M};
+                        &c->b_Llo, &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M, &c->dav_W, &c->dav_vec,
+                        &c->dav_wd, &c->mlp, &c->ident};
     for (auto* b : bufs) b->release();
     c->b_mix_heff.buf.release();
     c->b_mix_envl.buf.release();
@@ -679,6 +717,7 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->eig_method = 0;
     o->precond = 1;
     o->precond_delta = 1e-2;
+    o->skip_identity = 1;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
@@ -740,7 +779,25 @@ int tn_site_bind(tn_ctx* c, int d, int D1, int D2, int Dw1, int Dw2, const doubl
         make_lowprec(c, precision, c->b_L.as<double>(), c->b_Llo.p, (long)Li.size());
         make_lowprec(c, precision, c->b_R.as<double>(), c->b_Rlo.p, (long)Ri.size());
     }
-    mix_build(c->b_mix_heff, MIX_HEFF, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream);
+    // first or last channel equal to the identity (FP64 data: within 1e-12)
+    auto dev_from_eye = [](const std::vector<double>& E, long sw, long sr, int w, int D) {
+        double m = 0.0;
+        for (int i = 0; i < D; i++)
+            for (int j = 0; j < D; j++) {
+                const double v = E[w * sw + i * sr + j] - (i == j ? 1.0 : 0.0);
+                m = std::isfinite(v) ? std::max(m, std::fabs(v)) : INFINITY;
+            }
+        return m;
+    };
+    auto pick = [&](const std::vector<double>& E, long sw, long sr, int Dw, int D) {
+        if (Dw < 2) return -1;
+        if (dev_from_eye(E, sw, sr, 0, D) <= kIdentTolFp64) return 0;
+        if (dev_from_eye(E, sw, sr, Dw - 1, D) <= kIdentTolFp64) return Dw - 1;
+        return -1;
+    };
+    c->b_wl = pick(Li, D1p, (long)Dw1 * D1p, Dw1, D1);
+    c->b_wr = pick(Ri, (long)D2p * D2p, D2p, Dw2, D2);
+    mix_build(c->b_mix_heff, MIX_HEFF, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream, c->b_wl, c->b_wr);
     const long n = (long)D1p * d * D2p;
     c->b_x.ensure(n * 8);
     c->b_y.ensure(n * 8);
@@ -765,6 +822,8 @@ static HeffOp bound_op(tn_ctx* c)
     op.precision = c->b_prec;
     op.mix = &c->b_mix_heff;
     op.tile = 0;
+    op.wl_id = c->b_wl;
+    op.wr_id = c->b_wr;
     return op;
 }
 
@@ -1373,13 +1432,15 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
     ABI_END(c)
 }
 
-int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, double* s1_ms, double* mix_ms,
-                        double* s3_ms, double* s1_flops, double* s3_flops)
+int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, int skip_identity, int env_precision,
+                        double* s1_ms, double* mix_ms, double* s3_ms, double* s1_flops, double* s3_flops)
 {
     if (!ch || site < 0 || site >= ch->L || reps < 1) return TN_ERR_ARG;
     tn_ctx* c = ch->ctx;
     ABI_BEGIN
-    HeffOp op = make_heff(ch, site, precision, 0);
+    int wl = -1, wr = -1;
+    if (skip_identity) env_identity(ch, site, ident_tol(env_precision), &wl, &wr);
+    HeffOp op = make_heff(ch, site, precision, 0, wl, wr);
     const long N = op.n();
     c->b_y.ensure((size_t)N * 8);
     double a = 0, b = 0, e = 0;
@@ -1397,8 +1458,8 @@ int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, double*
     if (s1_ms) *s1_ms = a / reps;
     if (mix_ms) *mix_ms = b / reps;
     if (s3_ms) *s3_ms = e / reps;
-    if (s1_flops) *s1_flops = 2.0 * D1 * op.p * op.Dw2 * D2 * D2;
-    if (s3_flops) *s3_flops = 2.0 * D1 * op.p * D2 * op.Dw1 * D1;
+    if (s1_flops) *s1_flops = 2.0 * D1 * op.p * (op.Dw2 - (wr >= 0 ? 1 : 0)) * D2 * D2;
+    if (s3_flops) *s3_flops = 2.0 * D1 * op.p * D2 * (op.Dw1 - (wl >= 0 ? 1 : 0)) * D1;
     return TN_OK;
     ABI_END(c)
 }

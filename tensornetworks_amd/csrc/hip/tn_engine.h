@@ -53,12 +53,16 @@ struct DevMix {
     MixTable t{};
     // key
     const double* wsrc = nullptr;
-    int op = -1, D1p = -1, D2p = -1, p = -1, dw1 = -1, dw2 = -1;
+    int op = -1, D1p = -1, D2p = -1, p = -1, dw1 = -1, dw2 = -1, wl = -1, wr = -1;
 };
 enum MixOp { MIX_HEFF = 0, MIX_ENVL = 1, MIX_ENVR = 2 };
-// W host array [dw1][dw2][p][p]; builds (or reuses) the table for the op/dims
+// W host array [dw1][dw2][p][p]; builds (or reuses) the table for the op/dims.
+// MIX_HEFF with wl >= 0 or wr >= 0: table for the operator whose left channel
+// wl / right channel wr is the identity (first or last channel), with the
+// stage-1 output holding only the other right channels and the stage-3 input
+// only the other left channels (see HeffOp).
 void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
-               hipStream_t s);
+               hipStream_t s, int wl = -1, int wr = -1);
 
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -82,8 +86,11 @@ struct tn_ctx {
     tn::DBuf tune_C, splitk;
     // Davidson workspace and the preconditioner network (params + norm, FP32)
     tn::DBuf dav_W, dav_vec, dav_wd, mlp;
+    // identity-channel check of the environments (4 device doubles)
+    tn::DBuf ident;
     // seam state (tn_site_bind)
     int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
+    int b_wl = -1, b_wr = -1;  // identity channels of the bound environments
     std::vector<double> b_W;
     tn::DBuf b_L, b_R, b_Llo, b_Rlo, b_x, b_y, b_M;
     tn::DevMix b_mix_heff, b_mix_envl, b_mix_envr;
@@ -111,10 +118,18 @@ struct HeffOp {
     const void* Llo;  // low precision copies (nullptr when precision == FP64)
     const void* Rlo;
     int precision;
-    const DevMix* mix;  // MIX_HEFF table for (D1p, D2p)
+    const DevMix* mix;  // MIX_HEFF table for (D1p, D2p, wl_id, wr_id)
     int tile;
+    // identity environment channels (-1: none).  L[a][wl_id][a'] = delta(a,a')
+    // and R[wr_id][b][b'] = delta(b,b') on the logical block (zero padding):
+    // stage 1 skips channel wr_id (its slice is x itself), stage 3 skips
+    // channel wl_id (its contribution is the mixed slice copied into y).
+    // Each must be 0 or the last channel of its side, with Dw >= 2.
+    int wl_id = -1, wr_id = -1;
     long n() const { return (long)D1p * p * D2p; }
     double flops_alg(int D1, int D2) const;  // reference formula on logical dims
+    // flops of the two GEMM stages as launched (identity channels excluded)
+    double flops_gemm(int D1, int D2) const;
 };
 // y = Heff x (fp64 in/out).  gemm_ms/total_ms accumulate device times when
 // timing != nullptr (events recorded; read after the next sync).

@@ -141,12 +141,19 @@ void gemm_run(tn_ctx* c, const GemmDesc& g)
 }
 
 void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
-               hipStream_t s)
+               hipStream_t s, int wl, int wr)
 {
+    if (op != MIX_HEFF) wl = wr = -1;
     if (m.wsrc == W && m.op == op && m.D1p == D1p && m.D2p == D2p && m.p == p && m.dw1 == dw1 &&
-        m.dw2 == dw2)
+        m.dw2 == dw2 && m.wl == wl && m.wr == wr)
         return;
-    std::vector<int> optr;
+    const bool ident = wl >= 0 || wr >= 0;
+    // right channels kept in the stage-1 output and left channels kept in the
+    // stage-3 input, renumbered without the identity channel
+    const int nR = dw2 - (wr >= 0 ? 1 : 0);
+    auto cl = [&](int w) { return w - (wl >= 0 && w > wl ? 1 : 0); };
+    auto cr = [&](int w) { return w - (wr >= 0 && w > wr ? 1 : 0); };
+    std::vector<int> optr, esel, osel;
     std::vector<long> ooff, eoff;
     std::vector<double> ev;
     auto Wv = [&](int w, int wp, int mm, int n) { return W[(((long)w * dw2 + wp) * p + mm) * p + n]; };
@@ -154,13 +161,25 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     if (op == MIX_HEFF || op == MIX_ENVR) {
         for (int w = 0; w < dw1; w++)
             for (int mm = 0; mm < p; mm++) {
-                ooff.push_back(op == MIX_HEFF ? (long)w * D1p * p * D2p + (long)mm * D2p
-                                              : (long)w * p * D2p + (long)mm * D2p);
+                if (op == MIX_HEFF && w == wl) {
+                    ooff.push_back((long)mm * D2p);  // y[a][mm][b]
+                    osel.push_back(1);
+                } else {
+                    ooff.push_back(op == MIX_HEFF ? (long)cl(w) * D1p * p * D2p + (long)mm * D2p
+                                                  : (long)w * p * D2p + (long)mm * D2p);
+                    osel.push_back(0);
+                }
                 for (int wp = 0; wp < dw2; wp++)
                     for (int n = 0; n < p; n++) {
                         double v = Wv(w, wp, mm, n);
                         if (v != 0.0) {
-                            eoff.push_back((long)n * dw2 * D2p + (long)wp * D2p);
+                            if (wp == wr) {
+                                eoff.push_back((long)n * D2p);  // x[a'][n][b]
+                                esel.push_back(1);
+                            } else {
+                                eoff.push_back((long)n * nR * D2p + (long)cr(wp) * D2p);
+                                esel.push_back(0);
+                            }
                             ev.push_back(v);
                         }
                     }
@@ -183,11 +202,12 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     }
     const int nout = (int)ooff.size(), nnz = (int)ev.size();
     std::vector<float> evf(ev.begin(), ev.end());
-    // layout: optr | ooff | eoff | eval | evalf  (each 256-byte aligned)
+    // layout: optr | ooff | eoff | eval | evalf | esel | osel  (each 256-byte aligned)
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o1 = 0, o2 = o1 + al((nout + 1) * sizeof(int)), o3 = o2 + al(nout * sizeof(long) + 8),
            o4 = o3 + al(nnz * sizeof(long) + 8), o5 = o4 + al(nnz * sizeof(double) + 8),
-           tot = o5 + al(nnz * sizeof(float) + 8);
+           o6 = o5 + al(nnz * sizeof(float) + 8), o7 = o6 + al(nnz * sizeof(int) + 8),
+           tot = o7 + al(nout * sizeof(int) + 8);
     std::vector<char> host(tot, 0);
     std::memcpy(host.data() + o1, optr.data(), (nout + 1) * sizeof(int));
     if (nout) std::memcpy(host.data() + o2, ooff.data(), nout * sizeof(long));
@@ -195,6 +215,10 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
         std::memcpy(host.data() + o3, eoff.data(), nnz * sizeof(long));
         std::memcpy(host.data() + o4, ev.data(), nnz * sizeof(double));
         std::memcpy(host.data() + o5, evf.data(), nnz * sizeof(float));
+    }
+    if (ident) {
+        if (nnz) std::memcpy(host.data() + o6, esel.data(), nnz * sizeof(int));
+        if (nout) std::memcpy(host.data() + o7, osel.data(), nout * sizeof(int));
     }
     m.buf.ensure(tot);
     TN_HIP(hipMemcpyAsync(m.buf.p, host.data(), tot, hipMemcpyHostToDevice, s));
@@ -207,6 +231,8 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     m.t.eoff = (const long*)(b + o3);
     m.t.eval = (const double*)(b + o4);
     m.t.evalf = (const float*)(b + o5);
+    m.t.esel = ident ? (const int*)(b + o6) : nullptr;
+    m.t.osel = ident ? (const int*)(b + o7) : nullptr;
     m.wsrc = W;
     m.op = op;
     m.D1p = D1p;
@@ -214,6 +240,8 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     m.p = p;
     m.dw1 = dw1;
     m.dw2 = dw2;
+    m.wl = wl;
+    m.wr = wr;
 }
 
 // ---------------------------------------------------------------------------
@@ -227,50 +255,73 @@ double HeffOp::flops_alg(int D1, int D2) const
     return s1 + s2 + s3;
 }
 
+double HeffOp::flops_gemm(int D1, int D2) const
+{
+    const int n2 = Dw2 - (wr_id >= 0 ? 1 : 0), n1 = Dw1 - (wl_id >= 0 ? 1 : 0);
+    return 2.0 * D1 * p * (double)n2 * D2 * D2 + 2.0 * D1 * (double)p * D2 * n1 * D1;
+}
+
 static inline int lo_type(int prec) { return prec == TN_FP32 ? T_F32 : T_BF16; }
 
 void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed)
 {
     hipStream_t s = c->stream;
     const int p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
+    const bool idl = op.wl_id >= 0, idr = op.wr_id >= 0, ident = idl || idr;
+    TN_REQUIRE(!ident || (op.mix->wl == op.wl_id && op.mix->wr == op.wr_id), TN_ERR_STATE,
+               "heff_apply: mixing table does not match the identity channels");
+    TN_REQUIRE(!idl || (Dw1 >= 2 && (op.wl_id == 0 || op.wl_id == Dw1 - 1)), TN_ERR_STATE,
+               "heff_apply: left identity channel must be the first or last of >= 2");
+    TN_REQUIRE(!idr || (Dw2 >= 2 && (op.wr_id == 0 || op.wr_id == Dw2 - 1)), TN_ERR_STATE,
+               "heff_apply: right identity channel must be the first or last of >= 2");
+    // channels taken by the GEMMs: right [r0, r0 + nR), left [l0, l0 + nL)
+    const int nR = Dw2 - (idr ? 1 : 0), r0 = op.wr_id == 0 ? 1 : 0;
+    const int nL = Dw1 - (idl ? 1 : 0), l0 = op.wl_id == 0 ? 1 : 0;
     const long n = op.n();
-    const long n1 = (long)D1p * p * Dw2 * D2p;
-    const long n2 = (long)Dw1 * D1p * p * D2p;
+    const long n1 = (long)D1p * p * nR * D2p;
+    const long n2 = (long)nL * D1p * p * D2p;
     if (timed) TN_HIP(hipEventRecord(c->ev[0], s));
     GemmDesc g1{}, g3{};
     g1.a_kmajor = true;
     g1.b_kmajor = true;
     g1.M = D1p * p;
-    g1.N = Dw2 * D2p;
+    g1.N = nR * D2p;
     g1.K = D2p;
     g1.batch = 1;
     g1.lda = D2p;
     g1.ldb = D2p;
-    g1.ldc = (long)Dw2 * D2p;
+    g1.ldc = (long)nR * D2p;
     g1.alpha = 1.0f;
     g1.tile = op.tile;
     g3.a_kmajor = true;
     g3.b_kmajor = false;
     g3.M = D1p;
     g3.N = p * D2p;
-    g3.K = Dw1 * D1p;
+    g3.K = nL * D1p;
     g3.batch = 1;
     g3.lda = (long)Dw1 * D1p;
     g3.ldb = (long)p * D2p;
     g3.ldc = (long)p * D2p;
     g3.alpha = 1.0f;
     g3.tile = op.tile;
+    g3.accumulate = idl ? 1 : 0;  // y already holds the identity channel's slice
+    const long roff = (long)r0 * D2p * D2p;  // element offset of the first kept R channel
+    const long loff = (long)l0 * D1p;        // column offset of the first kept L channel
     if (op.precision == TN_FP64) {
         c->T1.ensure(n1 * 8);
         c->T2.ensure(n2 * 8);
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
-        g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R; g1.C = c->T1.p;
+        g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R + roff; g1.C = c->T1.p;
         gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
-                    op.mix->t, s));
+        if (ident)
+            TN_HIP(wmix_id(T_F64, T_F64, T_F64, c->T1.p, (long)p * nR * D2p, x, (long)p * D2p, c->T2.p,
+                           (long)p * D2p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
+        else
+            TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
+                        op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
-        g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L; g3.B = c->T2.p; g3.C = y;
+        g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L + loff; g3.B = c->T2.p; g3.C = y;
         gemm_run(c, g3);
     } else {
         const int tl = lo_type(op.precision);
@@ -280,13 +331,17 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         c->T2.ensure(n2 * es);
         TN_HIP(convert(T_F64, x, tl, c->xlo.p, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
-        g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = op.Rlo; g1.C = c->T1.p;
+        g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = (const char*)op.Rlo + roff * es; g1.C = c->T1.p;
         gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
-                    op.mix->t, s));
+        if (ident)
+            TN_HIP(wmix_id(T_F32, tl, tl, c->T1.p, (long)p * nR * D2p, c->xlo.p, (long)p * D2p, c->T2.p,
+                           (long)p * D2p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
+        else
+            TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
+                        op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
-        g3.tin = tl; g3.tout = T_F64; g3.A = op.Llo; g3.B = c->T2.p; g3.C = y;
+        g3.tin = tl; g3.tout = T_F64; g3.A = (const char*)op.Llo + loff * es; g3.B = c->T2.p; g3.C = y;
         gemm_run(c, g3);
     }
     if (timed) TN_HIP(hipEventRecord(c->ev[4], s));

@@ -71,9 +71,18 @@ struct MixTable {
     const long* eoff;    // device, nnz
     const double* eval;  // device, nnz
     const float* evalf;  // device, nnz (same values, fp32)
+    const int* esel;     // device, nnz (identity-channel tables only): 1 = entry reads in2
+    const int* osel;     // device, nout (identity-channel tables only): 1 = output goes to out2
 };
 hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx, int X,
                 int Bn, const MixTable& t, hipStream_t s);
+// The same mixing with a second input and a second output: entries e with
+// esel[e] read in2[x*isx2 + eoff[e] + b] (type tx), outputs o with osel[o]
+// write out2[x*osx2 + ooff[o] + b] (fp64).  Used when an environment channel is
+// the identity: its stage-1 slice is the input vector itself (in2) and its
+// stage-3 contribution is a copy of the mixed slice into y (out2).
+hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
+                   long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
 
 // elementwise
 hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s);

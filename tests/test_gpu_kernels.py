@@ -133,6 +133,50 @@ def test_heff_apply_reduced_precision(ctx, prec, tol):
     assert err < tol, err
 
 
+def with_identity_channel(F, Dw, D, w):
+    """Environment (reference layout F[w + Dw*(i + D*j)]) with channel w set to the identity."""
+    Fr = F.reshape(D, D, Dw).copy()
+    Fr[:, :, w] = np.eye(D)
+    return Fr.ravel()
+
+
+# (d, D1, D2, Dw1, Dw2, identity channel of L, of R): first/last channels,
+# one side only, padded (D not a multiple of 32) and unpadded shapes
+IDENT_CASES = [(2, 8, 8, 5, 5, 0, 4), (3, 27, 40, 5, 5, 4, 0), (2, 37, 19, 5, 5, 0, -1), (3, 64, 64, 5, 5, -1, 4),
+               (4, 5, 9, 3, 4, 2, 3), (2, 32, 33, 2, 2, 1, 0)]
+
+
+@pytest.mark.parametrize("d,D1,D2,Dw1,Dw2,wl,wr", IDENT_CASES)
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2)])
+def test_heff_apply_identity_channels_match_oracle(ctx, d, D1, D2, Dw1, Dw2, wl, wr, prec, tol):
+    # an environment channel equal to the identity is applied as a copy of
+    # the input (right side) or of the mixed slice into y (left side); the
+    # oracle contracts it like any other channel
+    W = sym_W(Dw1, Dw2, d, 60)
+    L, R = rand(Dw1 * D1 * D1, 61), rand(Dw2 * D2 * D2, 62)
+    if wl >= 0:
+        L = with_identity_channel(L, Dw1, D1, wl)
+    if wr >= 0:
+        R = with_identity_channel(R, Dw2, D2, wr)
+    x = rand(d * D1 * D2, 63)
+    y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    ctx.site_bind(W, L, R, d, D1, D2, prec)
+    y = ctx.heff_apply(x)
+    if prec == tn.TN_FP64:
+        err = np.abs(y - y_ref).max() / np.abs(y_ref).max()
+    else:
+        err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    assert err < tol, err
+    # a channel off the identity by more than the tolerance is contracted
+    if wl >= 0:
+        L2 = L.copy()
+        L2[wl] += 1e-6  # entry (0, 0) of channel wl
+        y2_ref = O.heff_apply_ref(W, L2, R, x, d, D1, D2)
+        ctx.site_bind(W, L2, R, d, D1, D2, tn.TN_FP64)
+        y2 = ctx.heff_apply(x)
+        assert np.abs(y2 - y2_ref).max() / np.abs(y2_ref).max() < 1e-13
+
+
 @pytest.mark.parametrize("d,D1,D2,Dw1,Dw2", HEFF_CASES)
 def test_env_updates_match_oracle(ctx, d, D1, D2, Dw1, Dw2):
     W = sym_W(Dw1, Dw2, d, 30)

@@ -114,3 +114,32 @@ def test_qr_normalisation_right_canonical_large(ctx):
     e1 = ch.expectation() / ch.expectation(identity_mpo(L, 3))
     e2 = ch2.expectation() / ch2.expectation(identity_mpo(L, 3))
     assert abs(e1 - e2) < 1e-10 * max(1.0, abs(e2))
+
+
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-12), (tn.TN_FP32, 1e-6)])
+def test_identity_channels_same_sweeps(ctx, prec, tol):
+    # orthonormal blocks make the first/last MPO channel of each environment
+    # the identity: applying it as a copy gives the sweeps of the full
+    # contraction (FP64: to rounding; FP32: to the FP32 operator's accuracy)
+    L, D = 16, 48
+    energies, stats = [], []
+    for skip in (0, 1):
+        ch = tn.Chain(ctx, xxz_mpo(L, 1.0, Dz=0.1), D)
+        ch.initialize(1, 99)
+        ch.normalize_qr(1)
+        o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1, precision=prec, env_precision=prec, refine=0,
+                                 skip_identity=skip)
+        ch.load_caches(o)
+        st = tn.SweepStats()
+        for _ in range(3):
+            ch.sweep(0, o, st)
+            ch.sweep(1, o, st)
+        energies.append(ch.expectation())
+        stats.append(st)
+        ch.close()
+    assert abs(energies[1] - energies[0]) < tol * abs(energies[0]), energies
+    assert stats[0].n_ident_left == 0 and stats[0].n_ident_right == 0
+    # every bulk solve sees an identity channel on both sides
+    solves = 6 * L
+    assert stats[1].n_ident_left >= solves - 12 and stats[1].n_ident_right >= solves - 12, \
+        (stats[1].n_ident_left, stats[1].n_ident_right)
