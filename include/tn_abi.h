@@ -145,6 +145,9 @@ typedef struct tn_sweep_opts {
                             orthonormal block) is applied as a copy instead of a GEMM
                             slice; checked per local solve against the identity
                             (1e-12 for FP64 environments, 1e-4 for FP32 ones) */
+    int time_heff;       /* 1 = HIP events around the stages of every Heff apply
+                            (tn_sweep_stats.heff_ms / heff_gemm_ms); 0 = counters only,
+                            no events in the stream */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {

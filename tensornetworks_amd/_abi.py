@@ -38,6 +38,7 @@ class SweepOpts(ctypes.Structure):
         ("precond", ctypes.c_int),
         ("precond_delta", ctypes.c_double),
         ("skip_identity", ctypes.c_int),
+        ("time_heff", ctypes.c_int),
     ]
 
     @classmethod

@@ -42,9 +42,13 @@ namespace tn {
 struct HeffCall {
     HeffOp op;
     tn_sweep_stats* st;
-    int D1, D2;  // logical dims for the algorithmic flop count
+    int D1, D2;         // logical dims for the algorithmic flop count
+    bool timed = true;  // HIP events around the stages (device times into st)
 };
+// y = Heff x with the counters of hc->st; the event timings of a timed apply
+// are read at the next apply or by heff_timing_flush, never by waiting here
 void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y);
+void heff_timing_flush(tn_ctx* c);
 const double* w_diag(tn_chain* ch, int i);
 double now_ms();
 void ensure_lo(tn_chain* ch, int prec);

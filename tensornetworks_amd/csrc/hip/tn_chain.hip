@@ -23,7 +23,10 @@ namespace tn {
 
 int fail(tn_ctx* c, const TnError& e)
 {
-    if (c) c->err = e.msg;
+    if (c) {
+        c->err = e.msg;
+        c->heff_pending = nullptr;  // the caller's stats may not outlive the call
+    }
     return e.code;
 }
 
@@ -155,13 +158,21 @@ void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_s
 }
 
 // --- Heff apply callback for the Lanczos driver ---------------------------
+void heff_timing_flush(tn_ctx* c)
+{
+    if (!c->heff_pending) return;
+    heff_timing_read(c, &c->heff_pending->heff_gemm_ms, &c->heff_pending->heff_ms);
+    c->heff_pending = nullptr;
+}
+
 void heff_cb(tn_ctx* c, const void* vop, const double* x, double* y)
 {
     const HeffCall* hc = (const HeffCall*)vop;
-    const bool timed = hc->st != nullptr;
+    const bool timed = hc->st != nullptr && hc->timed;
+    heff_timing_flush(c);  // the events are reused below
     heff_apply(c, hc->op, x, y, timed);
-    if (timed) {
-        heff_timing_read(c, &hc->st->heff_gemm_ms, &hc->st->heff_ms);
+    if (timed) c->heff_pending = hc->st;
+    if (hc->st) {
         hc->st->heff_flops += hc->op.flops_alg(hc->D1, hc->D2);
         hc->st->heff_gemm_flops += hc->op.flops_gemm(hc->D1, hc->D2);
         hc->st->n_matvec++;
@@ -194,7 +205,7 @@ void env_identity(tn_chain* ch, int i, double tol, int* wl, int* wr)
         TN_HIP(eye_deviation(R + (long)wrk * D2p * D2p, D2p, D2, dv + 2 + k, part, s));
     }
     double h[4];
-    d2h(c, h, dv, sizeof(h));
+    d2h_small(c, h, dv, 4);
     auto ok = [&](double v) { return std::isfinite(v) && v <= tol; };
     if (Dw1 >= 2) *wl = ok(h[0]) ? 0 : (ok(h[1]) ? Dw1 - 1 : -1);
     if (Dw2 >= 2) *wr = ok(h[2]) ? 0 : (ok(h[3]) ? Dw2 - 1 : -1);
@@ -250,6 +261,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     HeffCall hc;
     hc.op = make_heff(ch, i, o.precision, o.gemm_tile, wl, wr);
     hc.st = st;
+    hc.timed = o.time_heff != 0;
     hc.D1 = ch->D1[i];
     hc.D2 = ch->D2[i];
     const long N = hc.op.n();
@@ -264,6 +276,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
                                      Wd, ch->A[i],
                                      o.eig_tol, o.max_krylov, o.max_krylov * (1 + o.max_restarts), o.precond,
                                      o.precond_delta);
+        heff_timing_flush(c);
         if (st) st->lanczos_ms += now_ms() - t0;
         *e0 = dr.e0;
         *e1 = dr.e1;
@@ -280,6 +293,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
         r.e0 = r2.e0;
         r.e1 = r2.e1;
     }
+    heff_timing_flush(c);
     if (st) st->lanczos_ms += now_ms() - t0;
     *e0 = r.e0;
     *e1 = r.e1;
@@ -603,7 +617,11 @@ int tn_ctx_create(int device, tn_ctx** out)
     TN_HIP(hipSetDevice(device));
     TN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) TN_HIP(hipEventCreate(&e));
-    TN_HIP(hipHostMalloc((void**)&c->sc.h, 64 * sizeof(double)));
+    TN_HIP(hipHostMalloc((void**)&c->sc.h, 1024, hipHostMallocCoherent | hipHostMallocMapped));
+    TN_HIP(hipHostGetDevicePointer((void**)&c->sc.d, c->sc.h, 0));
+    c->sc.flag_h = (unsigned long long*)((char*)c->sc.h + 512);
+    c->sc.flag_d = (unsigned long long*)((char*)c->sc.d + 512);
+    *c->sc.flag_h = 0;
     *out = c;
     return TN_OK;
     }
@@ -718,6 +736,7 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->precond = 1;
     o->precond_delta = 1e-2;
     o->skip_identity = 1;
+    o->time_heff = 1;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
@@ -1356,6 +1375,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.op.mix = &ch->mix2H[i];
         hc.op.tile = o.gemm_tile;
         hc.st = st;
+        hc.timed = o.time_heff != 0;
         hc.D1 = D1;
         hc.D2 = D3;
         double tl = now_ms();
@@ -1369,6 +1389,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
             r.e0 = r2.e0;
             r.e1 = r2.e1;
         }
+        heff_timing_flush(c);
         if (st) {
             st->lanczos_ms += now_ms() - tl;
             st->energy = r.e0;
@@ -1438,6 +1459,7 @@ int tn_chain_heff_bench(tn_chain* ch, int site, int precision, int reps, int ski
     if (!ch || site < 0 || site >= ch->L || reps < 1) return TN_ERR_ARG;
     tn_ctx* c = ch->ctx;
     ABI_BEGIN
+    heff_timing_flush(c);  // the events are reused below
     int wl = -1, wr = -1;
     if (skip_identity) env_identity(ch, site, ident_tol(env_precision), &wl, &wr);
     HeffOp op = make_heff(ch, site, precision, 0, wl, wr);

@@ -41,10 +41,14 @@ struct DBuf {
     template <typename T> T* as() const { return (T*)p; }
 };
 
-// a few device scalars + pinned host mirror for the per-iteration readback
+// host-mapped coherent memory for the per-iteration readback of a few
+// scalars: the device publishes values and a sequence number, the host polls
 struct Scalars {
-    double* d = nullptr;  // device, 64 doubles
-    double* h = nullptr;  // pinned host, 64 doubles
+    double* h = nullptr;              // host view: 64 doubles, then the flag
+    double* d = nullptr;              // device view of the same memory
+    unsigned long long* flag_h = nullptr;
+    unsigned long long* flag_d = nullptr;
+    unsigned long long seq = 0;
 };
 
 // Sparse MPO mixing table on the device (see wmix in tn_kernels.h)
@@ -88,6 +92,8 @@ struct tn_ctx {
     tn::DBuf dav_W, dav_vec, dav_wd, mlp;
     // identity-channel check of the environments (4 device doubles)
     tn::DBuf ident;
+    // stats whose Heff event timings are recorded but not yet read
+    tn_sweep_stats* heff_pending = nullptr;
     // seam state (tn_site_bind)
     int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
     int b_wl = -1, b_wr = -1;  // identity channels of the bound environments
@@ -202,5 +208,8 @@ void sync(tn_ctx* c);
 // stream-ordered copies that return when the data has arrived
 void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes);
 void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes);
+// n (<= 64) doubles device -> host through the published scalars (no stream
+// synchronisation: the host polls the completion flag)
+void d2h_small(tn_ctx* c, double* dst, const double* src, int n);
 
 }  // namespace tn

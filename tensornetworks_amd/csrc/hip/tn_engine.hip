@@ -3,6 +3,8 @@
 #include "tn_engine.h"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -41,6 +43,28 @@ void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes)
     TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     sync(c);
 }
+void d2h_small(tn_ctx* c, double* dst, const double* src, int n)
+{
+    TN_REQUIRE(n >= 0 && n <= 64 && c->sc.h, TN_ERR_ARG, "d2h_small: bad size");
+    const unsigned long long seq = ++c->sc.seq;
+    TN_HIP(publish(src, n, c->sc.d, c->sc.flag_d, seq, c->stream));
+    volatile unsigned long long* f = c->sc.flag_h;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned long spins = 1; *f != seq; spins++) {
+        if ((spins & 0xffff) == 0) {
+            // a long wait: hand over to the stream synchronisation, which also
+            // reports a failed launch
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+                sync(c);
+                TN_REQUIRE(*f == seq, TN_ERR_STATE, "d2h_small: completion flag not set");
+                break;
+            }
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    for (int i = 0; i < n; i++) dst[i] = ((volatile double*)c->sc.h)[i];
+}
+
 void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes)
 {
     TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -587,11 +611,17 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
     c->part.ensure(pcap * 8);
     double* part = c->part.as<double>();
     std::vector<double> al(kmax), be(kmax), z(kmax), hh(kmax + 4);
+    // fused step kernels: 16-byte vector access needs an even length
+    const bool fused = N % 2 == 0 && kmax <= 512;
 
+    auto read = [&](double* dst, const double* src, int n) {
+        if (n <= 64) d2h_small(c, dst, src, n);
+        else d2h(c, dst, src, (size_t)n * 8);
+    };
     auto norm_host = [&](const double* v) {
         TN_HIP(mdot(v, N, 1, v, N, nrm, part, (int)pcap, s));
         double v2 = 0;
-        d2h(c, &v2, nrm, 8);
+        read(&v2, nrm, 1);
         return std::sqrt(v2);
     };
 
@@ -624,11 +654,19 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
             // then one classical Gram-Schmidt pass against every Krylov vector
             // (h[1..j+1]) removes what rounding left; h[j+2] = |w|^2
             TN_HIP(mdot(vj, N, 1, w, N, h, part, (int)pcap, s));
-            TN_HIP(three_term(w, vj, j > 0 ? V + (long)(j - 1) * N : nullptr, h, j > 0 ? be[j - 1] : 0.0, N, s));
-            TN_HIP(mdot(V, N, j + 1, w, N, h + 1, part, (int)pcap, s));
-            TN_HIP(gemv_acc(V, N, j + 1, h + 1, -1.0, 1.0, w, N, s));
-            TN_HIP(mdot(w, N, 1, w, N, h + j + 2, part, (int)pcap, s));
-            d2h(c, hh.data(), h, (size_t)(j + 3) * 8);
+            if (fused) {
+                // three-term step fused with the Gram-Schmidt dot products, then
+                // the Gram-Schmidt update fused with the norm
+                TN_HIP(lz_step_dots(w, V, N, j + 1, h, j > 0 ? be[j - 1] : 0.0, N, h + 1, part, pcap, s));
+                TN_HIP(lz_update_norm(w, V, N, j + 1, h + 1, N, h + j + 2, part, pcap, s));
+            } else {
+                TN_HIP(three_term(w, vj, j > 0 ? V + (long)(j - 1) * N : nullptr, h, j > 0 ? be[j - 1] : 0.0, N,
+                                  s));
+                TN_HIP(mdot(V, N, j + 1, w, N, h + 1, part, (int)pcap, s));
+                TN_HIP(gemv_acc(V, N, j + 1, h + 1, -1.0, 1.0, w, N, s));
+                TN_HIP(mdot(w, N, 1, w, N, h + j + 2, part, (int)pcap, s));
+            }
+            read(hh.data(), h, j + 3);
             const double alpha = hh[0] + hh[j + 1];
             const double nw2 = hh[j + 2];
             al[j] = alpha;
@@ -912,7 +950,7 @@ static bool polar_finish(tn_ctx* c, const double* Q1, int m, int n, int nl, doub
     double* h = c->svd_s.as<double>();
     TN_HIP(mdot(E, nn, 1, E, nn, h, c->part.as<double>(), (int)pcap, s));
     double e2 = 0;
-    d2h(c, &e2, h, 8);
+    d2h_small(c, &e2, h, 1);
     if (!(std::sqrt(e2) <= 1e-3)) return false;
     GemmDesc g{};
     g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
@@ -968,7 +1006,7 @@ bool cholqr_device(tn_ctx* c, const double* A, long lda, int m, int n, int nl, d
         if (pass >= 3) {
             TN_HIP(eye_deviation(G, n, nl, dev, c->part.as<double>(), s));
             double h = 0;
-            d2h(c, &h, dev, 8);
+            d2h_small(c, &h, dev, 1);
             if (std::isfinite(h) && h <= tol) {
                 if (X != Q) TN_HIP(hipMemcpyAsync(Q, X, (size_t)m * n * 8, hipMemcpyDeviceToDevice, s));
                 return true;

@@ -84,6 +84,12 @@ hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx
 hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
                    long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
 
+// Small device -> host transfer without a stream synchronisation: copies n
+// (<= 64) doubles to host-mapped coherent memory, then (system-scope fence)
+// stores seq into *flag; the host polls the flag.
+hipError_t publish(const double* src, int n, double* host_dst, unsigned long long* host_flag,
+                   unsigned long long seq, hipStream_t s);
+
 // elementwise
 hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s);
 hipError_t fill_zero(void* p, long bytes, hipStream_t s);
@@ -112,6 +118,15 @@ hipError_t absmax(const double* x, long n, double* out, double* part, int part_c
 hipError_t scale_by_inv(double* v, const double* den, long n, hipStream_t s);
 hipError_t axpby(double a, const double* x, double b, double* y, long n, hipStream_t s);
 // w -= alpha[0] v + beta u (u may be null); alpha is a device scalar
+// Fused Lanczos step (n even, vectors 16-byte aligned, k <= 512):
+//  step_dots : w <- w - alpha[0] V_{k-1} - beta V_{k-2} (the latter when k >= 2),
+//              then out[i] = V_i . w for i < k (two-stage, fixed order)
+//  update_norm: w <- w - sum_{i<k} coef[i] V_i, then out[0] = |w|^2 (two-stage)
+// part: mdot_part_size(n, k + 1) doubles.
+hipError_t lz_step_dots(double* w, const double* V, long ldv, int k, const double* alpha, double beta, long n,
+                        double* out, double* part, long part_cap, hipStream_t s);
+hipError_t lz_update_norm(double* w, const double* V, long ldv, int k, const double* coef, long n, double* out,
+                          double* part, long part_cap, hipStream_t s);
 hipError_t three_term(double* w, const double* v, const double* u, const double* alpha, double beta, long n,
                       hipStream_t s);
 // y = a x (y is not read; x and y may be the same buffer)

@@ -620,6 +620,26 @@ hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const vo
     return hipErrorInvalidValue;
 }
 
+__global__ void __launch_bounds__(64) publish_k(const double* __restrict__ src, int n, double* host_dst,
+                                                unsigned long long* host_flag, unsigned long long seq)
+{
+    const int t = threadIdx.x;
+    if (t < n) host_dst[t] = src[t];
+    __syncthreads();
+    if (t == 0) {
+        __threadfence_system();
+        *(volatile unsigned long long*)host_flag = seq;
+    }
+}
+
+hipError_t publish(const double* src, int n, double* host_dst, unsigned long long* host_flag,
+                   unsigned long long seq, hipStream_t s)
+{
+    if (n < 0 || n > 64) return hipErrorInvalidValue;
+    publish_k<<<1, 64, 0, s>>>(src, n, host_dst, host_flag, seq);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // elementwise
 template <typename TI, typename TO>
@@ -786,6 +806,134 @@ hipError_t mdot(const double* V, long ldv, int k, const double* w, long n, doubl
         mdot1_k<<<G, MD_BLOCK, 0, s>>>(V + (long)i0 * ldv, ldv, kc, w, n, part);
         mdot2_k<<<kc, 256, 0, s>>>(part, (int)G, out + i0);
     }
+    return hipGetLastError();
+}
+
+// Fused Lanczos kernels.  A block owns a chunk of MD_BLOCK * MD_EPT entries,
+// each thread MD_EPT / 2 pairs (16-byte loads, consecutive lanes on
+// consecutive pairs).  update_norm visits the chunks in the reverse order of
+// step_dots, so its first chunks of the basis are the ones step_dots read
+// last (still in the last-level cache when the basis exceeds it).
+#define LZ_PAIRS (MD_EPT / 2)
+__global__ void __launch_bounds__(MD_BLOCK) lz_step_dots_k(double* __restrict__ w, const double* __restrict__ V,
+                                                           long ldv, int k, const double* __restrict__ alpha,
+                                                           double beta, long n, double* __restrict__ part)
+{
+    __shared__ double red[MD_MAXK][MD_BLOCK / 64];
+    const long c0 = (long)blockIdx.x * MD_BLOCK * MD_EPT;
+    const double a = alpha[0];
+    const double* vj = V + (long)(k - 1) * ldv;
+    const double* vm = k >= 2 ? V + (long)(k - 2) * ldv : nullptr;
+    double2 wr[LZ_PAIRS];
+#pragma unroll
+    for (int e = 0; e < LZ_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * MD_BLOCK + threadIdx.x);
+        if (idx < n) {
+            double2 x = *(const double2*)(w + idx);
+            const double2 p = *(const double2*)(vj + idx);
+            x.x -= a * p.x;
+            x.y -= a * p.y;
+            if (vm) {
+                const double2 q = *(const double2*)(vm + idx);
+                x.x -= beta * q.x;
+                x.y -= beta * q.y;
+            }
+            *(double2*)(w + idx) = x;
+            wr[e] = x;
+        } else {
+            wr[e] = make_double2(0.0, 0.0);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i < k; i++) {
+        const double* vi = V + (long)i * ldv;
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < LZ_PAIRS; e++) {
+            const long idx = c0 + 2 * ((long)e * MD_BLOCK + threadIdx.x);
+            if (idx < n) {
+                const double2 p = *(const double2*)(vi + idx);
+                acc += p.x * wr[e].x + p.y * wr[e].y;
+            }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) red[i][wid] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += MD_BLOCK) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < MD_BLOCK / 64; q++) sum += red[i][q];
+        part[(long)i * gridDim.x + blockIdx.x] = sum;
+    }
+}
+
+__global__ void __launch_bounds__(MD_BLOCK) lz_update_norm_k(double* __restrict__ w, const double* __restrict__ V,
+                                                             long ldv, int k, const double* __restrict__ coef,
+                                                             long n, double* __restrict__ part)
+{
+    __shared__ double c[MD_MAXK];
+    __shared__ double red[MD_BLOCK / 64];
+    for (int i = threadIdx.x; i < k; i += MD_BLOCK) c[i] = coef[i];
+    __syncthreads();
+    const int blk = gridDim.x - 1 - blockIdx.x;
+    const long c0 = (long)blk * MD_BLOCK * MD_EPT;
+    double2 acc[LZ_PAIRS];
+#pragma unroll
+    for (int e = 0; e < LZ_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * MD_BLOCK + threadIdx.x);
+        acc[e] = idx < n ? *(const double2*)(w + idx) : make_double2(0.0, 0.0);
+    }
+    for (int i = 0; i < k; i++) {
+        const double* vi = V + (long)i * ldv;
+        const double ci = c[i];
+#pragma unroll
+        for (int e = 0; e < LZ_PAIRS; e++) {
+            const long idx = c0 + 2 * ((long)e * MD_BLOCK + threadIdx.x);
+            if (idx < n) {
+                const double2 p = *(const double2*)(vi + idx);
+                acc[e].x -= ci * p.x;
+                acc[e].y -= ci * p.y;
+            }
+        }
+    }
+    double sq = 0.0;
+#pragma unroll
+    for (int e = 0; e < LZ_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * MD_BLOCK + threadIdx.x);
+        if (idx < n) {
+            *(double2*)(w + idx) = acc[e];
+            sq += acc[e].x * acc[e].x + acc[e].y * acc[e].y;
+        }
+    }
+    sq = wave_sum(sq);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < MD_BLOCK / 64; q++) sum += red[q];
+        part[blk] = sum;
+    }
+}
+
+hipError_t lz_step_dots(double* w, const double* V, long ldv, int k, const double* alpha, double beta, long n,
+                        double* out, double* part, long part_cap, hipStream_t s)
+{
+    const long G = (n + MD_BLOCK * MD_EPT - 1) / (MD_BLOCK * MD_EPT);
+    if (k < 1 || k > MD_MAXK || (n & 1) || G * k > part_cap) return hipErrorInvalidValue;
+    lz_step_dots_k<<<G, MD_BLOCK, 0, s>>>(w, V, ldv, k, alpha, beta, n, part);
+    mdot2_k<<<k, 256, 0, s>>>(part, (int)G, out);
+    return hipGetLastError();
+}
+
+hipError_t lz_update_norm(double* w, const double* V, long ldv, int k, const double* coef, long n, double* out,
+                          double* part, long part_cap, hipStream_t s)
+{
+    const long G = (n + MD_BLOCK * MD_EPT - 1) / (MD_BLOCK * MD_EPT);
+    if (k < 1 || k > MD_MAXK || (n & 1) || G > part_cap) return hipErrorInvalidValue;
+    lz_update_norm_k<<<G, MD_BLOCK, 0, s>>>(w, V, ldv, k, coef, n, part);
+    mdot2_k<<<1, 256, 0, s>>>(part, (int)G, out);
     return hipGetLastError();
 }
 
@@ -1200,89 +1348,83 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 
 // ---------------------------------------------------------------------------
 // CholeskyQR pieces.  chol_diag_k factors one CB x CB diagonal block of a
-// symmetric positive definite G in LDS (one wavefront) and writes L_kk (lower)
-// and its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
+// symmetric positive definite G (one wavefront) and writes L_kk (lower) and
+// its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
 #define CB 64
-// One wavefront, everything in LDS.  Left-looking (Crout) Cholesky: step j
-// has lane r >= j form A[r][j] - sum_{k<j} L[r][k] L[j][k] (L[j][k] is a
-// broadcast read, L[r][k] conflict-free across lanes), then lane j takes the
-// square root and the others divide.  The inverse follows row by row with the
-// same access pattern.  Dot products use eight partial sums so the LDS reads
-// pipeline.  (A register-resident variant needs the steps unrolled, and the
-// unrolled triangle does not fit the register file.)
+// v from lane `lane` (wave-uniform index) of the wavefront, as a register
+// broadcast (two 32-bit readlanes)
+__device__ __forceinline__ double wave_bcast(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One wavefront, the block in registers.  Lane r keeps the not yet factored
+// part of row r, a[kk] = A[r][j + kk], shifted down by one column per step so
+// that every register index is a compile-time constant while the step loop
+// stays rolled.  Right-looking step j: the pivot A[j][j] comes from lane j,
+// lane r > j forms L[r][j] = A[r][j] / L[j][j], and the trailing update
+// A[r][k] -= L[r][j] L[k][j] takes L[k][j] from lane k by readlane (register
+// broadcast: no LDS round trip, no barrier).  The same broadcast drives the
+// forward substitution for L^-1, fused into the step: lane c keeps the
+// pending right-hand side of column c, sv[kk] = row j + kk, emits X[j][c]
+// and subtracts L[j+kk][j] X[j][c].  Lane indices wrap to lanes < j, whose
+// column entry is zero, so the full-width updates need no guard.
 __device__ __forceinline__ void chol_block(const double* __restrict__ G, long ldg, int k0, int nbk,
                                            double* __restrict__ L, long ldl, double* __restrict__ Linv, long ldi,
-                                           int* __restrict__ flag, double (*Ls)[CB + 1], double (*Xs)[CB + 1],
-                                           int& bad)
+                                           int* __restrict__ flag, double (*Ls)[CB + 1])
 {
     const int r = threadIdx.x;
-    if (r == 0) bad = 0;
-    // coalesced load, 16 independent loads in flight per lane before the LDS
-    // stores (a loop of load-then-store would pay one memory latency per
-    // row); a block narrower than CB (the last one) is completed with the
-    // identity
+    // coalesced load (lane = column), 16 loads in flight per lane; a block
+    // narrower than CB (the last one) is completed with the identity
 #pragma unroll
-    for (int e0 = 0; e0 < CB * CB; e0 += 64 * 16) {
+    for (int i0 = 0; i0 < CB; i0 += 16) {
         double tmp[16];
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int e = e0 + u * 64 + r, i = e / CB, c = e % CB;
-            tmp[u] = (i < nbk && c < nbk) ? G[(long)(k0 + i) * ldg + k0 + c] : (i == c ? 1.0 : 0.0);
+            const int i = i0 + u;
+            tmp[u] = (i < nbk && r < nbk) ? G[(long)(k0 + i) * ldg + k0 + r] : (i == r ? 1.0 : 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int e = e0 + u * 64 + r;
-            Ls[e / CB][e % CB] = tmp[u];
+        for (int u = 0; u < 16; u++) Ls[i0 + u][r] = tmp[u];
+    }
+    __syncthreads();
+    double a[CB], sv[CB];
+#pragma unroll
+    for (int k = 0; k < CB; k++) {
+        a[k] = Ls[r][k];
+        sv[k] = k == r ? 1.0 : 0.0;
+    }
+    __syncthreads();  // every row is in registers before Ls takes L
+    int bad = 0;
+    for (int j = 0; j < CB; j++) {
+        const double piv = wave_bcast(a[0], j);
+        const bool ok = piv > 0.0 && isfinite(piv);
+        bad |= ok ? 0 : 1;
+        const double d = ok ? sqrt(piv) : 1.0;
+        const double inv = 1.0 / d;
+        const double cj = r > j ? a[0] * inv : (r == j ? d : 0.0);
+        const double xj = sv[0] * inv;
+        Ls[r][j] = cj;
+        if (j < nbk && r < nbk) Linv[(long)(k0 + j) * ldi + k0 + r] = xj;
+#pragma unroll
+        for (int k0c = 1; k0c < CB; k0c += 9) {
+#pragma unroll
+            for (int u = 0; u < 9; u++) {
+                const int kk = k0c + u;
+                const double lk = wave_bcast(cj, (j + kk) & (CB - 1));  // L[j+kk][j]
+                a[kk - 1] = fma(-cj, lk, a[kk]);
+                sv[kk - 1] = fma(-lk, xj, sv[kk]);
+            }
+            __builtin_amdgcn_sched_barrier(0);  // bounds the broadcasts in flight (SGPRs)
         }
     }
     __syncthreads();
-    for (int j = 0; j < CB; j++) {
-        double sp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        if (r >= j) {
-            int k = 0;
-            for (; k + 7 < j; k += 8) {
-#pragma unroll
-                for (int u = 0; u < 8; u++) sp[u] += Ls[r][k + u] * Ls[j][k + u];
-            }
-            for (; k < j; k++) sp[0] += Ls[r][k] * Ls[j][k];
-        }
-        const double v = Ls[r][j] - (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])));
-        __syncthreads();
-        if (r == j) {
-            double d = v;
-            if (!(d > 0.0) || !isfinite(d)) {
-                bad = 1;
-                d = 1.0;
-            } else {
-                d = sqrt(d);
-            }
-            Ls[j][j] = d;
-        }
-        __syncthreads();
-        if (r > j) Ls[r][j] = v / Ls[j][j];
-        __syncthreads();
-    }
-    // X = L^-1 row by row: lane c forms X[i][c]; X[k][c] = 0 for k < c, so
-    // every lane runs the same k range
-    for (int i = 0; i < CB; i++) {
-        double sp[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int k = 0;
-        for (; k + 7 < i; k += 8) {
-#pragma unroll
-            for (int u = 0; u < 8; u++) sp[u] += Ls[i][k + u] * Xs[k + u][r];
-        }
-        for (; k < i; k++) sp[0] += Ls[i][k] * Xs[k][r];
-        const double v = (i == r ? 1.0 : 0.0) - (((sp[0] + sp[1]) + (sp[2] + sp[3])) + ((sp[4] + sp[5]) + (sp[6] + sp[7])));
-        Xs[i][r] = (r <= i) ? v / Ls[i][i] : 0.0;
-        __syncthreads();
-    }
-    for (int e = r; e < CB * CB; e += 64) {
-        const int i = e / CB, c = e % CB;
-        if (i < nbk && c < nbk) {
-            L[(long)(k0 + i) * ldl + k0 + c] = (c <= i) ? Ls[i][c] : 0.0;
-            Linv[(long)(k0 + i) * ldi + k0 + c] = Xs[i][c];
-        }
-    }
+    // L (lower; Ls holds zeros above the diagonal), coalesced rows
+    for (int i = 0; i < nbk; i++)
+        if (r < nbk) L[(long)(k0 + i) * ldl + k0 + r] = Ls[i][r];
     if (r == 0 && bad) flag[0] = 1;
 }
 
@@ -1292,19 +1434,15 @@ __global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, 
                                                   int* __restrict__ flag)
 {
     __shared__ double Ls[CB][CB + 1];
-    __shared__ double Xs[CB][CB + 1];
-    __shared__ int bad;
-    chol_block(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag, Ls, Xs, bad);
+    chol_block(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag, Ls);
 }
 
 // one diagonal block per workgroup, one job per chain
 __global__ void __launch_bounds__(64) chol_diag_batch_k(const CholJob* __restrict__ jobs)
 {
     __shared__ double Ls[CB][CB + 1];
-    __shared__ double Xs[CB][CB + 1];
-    __shared__ int bad;
     const CholJob& j = jobs[blockIdx.x];
-    chol_block(j.G, j.ldg, j.k0, j.nbk, j.L, j.ldl, j.Linv, j.ldi, j.flag, Ls, Xs, bad);
+    chol_block(j.G, j.ldg, j.k0, j.nbk, j.L, j.ldl, j.Linv, j.ldi, j.flag, Ls);
 }
 
 hipError_t chol_diag_batch(const CholJob* jobs, int n, hipStream_t s)
