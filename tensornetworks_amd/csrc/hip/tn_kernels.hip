@@ -1409,14 +1409,18 @@ __device__ __forceinline__ void chol_block(const double* __restrict__ G, long ld
         const double xj = sv[0] * inv;
         Ls[r][j] = cj;
         if (j < nbk && r < nbk) Linv[(long)(k0 + j) * ldi + k0 + r] = xj;
+        // chunks wholly past the last row (kk >= CB - j) are skipped: their
+        // entries are never read again, so they may keep stale values
 #pragma unroll
         for (int k0c = 1; k0c < CB; k0c += 9) {
+            if (k0c < CB - j) {
 #pragma unroll
-            for (int u = 0; u < 9; u++) {
-                const int kk = k0c + u;
-                const double lk = wave_bcast(cj, (j + kk) & (CB - 1));  // L[j+kk][j]
-                a[kk - 1] = fma(-cj, lk, a[kk]);
-                sv[kk - 1] = fma(-lk, xj, sv[kk]);
+                for (int u = 0; u < 9; u++) {
+                    const int kk = k0c + u;
+                    const double lk = wave_bcast(cj, (j + kk) & (CB - 1));  // L[j+kk][j]
+                    a[kk - 1] = fma(-cj, lk, a[kk]);
+                    sv[kk - 1] = fma(-lk, xj, sv[kk]);
+                }
             }
             __builtin_amdgcn_sched_barrier(0);  // bounds the broadcasts in flight (SGPRs)
         }
