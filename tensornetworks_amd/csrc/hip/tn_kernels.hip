@@ -255,9 +255,16 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
     const int nblk = ntn * ntm;
     int bid = blockIdx.x;
     // consecutive block ids land on different XCDs; give each XCD a contiguous
-    // range of output tiles so tiles sharing A rows share an L2
+    // range of output tiles so tiles sharing operands share an L2
     if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
-    const int tm = bid / ntn, tn = bid - tm * ntn;
+    // inside the range, GM tile rows at a time and column by column within
+    // them: the tiles resident together on an XCD share GM row blocks of A and
+    // a few column blocks of B, instead of one row block of A and all of B
+    constexpr int GM = 8;
+    const int gsz = GM * ntn, g = bid / gsz, first = g * GM;
+    const int rows = ntm - first < GM ? ntm - first : GM;
+    const int local = bid - g * gsz;
+    const int tm = first + local % rows, tn = local / rows;
     const long bz = blockIdx.y;
     gemm_tile<TI, TO, BM, BN, BK, AKM, BKM>((const TI*)p.A + bz * p.sA, (const TI*)p.B + bz * p.sB,
                                             (TO*)p.C + bz * p.sC, p.lda, p.ldb, p.ldc, p.K, p.alpha,
