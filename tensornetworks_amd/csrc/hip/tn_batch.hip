@@ -6,9 +6,12 @@
 //   * the Heff stages and the environment updates are persistent grouped GEMMs
 //     (one launch per tile shape; the tile of each problem comes from the
 //     per-shape tuning table, i.e. per (D1, D2, d, Dw) bucket);
-//   * the Lanczos vector steps (dots, three-term step, Gram-Schmidt update,
-//     scaling) are batched kernels with one job per chain, and one host
-//     synchronisation per Lanczos iteration serves every chain;
+//   * the Lanczos vector steps (three-term step fused with the Gram-Schmidt
+//     dots, Gram-Schmidt update fused with the norm, scaling) are batched
+//     kernels with one job per chain, and one polled read back per Lanczos
+//     iteration serves every chain;
+//   * an environment channel equal to the identity is applied as a copy, as
+//     in the single-chain operator (one batched check per round);
 //   * the CholeskyQR gauge move runs its 64x64 diagonal blocks for all chains
 //     in one launch (one workgroup per chain), so their latency is paid once
 //     per block step instead of once per chain.
@@ -64,10 +67,12 @@ const T* push(tn_batch* b, const std::vector<T>& v)
     return (const T*)d;
 }
 
-// read n doubles back; synchronises, which also frees the ring
+// read n doubles back; waits for the stream to reach the read (polled
+// completion flag up to TN_PUBLISH_MAX values), which also frees the ring
 void readback(tn_batch* b, double* dst, const double* src, size_t n)
 {
-    d2h(b->ctx, dst, src, n * 8);
+    if (n <= (size_t)TN_PUBLISH_MAX) d2h_small(b->ctx, dst, src, (int)n);
+    else d2h(b->ctx, dst, src, n * 8);
     b->ring_off = 0;
 }
 
@@ -112,7 +117,11 @@ struct Slot {
     long N;     // padded vector length
 };
 
-// y_s = Heff_s x_s for every slot (FP32 contractions, FP64 in/out)
+// y_s = Heff_s x_s for every slot (FP32 contractions, FP64 in/out).  An
+// identity environment channel of a slot (op.wl_id / op.wr_id) is applied as
+// a copy, as in heff_apply: stage 1 skips the right one, the mixing stage
+// reads x for it and writes the left one's slice into y, stage 3 skips the
+// left one and accumulates.
 void heff_batch(tn_batch* b, const std::vector<Slot>& S, const std::vector<const double*>& x,
                 const std::vector<double*>& y)
 {
@@ -124,30 +133,43 @@ void heff_batch(tn_batch* b, const std::vector<Slot>& S, const std::vector<const
     for (size_t k = 0; k < S.size(); k++) {
         const HeffOp& op = S[k].op;
         const int q = S[k].q, p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
+        const bool idl = op.wl_id >= 0, idr = op.wr_id >= 0;
+        const int nR = Dw2 - (idr ? 1 : 0), r0 = op.wr_id == 0 ? 1 : 0;
+        const int nL = Dw1 - (idl ? 1 : 0), l0 = op.wl_id == 0 ? 1 : 0;
         float* xl = b->xlo[q].as<float>();
         cj.push_back(ConvJob{x[k], xl, op.n()});
         nmax = std::max(nmax, op.n());
         GemmDesc g1{};
         g1.tin = T_F32; g1.tout = T_F32; g1.a_kmajor = true; g1.b_kmajor = true;
-        g1.M = D1p * p; g1.N = Dw2 * D2p; g1.K = D2p; g1.batch = 1;
-        g1.A = xl; g1.lda = D2p; g1.B = op.Rlo; g1.ldb = D2p;
-        g1.C = b->T1[q].p; g1.ldc = (long)Dw2 * D2p; g1.alpha = 1.0f;
+        g1.M = D1p * p; g1.N = nR * D2p; g1.K = D2p; g1.batch = 1;
+        g1.A = xl; g1.lda = D2p; g1.B = (const float*)op.Rlo + (long)r0 * D2p * D2p; g1.ldb = D2p;
+        g1.C = b->T1[q].p; g1.ldc = (long)nR * D2p; g1.alpha = 1.0f;
         g1s.push_back(g1);
-        mj.push_back(MixJob{b->T1[q].p, (long)p * Dw2 * D2p, b->T2[q].p, (long)p * D2p, D1p, D2p, op.mix->t});
+        MixJob m{b->T1[q].p, (long)p * nR * D2p, b->T2[q].p, (long)p * D2p, D1p, D2p, op.mix->t, nullptr, 0, nullptr, 0};
+        if (idl || idr) {
+            TN_REQUIRE(op.mix->t.esel && op.mix->wl == op.wl_id && op.mix->wr == op.wr_id, TN_ERR_STATE,
+                       "heff_batch: mixing table does not match the identity channels");
+            m.in2 = xl;
+            m.isx2 = (long)p * D2p;
+            m.out2 = y[k];
+            m.osx2 = (long)p * D2p;
+        }
+        mj.push_back(m);
         Xmax = std::max(Xmax, D1p);
         Bnmax = std::max(Bnmax, D2p);
         GemmDesc g3{};
         g3.tin = T_F32; g3.tout = T_F64; g3.a_kmajor = true; g3.b_kmajor = false;
-        g3.M = D1p; g3.N = p * D2p; g3.K = Dw1 * D1p; g3.batch = 1;
-        g3.A = op.Llo; g3.lda = (long)Dw1 * D1p; g3.B = b->T2[q].p; g3.ldb = (long)p * D2p;
-        g3.C = y[k]; g3.ldc = (long)p * D2p; g3.alpha = 1.0f;
+        g3.M = D1p; g3.N = p * D2p; g3.K = nL * D1p; g3.batch = 1;
+        g3.A = (const float*)op.Llo + (long)l0 * D1p; g3.lda = (long)Dw1 * D1p; g3.B = b->T2[q].p;
+        g3.ldb = (long)p * D2p;
+        g3.C = y[k]; g3.ldc = (long)p * D2p; g3.alpha = 1.0f; g3.accumulate = idl ? 1 : 0;
         g3s.push_back(g3);
     }
     hipStream_t s = b->ctx->stream;
     TN_HIP(bconvert(push(b, cj), (int)cj.size(), nmax, s));
     gemm_group(b, g1s);
     TN_HIP(bwmix(T_F32, push(b, mj), (int)mj.size(), Xmax, Bnmax, s));
-    gemm_group(b, g3s);
+    gemm_group(b, g3s);  // the accumulate flag travels per problem
 }
 
 // squared norms of the vectors v_s (length n_s), read back
@@ -222,28 +244,25 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             if (st) TN_HIP(hipEventRecord(c->ev[0], s));
             heff_batch(b, sa, xin, yout);
             if (st) TN_HIP(hipEventRecord(c->ev[1], s));
-            std::vector<DotJob> d1, d2, d3;
-            std::vector<ThreeJob> tj;
-            std::vector<GemvJob> gj;
+            // alpha = v_j . w; three-term step fused with the Gram-Schmidt dots;
+            // Gram-Schmidt update fused with the norm (the kernels of tn::lanczos)
+            std::vector<DotJob> d1;
+            std::vector<StepJob> sj3;
+            std::vector<UpdJob> uj;
             for (int k : act) {
                 double* Vk = V(k);
                 const long N = S[k].N;
                 double* w = Vk + (long)(j + 1) * N;
                 double* h = coef + (size_t)k * str;
                 d1.push_back(DotJob{Vk + (long)j * N, N, 1, w, N, h});
-                tj.push_back(ThreeJob{w, Vk + (long)j * N, j > 0 ? Vk + (long)(j - 1) * N : nullptr, h,
-                                      j > 0 ? be[k][j - 1] : 0.0, N});
-                d2.push_back(DotJob{Vk, N, j + 1, w, N, h + 1});
-                gj.push_back(GemvJob{Vk, N, j + 1, h + 1, -1.0, 1.0, w, N});
-                d3.push_back(DotJob{w, N, 1, w, N, h + j + 2});
+                sj3.push_back(StepJob{w, Vk, N, j + 1, h, j > 0 ? be[k][j - 1] : 0.0, N, h + 1});
+                uj.push_back(UpdJob{w, Vk, N, j + 1, h + 1, N, h + j + 2});
             }
             const int na = (int)act.size();
             double* part = b->part.as<double>();
             TN_HIP(bmdot(push(b, d1), na, 1, nmax, part, s));
-            TN_HIP(bthree(push(b, tj), na, nmax, s));
-            TN_HIP(bmdot(push(b, d2), na, j + 1, nmax, part, s));
-            TN_HIP(bgemv(push(b, gj), na, nmax, s));
-            TN_HIP(bmdot(push(b, d3), na, 1, nmax, part, s));
+            TN_HIP(bstep_dots(push(b, sj3), na, j + 1, nmax, part, s));
+            TN_HIP(bupdate_norm(push(b, uj), na, nmax, part, s));
             readback(b, b->hcoef.data(), coef, (size_t)ns * str);
             if (st) {
                 float ms = 0;
@@ -743,6 +762,7 @@ int tn_batch_sweep(tn_batch* b, int dir, const tn_sweep_opts* opts, tn_sweep_sta
         TN_REQUIRE(o.max_krylov >= 2 && o.max_krylov <= b->kmax, TN_ERR_ARG,
                    "tn_batch_sweep: max_krylov outside [2, the batch's]");
         const int nch = (int)b->ch.size();
+        hipStream_t s = c->stream;
         int Lmax = 0;
         for (auto* ch : b->ch) {
             Lmax = std::max(Lmax, ch->L);
@@ -766,9 +786,43 @@ int tn_batch_sweep(tn_batch* b, int dir, const tn_sweep_opts* opts, tn_sweep_sta
                 Slot sl;
                 sl.q = q;
                 sl.site = i;
-                sl.op = make_heff(ch, i, TN_FP32, 0);
-                sl.N = sl.op.n();
                 S.push_back(sl);
+            }
+            // identity environment channels of every slot (first / last channel
+            // of L and R against the identity, one launch, one read back)
+            std::vector<int> wl(S.size(), -1), wr(S.size(), -1);
+            if (o.skip_identity) {
+                std::vector<EyeJob> ej;
+                double* dv = b->coef.as<double>();
+                for (size_t k = 0; k < S.size(); k++) {
+                    tn_chain* ch = b->ch[S[k].q];
+                    const int i = S[k].site, D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+                    const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+                    const double* L = i > 0 ? ch->Lenv[i - 1] : ch->bL;
+                    const double* R = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
+                    ej.push_back(EyeJob{L, (long)Dw1 * D1p, D1, dv + 4 * k});
+                    ej.push_back(EyeJob{L + (long)(Dw1 - 1) * D1p, (long)Dw1 * D1p, D1, dv + 4 * k + 1});
+                    ej.push_back(EyeJob{R, D2p, D2, dv + 4 * k + 2});
+                    ej.push_back(EyeJob{R + (long)(Dw2 - 1) * D2p * D2p, D2p, D2, dv + 4 * k + 3});
+                }
+                TN_HIP(beye_deviation(push(b, ej), (int)ej.size(), b->part.as<double>(), s));
+                std::vector<double> h(4 * S.size());
+                readback(b, h.data(), dv, h.size());
+                auto ok = [](double v) { return std::isfinite(v) && v <= kIdentTolLow; };
+                for (size_t k = 0; k < S.size(); k++) {
+                    tn_chain* ch = b->ch[S[k].q];
+                    const int i = S[k].site, Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+                    if (Dw1 >= 2) wl[k] = ok(h[4 * k]) ? 0 : (ok(h[4 * k + 1]) ? Dw1 - 1 : -1);
+                    if (Dw2 >= 2) wr[k] = ok(h[4 * k + 2]) ? 0 : (ok(h[4 * k + 3]) ? Dw2 - 1 : -1);
+                    if (st) {
+                        st->n_ident_left += wl[k] >= 0;
+                        st->n_ident_right += wr[k] >= 0;
+                    }
+                }
+            }
+            for (size_t k = 0; k < S.size(); k++) {
+                S[k].op = make_heff(b->ch[S[k].q], S[k].site, TN_FP32, 0, wl[k], wr[k]);
+                S[k].N = S[k].op.n();
             }
             std::vector<double> e0, e1;
             lanczos_batch(b, S, o.eig_tol, o.max_restarts, e0, e1, st);

@@ -617,10 +617,11 @@ int tn_ctx_create(int device, tn_ctx** out)
     TN_HIP(hipSetDevice(device));
     TN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) TN_HIP(hipEventCreate(&e));
-    TN_HIP(hipHostMalloc((void**)&c->sc.h, 1024, hipHostMallocCoherent | hipHostMallocMapped));
+    const size_t pub = (size_t)TN_PUBLISH_MAX * sizeof(double);
+    TN_HIP(hipHostMalloc((void**)&c->sc.h, pub + 256, hipHostMallocCoherent | hipHostMallocMapped));
     TN_HIP(hipHostGetDevicePointer((void**)&c->sc.d, c->sc.h, 0));
-    c->sc.flag_h = (unsigned long long*)((char*)c->sc.h + 512);
-    c->sc.flag_d = (unsigned long long*)((char*)c->sc.d + 512);
+    c->sc.flag_h = (unsigned long long*)((char*)c->sc.h + pub);
+    c->sc.flag_d = (unsigned long long*)((char*)c->sc.d + pub);
     *c->sc.flag_h = 0;
     *out = c;
     return TN_OK;

@@ -44,7 +44,7 @@ struct DBuf {
 // host-mapped coherent memory for the per-iteration readback of a few
 // scalars: the device publishes values and a sequence number, the host polls
 struct Scalars {
-    double* h = nullptr;              // host view: 64 doubles, then the flag
+    double* h = nullptr;              // host view: TN_PUBLISH_MAX doubles, then the flag
     double* d = nullptr;              // device view of the same memory
     unsigned long long* flag_h = nullptr;
     unsigned long long* flag_d = nullptr;
@@ -208,8 +208,8 @@ void sync(tn_ctx* c);
 // stream-ordered copies that return when the data has arrived
 void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes);
 void d2h(tn_ctx* c, void* dst, const void* src, size_t bytes);
-// n (<= 64) doubles device -> host through the published scalars (no stream
-// synchronisation: the host polls the completion flag)
+// n (<= TN_PUBLISH_MAX) doubles device -> host through the published scalars
+// (no stream synchronisation: the host polls the completion flag)
 void d2h_small(tn_ctx* c, double* dst, const double* src, int n);
 
 }  // namespace tn

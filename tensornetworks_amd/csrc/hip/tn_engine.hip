@@ -45,7 +45,7 @@ void h2d(tn_ctx* c, void* dst, const void* src, size_t bytes)
 }
 void d2h_small(tn_ctx* c, double* dst, const double* src, int n)
 {
-    TN_REQUIRE(n >= 0 && n <= 64 && c->sc.h, TN_ERR_ARG, "d2h_small: bad size");
+    TN_REQUIRE(n >= 0 && n <= TN_PUBLISH_MAX && c->sc.h, TN_ERR_ARG, "d2h_small: bad size");
     const unsigned long long seq = ++c->sc.seq;
     TN_HIP(publish(src, n, c->sc.d, c->sc.flag_d, seq, c->stream));
     volatile unsigned long long* f = c->sc.flag_h;
@@ -615,7 +615,7 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
     const bool fused = N % 2 == 0 && kmax <= 512;
 
     auto read = [&](double* dst, const double* src, int n) {
-        if (n <= 64) d2h_small(c, dst, src, n);
+        if (n <= TN_PUBLISH_MAX) d2h_small(c, dst, src, n);
         else d2h(c, dst, src, (size_t)n * 8);
     };
     auto norm_host = [&](const double* v) {

@@ -85,8 +85,9 @@ hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const vo
                    long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
 
 // Small device -> host transfer without a stream synchronisation: copies n
-// (<= 64) doubles to host-mapped coherent memory, then (system-scope fence)
-// stores seq into *flag; the host polls the flag.
+// (<= TN_PUBLISH_MAX) doubles to host-mapped coherent memory, then
+// (system-scope fence) stores seq into *flag; the host polls the flag.
+#define TN_PUBLISH_MAX 16384
 hipError_t publish(const double* src, int n, double* host_dst, unsigned long long* host_flag,
                    unsigned long long seq, hipStream_t s);
 
@@ -225,14 +226,50 @@ struct ConvJob {  // y = (float) x
     long n;
 };
 hipError_t bconvert(const ConvJob* jobs, int n, long nmax, hipStream_t s);
-struct MixJob {  // FP32 in
+struct MixJob {  // FP32 in; with t.esel/t.osel set, the identity-channel form of wmix_id
     const void* in;
     long isx;
     void* out;
     long osx;
     int X, Bn;
     MixTable t;
+    const float* in2;  // identity form only: the FP32 input vector, row stride isx2
+    long isx2;
+    double* out2;      // identity form only: y, row stride osx2
+    long osx2;
 };
+// Fused Lanczos step for many problems (n even for every job), the batched
+// form of lz_step_dots / lz_update_norm.  part: bmdot_part_size(n, kmax, nmax)
+// doubles for bstep_dots, n * ceil(nmax / 2048) for bupdate_norm.
+struct StepJob {  // w -= alpha[0] V_{k-1} + beta V_{k-2} (k >= 2), then out[i] = V_i . w, i < k
+    double* w;
+    const double* V;
+    long ldv;
+    int k;
+    const double* alpha;
+    double beta;
+    long n;
+    double* out;
+};
+hipError_t bstep_dots(const StepJob* jobs, int n, int kmax, long nmax, double* part, hipStream_t s);
+struct UpdJob {  // w -= sum_{i<k} coef[i] V_i, then out[0] = |w|^2
+    double* w;
+    const double* V;
+    long ldv;
+    int k;
+    const double* coef;
+    long n;
+    double* out;
+};
+hipError_t bupdate_norm(const UpdJob* jobs, int n, long nmax, double* part, hipStream_t s);
+struct EyeJob {  // out[0] = max |G - I| over the leading n x n block (row stride ld)
+    const double* G;
+    long ld;
+    int n;
+    double* out;
+};
+// part: 64 * n doubles
+hipError_t beye_deviation(const EyeJob* jobs, int n, double* part, hipStream_t s);
 hipError_t bwmix(int tout, const MixJob* jobs, int n, int Xmax, int Bnmax, hipStream_t s);
 hipError_t add_diag(double* G, long ldg, int i0, int i1, double v, hipStream_t s);
 hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* part, hipStream_t s);

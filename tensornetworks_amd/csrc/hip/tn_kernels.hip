@@ -627,13 +627,12 @@ hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const vo
     return hipErrorInvalidValue;
 }
 
-__global__ void __launch_bounds__(64) publish_k(const double* __restrict__ src, int n, double* host_dst,
-                                                unsigned long long* host_flag, unsigned long long seq)
+__global__ void __launch_bounds__(256) publish_k(const double* __restrict__ src, int n, double* host_dst,
+                                                 unsigned long long* host_flag, unsigned long long seq)
 {
-    const int t = threadIdx.x;
-    if (t < n) host_dst[t] = src[t];
+    for (int t = threadIdx.x; t < n; t += 256) host_dst[t] = src[t];
     __syncthreads();
-    if (t == 0) {
+    if (threadIdx.x == 0) {
         __threadfence_system();
         *(volatile unsigned long long*)host_flag = seq;
     }
@@ -642,8 +641,8 @@ __global__ void __launch_bounds__(64) publish_k(const double* __restrict__ src, 
 hipError_t publish(const double* src, int n, double* host_dst, unsigned long long* host_flag,
                    unsigned long long seq, hipStream_t s)
 {
-    if (n < 0 || n > 64) return hipErrorInvalidValue;
-    publish_k<<<1, 64, 0, s>>>(src, n, host_dst, host_flag, seq);
+    if (n < 0 || n > TN_PUBLISH_MAX) return hipErrorInvalidValue;
+    publish_k<<<1, 256, 0, s>>>(src, n, host_dst, host_flag, seq);
     return hipGetLastError();
 }
 
@@ -1642,6 +1641,193 @@ hipError_t bmdot(const DotJob* jobs, int n, int kmax, long nmax, double* part, h
 }
 long bmdot_part_size(int n, int kmax, long nmax) { return (long)n * kmax * ((nmax + BV_CHUNK - 1) / BV_CHUNK); }
 
+// Fused Lanczos step, batched: block (b, q) owns chunk b of job q's vectors
+// (BV_CHUNK entries as BV_EPT / 2 pairs per thread), the layout of lz_step_dots_k.
+#define BV_PAIRS (BV_EPT / 2)
+__global__ void __launch_bounds__(BV_BLOCK) bstep_dots_k(const StepJob* __restrict__ jobs, int kmax, int G,
+                                                         double* __restrict__ part)
+{
+    __shared__ double red[MD_MAXK][BV_BLOCK / 64];
+    const StepJob& j = jobs[blockIdx.y];
+    const long c0 = (long)blockIdx.x * BV_CHUNK;
+    if (c0 >= j.n) return;  // the finalising sum reads only this job's chunks
+    const double a = j.alpha[0];
+    const double* vj = j.V + (long)(j.k - 1) * j.ldv;
+    const double* vm = j.k >= 2 ? j.V + (long)(j.k - 2) * j.ldv : nullptr;
+    double2 wr[BV_PAIRS];
+#pragma unroll
+    for (int e = 0; e < BV_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * BV_BLOCK + threadIdx.x);
+        if (idx < j.n) {
+            double2 x = *(const double2*)(j.w + idx);
+            const double2 p = *(const double2*)(vj + idx);
+            x.x -= a * p.x;
+            x.y -= a * p.y;
+            if (vm) {
+                const double2 q = *(const double2*)(vm + idx);
+                x.x -= j.beta * q.x;
+                x.y -= j.beta * q.y;
+            }
+            *(double2*)(j.w + idx) = x;
+            wr[e] = x;
+        } else {
+            wr[e] = make_double2(0.0, 0.0);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int i = 0; i < j.k; i++) {
+        const double* vi = j.V + (long)i * j.ldv;
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < BV_PAIRS; e++) {
+            const long idx = c0 + 2 * ((long)e * BV_BLOCK + threadIdx.x);
+            if (idx < j.n) {
+                const double2 p = *(const double2*)(vi + idx);
+                acc += p.x * wr[e].x + p.y * wr[e].y;
+            }
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) red[i][wid] = acc;
+    }
+    __syncthreads();
+    double* pq = part + (size_t)blockIdx.y * kmax * G;
+    for (int i = threadIdx.x; i < j.k; i += BV_BLOCK) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < BV_BLOCK / 64; q++) sum += red[i][q];
+        pq[(size_t)i * G + blockIdx.x] = sum;
+    }
+}
+// out[i] = sum over the job's chunks, fixed order; one workgroup per (row, job)
+__global__ void __launch_bounds__(256) bstep_sum_k(const StepJob* __restrict__ jobs, int kmax, int G,
+                                                   const double* __restrict__ part)
+{
+    __shared__ double red[4];
+    const StepJob& j = jobs[blockIdx.y];
+    const int i = blockIdx.x;
+    if (i >= j.k) return;
+    const long nb = (j.n + BV_CHUNK - 1) / BV_CHUNK;
+    const double* pq = part + ((size_t)blockIdx.y * kmax + i) * G;
+    double acc = 0.0;
+    for (long b = threadIdx.x; b < nb; b += 256) acc += pq[b];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) j.out[i] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+hipError_t bstep_dots(const StepJob* jobs, int n, int kmax, long nmax, double* part, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (kmax < 1 || kmax > MD_MAXK) return hipErrorInvalidValue;
+    const int G = (int)((nmax + BV_CHUNK - 1) / BV_CHUNK);
+    bstep_dots_k<<<dim3(G, n), BV_BLOCK, 0, s>>>(jobs, kmax, G, part);
+    bstep_sum_k<<<dim3(kmax, n), 256, 0, s>>>(jobs, kmax, G, part);
+    return hipGetLastError();
+}
+
+// chunks in reverse order of bstep_dots_k (see lz_update_norm_k)
+__global__ void __launch_bounds__(BV_BLOCK) bupdate_norm_k(const UpdJob* __restrict__ jobs, int G,
+                                                           double* __restrict__ part)
+{
+    __shared__ double c[MD_MAXK];
+    __shared__ double red[BV_BLOCK / 64];
+    const UpdJob& j = jobs[blockIdx.y];
+    const int nb = (int)((j.n + BV_CHUNK - 1) / BV_CHUNK);
+    if ((int)blockIdx.x >= nb) return;
+    for (int i = threadIdx.x; i < j.k; i += BV_BLOCK) c[i] = j.coef[i];
+    __syncthreads();
+    const int blk = nb - 1 - blockIdx.x;
+    const long c0 = (long)blk * BV_CHUNK;
+    double2 acc[BV_PAIRS];
+#pragma unroll
+    for (int e = 0; e < BV_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * BV_BLOCK + threadIdx.x);
+        acc[e] = idx < j.n ? *(const double2*)(j.w + idx) : make_double2(0.0, 0.0);
+    }
+    for (int i = 0; i < j.k; i++) {
+        const double* vi = j.V + (long)i * j.ldv;
+        const double ci = c[i];
+#pragma unroll
+        for (int e = 0; e < BV_PAIRS; e++) {
+            const long idx = c0 + 2 * ((long)e * BV_BLOCK + threadIdx.x);
+            if (idx < j.n) {
+                const double2 p = *(const double2*)(vi + idx);
+                acc[e].x -= ci * p.x;
+                acc[e].y -= ci * p.y;
+            }
+        }
+    }
+    double sq = 0.0;
+#pragma unroll
+    for (int e = 0; e < BV_PAIRS; e++) {
+        const long idx = c0 + 2 * ((long)e * BV_BLOCK + threadIdx.x);
+        if (idx < j.n) {
+            *(double2*)(j.w + idx) = acc[e];
+            sq += acc[e].x * acc[e].x + acc[e].y * acc[e].y;
+        }
+    }
+    sq = wave_sum(sq);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * G + blk] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ void __launch_bounds__(256) bupdate_sum_k(const UpdJob* __restrict__ jobs, int G,
+                                                     const double* __restrict__ part)
+{
+    __shared__ double red[4];
+    const UpdJob& j = jobs[blockIdx.x];
+    const long nb = (j.n + BV_CHUNK - 1) / BV_CHUNK;
+    const double* pq = part + (size_t)blockIdx.x * G;
+    double acc = 0.0;
+    for (long b = threadIdx.x; b < nb; b += 256) acc += pq[b];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) j.out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+hipError_t bupdate_norm(const UpdJob* jobs, int n, long nmax, double* part, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int G = (int)((nmax + BV_CHUNK - 1) / BV_CHUNK);
+    bupdate_norm_k<<<dim3(G, n), BV_BLOCK, 0, s>>>(jobs, G, part);
+    bupdate_sum_k<<<n, 256, 0, s>>>(jobs, G, part);
+    return hipGetLastError();
+}
+
+// deviation from the identity, many matrices: 64 workgroups per job
+__global__ void __launch_bounds__(256) beye_dev_k(const EyeJob* __restrict__ jobs, double* __restrict__ part)
+{
+    __shared__ double red[4];
+    const EyeJob& j = jobs[blockIdx.y];
+    double m = 0.0;
+    const long tot = (long)j.n * j.n;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < tot; e += (long)gridDim.x * 256) {
+        const long i = e / j.n, c = e - i * j.n;
+        const double v = j.G[i * j.ld + c] - (i == c ? 1.0 : 0.0);
+        m = fmax(m, isfinite(v) ? fabs(v) : 1e300);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+__global__ void __launch_bounds__(64) beye_max_k(const EyeJob* __restrict__ jobs, int G, const double* __restrict__ part)
+{
+    double m = 0.0;
+    for (int b = threadIdx.x; b < G; b += 64) m = fmax(m, part[(size_t)blockIdx.x * G + b]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (threadIdx.x == 0) jobs[blockIdx.x].out[0] = m;
+}
+hipError_t beye_deviation(const EyeJob* jobs, int n, double* part, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    beye_dev_k<<<dim3(64, n), 256, 0, s>>>(jobs, part);
+    beye_max_k<<<n, 64, 0, s>>>(jobs, 64, part);
+    return hipGetLastError();
+}
+
 // w = beta w + cscale sum_i coef[i] V_i
 __global__ void __launch_bounds__(256) bgemv_k(const GemvJob* __restrict__ jobs)
 {
@@ -1724,6 +1910,17 @@ __global__ void __launch_bounds__(256) bwmix_k(const MixJob* __restrict__ jobs)
     if (b >= j.Bn || x >= j.X) return;
     const float* ip = (const float*)j.in + x * j.isx + b;
     TO* op = (TO*)j.out + x * j.osx + b;
+    if (j.t.esel) {
+        const float* ip2 = j.in2 + x * j.isx2 + b;
+        for (int o = 0; o < j.t.nout; o++) {
+            const int e0 = j.t.optr[o], e1 = j.t.optr[o + 1];
+            float acc = 0.0f;
+            for (int e = e0; e < e1; e++) acc += j.t.evalf[e] * (j.t.esel[e] ? ip2[j.t.eoff[e]] : ip[j.t.eoff[e]]);
+            if (j.t.osel[o]) j.out2[x * j.osx2 + j.t.ooff[o] + b] = (double)acc;
+            else op[j.t.ooff[o]] = cvt_from_f<TO>(acc);
+        }
+        return;
+    }
     for (int o = 0; o < j.t.nout; o++) {
         const int e0 = j.t.optr[o], e1 = j.t.optr[o + 1];
         float acc = 0.0f;

@@ -47,7 +47,29 @@ def test_batch_matches_single_chain_sweeps(ctx):
         assert abs(eb - es) < 1e-8 * max(1.0, abs(es)), (L, eb, es)
         assert cb.dims() == cs.dims()
     assert st.n_matvec > 0 and st.n_qr > 0
+    # the bulk sites apply their identity environment channels as copies
+    assert st.n_ident_left > 0 and st.n_ident_right > 0
     b.close()
+
+
+def test_batch_identity_channels_same_sweeps(ctx):
+    # the batched operator with identity channels copied gives the sweeps of
+    # the full contraction
+    spec = [(10, 16, 0.8, 1.2, 0.1), (7, 24, 1.2, 0.5, -0.3)]
+    energies = []
+    for skip in (0, 1):
+        o = fp32_opts(skip_identity=skip)
+        chains = [make_chain(ctx, *sp, seed=50 + k, opts=o) for k, sp in enumerate(spec)]
+        b = tn.Batch(ctx, chains, max_krylov=40)
+        st = tn.SweepStats()
+        for _ in range(2):
+            b.sweep(0, o, st)
+            b.sweep(1, o, st)
+        assert (st.n_ident_left > 0) == bool(skip)
+        energies.append([ch.expectation() for ch in chains])
+        b.close()
+    for e0, e1 in zip(*energies):
+        assert abs(e1 - e0) < 1e-6 * abs(e0), (e0, e1)
 
 
 def test_batch_reaches_exact_diagonalisation(ctx):
