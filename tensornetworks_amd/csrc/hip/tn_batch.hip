@@ -147,7 +147,8 @@ void heff_batch(tn_batch* b, const std::vector<Slot>& S, const std::vector<const
         g1s.push_back(g1);
         MixJob m{b->T1[q].p, (long)p * nR * D2p, b->T2[q].p, (long)p * D2p, D1p, D2p, op.mix->t, nullptr, 0, nullptr, 0};
         if (idl || idr) {
-            TN_REQUIRE(op.mix->t.esel && op.mix->wl == op.wl_id && op.mix->wr == op.wr_id, TN_ERR_STATE,
+            TN_REQUIRE(op.mix->op == MIX_HEFF && op.mix->t.esel && op.mix->wl == op.wl_id && op.mix->wr == op.wr_id,
+                       TN_ERR_STATE,
                        "heff_batch: mixing table does not match the identity channels");
             m.in2 = xl;
             m.isx2 = (long)p * D2p;
@@ -821,7 +822,7 @@ int tn_batch_sweep(tn_batch* b, int dir, const tn_sweep_opts* opts, tn_sweep_sta
                 }
             }
             for (size_t k = 0; k < S.size(); k++) {
-                S[k].op = make_heff(b->ch[S[k].q], S[k].site, TN_FP32, 0, wl[k], wr[k]);
+                S[k].op = make_heff(b->ch[S[k].q], S[k].site, TN_FP32, 0, wl[k], wr[k], MIX_HEFF);
                 S[k].N = S[k].op.n();
             }
             std::vector<double> e0, e1;

@@ -54,8 +54,9 @@ double now_ms();
 void ensure_lo(tn_chain* ch, int prec);
 const void* lo_left(tn_chain* ch, int prec, int i);
 const void* lo_right(tn_chain* ch, int prec, int i);
-// operator of site i; wl/wr: identity environment channels (see HeffOp), -1 none
-HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl = -1, int wr = -1);
+// operator of site i; wl/wr: identity environment channels (see HeffOp), -1 none;
+// mixop: MIX_HEFF_T for heff_apply, MIX_HEFF for the batched operator
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl = -1, int wr = -1, int mixop = MIX_HEFF_T);
 // identity channels of the environments of site i (first or last channel
 // within tol of the identity on the logical block), -1 where there is none
 void env_identity(tn_chain* ch, int i, double tol, int* wl, int* wr);

@@ -211,7 +211,7 @@ void env_identity(tn_chain* ch, int i, double tol, int* wl, int* wr)
     if (Dw2 >= 2) *wr = ok(h[2]) ? 0 : (ok(h[3]) ? Dw2 - 1 : -1);
 }
 
-HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl, int wr)
+HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl, int wr, int mixop)
 {
     HeffOp op;
     op.p = ch->d;
@@ -226,7 +226,7 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl, int wr)
     op.Rlo = lo_right(ch, prec, i);
     op.wl_id = wl;
     op.wr_id = wr;
-    mix_build(ch->mixH[i], MIX_HEFF, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream,
+    mix_build(ch->mixH[i], mixop, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream,
               op.wl_id, op.wr_id);
     op.mix = &ch->mixH[i];
     op.tile = tile;
@@ -817,7 +817,7 @@ int tn_site_bind(tn_ctx* c, int d, int D1, int D2, int Dw1, int Dw2, const doubl
     };
     c->b_wl = pick(Li, D1p, (long)Dw1 * D1p, Dw1, D1);
     c->b_wr = pick(Ri, (long)D2p * D2p, D2p, Dw2, D2);
-    mix_build(c->b_mix_heff, MIX_HEFF, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream, c->b_wl, c->b_wr);
+    mix_build(c->b_mix_heff, MIX_HEFF_T, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream, c->b_wl, c->b_wr);
     const long n = (long)D1p * d * D2p;
     c->b_x.ensure(n * 8);
     c->b_y.ensure(n * 8);
@@ -1372,7 +1372,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.op.Llo = lo_left(ch, o.precision, i);
         hc.op.Rlo = lo_right(ch, o.precision, j);
         ch->mix2H[i].wsrc = nullptr;  // W12 rebuilt each visit
-        mix_build(ch->mix2H[i], MIX_HEFF, ch->W12[i].data(), Dw1, Dw3, p2, D1p, D3p, s);
+        mix_build(ch->mix2H[i], MIX_HEFF_T, ch->W12[i].data(), Dw1, Dw3, p2, D1p, D3p, s);
         hc.op.mix = &ch->mix2H[i];
         hc.op.tile = o.gemm_tile;
         hc.st = st;

@@ -59,7 +59,10 @@ struct DevMix {
     const double* wsrc = nullptr;
     int op = -1, D1p = -1, D2p = -1, p = -1, dw1 = -1, dw2 = -1, wl = -1, wr = -1;
 };
-enum MixOp { MIX_HEFF = 0, MIX_ENVL = 1, MIX_ENVR = 2 };
+// MIX_HEFF_T: the Heff mixing of heff_apply, written transposed for an NT
+// stage 3 (T2t[(m,b)][(w,a')]), with esel/osel always present (identity
+// channels optional); MIX_HEFF: the [w][a'][m][b] form (batched path).
+enum MixOp { MIX_HEFF = 0, MIX_ENVL = 1, MIX_ENVR = 2, MIX_HEFF_T = 3 };
 // W host array [dw1][dw2][p][p]; builds (or reuses) the table for the op/dims.
 // MIX_HEFF with wl >= 0 or wr >= 0: table for the operator whose left channel
 // wl / right channel wr is the identity (first or last channel), with the

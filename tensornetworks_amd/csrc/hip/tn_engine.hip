@@ -167,11 +167,14 @@ void gemm_run(tn_ctx* c, const GemmDesc& g)
 void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int D1p, int D2p,
                hipStream_t s, int wl, int wr)
 {
-    if (op != MIX_HEFF) wl = wr = -1;
+    if (op != MIX_HEFF && op != MIX_HEFF_T) wl = wr = -1;
     if (m.wsrc == W && m.op == op && m.D1p == D1p && m.D2p == D2p && m.p == p && m.dw1 == dw1 &&
         m.dw2 == dw2 && m.wl == wl && m.wr == wr)
         return;
-    const bool ident = wl >= 0 || wr >= 0;
+    const bool ident = wl >= 0 || wr >= 0 || op == MIX_HEFF_T;
+    const bool heff = op == MIX_HEFF || op == MIX_HEFF_T;
+    // transposed stage-3 operand: row (m, b), column (w, a'), ldo = nL * D1p
+    const long ldo = (long)(dw1 - (wl >= 0 ? 1 : 0)) * D1p;
     // right channels kept in the stage-1 output and left channels kept in the
     // stage-3 input, renumbered without the identity channel
     const int nR = dw2 - (wr >= 0 ? 1 : 0);
@@ -182,15 +185,16 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     std::vector<double> ev;
     auto Wv = [&](int w, int wp, int mm, int n) { return W[(((long)w * dw2 + wp) * p + mm) * p + n]; };
     optr.push_back(0);
-    if (op == MIX_HEFF || op == MIX_ENVR) {
+    if (heff || op == MIX_ENVR) {
         for (int w = 0; w < dw1; w++)
             for (int mm = 0; mm < p; mm++) {
-                if (op == MIX_HEFF && w == wl) {
+                if (heff && w == wl) {
                     ooff.push_back((long)mm * D2p);  // y[a][mm][b]
                     osel.push_back(1);
                 } else {
-                    ooff.push_back(op == MIX_HEFF ? (long)cl(w) * D1p * p * D2p + (long)mm * D2p
-                                                  : (long)w * p * D2p + (long)mm * D2p);
+                    if (op == MIX_HEFF_T) ooff.push_back((long)mm * D2p * ldo + (long)cl(w) * D1p);
+                    else if (op == MIX_HEFF) ooff.push_back((long)cl(w) * D1p * p * D2p + (long)mm * D2p);
+                    else ooff.push_back((long)w * p * D2p + (long)mm * D2p);
                     osel.push_back(0);
                 }
                 for (int wp = 0; wp < dw2; wp++)
@@ -291,9 +295,9 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
 {
     hipStream_t s = c->stream;
     const int p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
-    const bool idl = op.wl_id >= 0, idr = op.wr_id >= 0, ident = idl || idr;
-    TN_REQUIRE(!ident || (op.mix->wl == op.wl_id && op.mix->wr == op.wr_id), TN_ERR_STATE,
-               "heff_apply: mixing table does not match the identity channels");
+    const bool idl = op.wl_id >= 0, idr = op.wr_id >= 0;
+    TN_REQUIRE(op.mix->op == MIX_HEFF_T && op.mix->wl == op.wl_id && op.mix->wr == op.wr_id, TN_ERR_STATE,
+               "heff_apply: mixing table does not match the operator");
     TN_REQUIRE(!idl || (Dw1 >= 2 && (op.wl_id == 0 || op.wl_id == Dw1 - 1)), TN_ERR_STATE,
                "heff_apply: left identity channel must be the first or last of >= 2");
     TN_REQUIRE(!idr || (Dw2 >= 2 && (op.wr_id == 0 || op.wr_id == Dw2 - 1)), TN_ERR_STATE,
@@ -317,14 +321,15 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
     g1.ldc = (long)nR * D2p;
     g1.alpha = 1.0f;
     g1.tile = op.tile;
+    // stage 3 as NT: the MPO stage writes T2 transposed, T2t[(m,b)][(w,a')]
     g3.a_kmajor = true;
-    g3.b_kmajor = false;
+    g3.b_kmajor = true;
     g3.M = D1p;
     g3.N = p * D2p;
     g3.K = nL * D1p;
     g3.batch = 1;
     g3.lda = (long)Dw1 * D1p;
-    g3.ldb = (long)p * D2p;
+    g3.ldb = (long)nL * D1p;
     g3.ldc = (long)p * D2p;
     g3.alpha = 1.0f;
     g3.tile = op.tile;
@@ -338,12 +343,8 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R + roff; g1.C = c->T1.p;
         gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        if (ident)
-            TN_HIP(wmix_id(T_F64, T_F64, T_F64, c->T1.p, (long)p * nR * D2p, x, (long)p * D2p, c->T2.p,
-                           (long)p * D2p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
-        else
-            TN_HIP(wmix(T_F64, T_F64, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
-                        op.mix->t, s));
+        TN_HIP(wmix_idt(T_F64, T_F64, T_F64, c->T1.p, (long)p * nR * D2p, x, (long)p * D2p, c->T2.p,
+                        (long)nL * D1p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
         g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L + loff; g3.B = c->T2.p; g3.C = y;
         gemm_run(c, g3);
@@ -358,12 +359,8 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = (const char*)op.Rlo + roff * es; g1.C = c->T1.p;
         gemm_run(c, g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        if (ident)
-            TN_HIP(wmix_id(T_F32, tl, tl, c->T1.p, (long)p * nR * D2p, c->xlo.p, (long)p * D2p, c->T2.p,
-                           (long)p * D2p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
-        else
-            TN_HIP(wmix(T_F32, tl, c->T1.p, (long)p * Dw2 * D2p, c->T2.p, (long)p * D2p, D1p, D2p,
-                        op.mix->t, s));
+        TN_HIP(wmix_idt(T_F32, tl, tl, c->T1.p, (long)p * nR * D2p, c->xlo.p, (long)p * D2p, c->T2.p,
+                        (long)nL * D1p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
         g3.tin = tl; g3.tout = T_F64; g3.A = (const char*)op.Llo + loff * es; g3.B = c->T2.p; g3.C = y;
         gemm_run(c, g3);

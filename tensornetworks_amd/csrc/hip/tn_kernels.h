@@ -83,6 +83,12 @@ hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx
 // stage-3 contribution is a copy of the mixed slice into y (out2).
 hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
                    long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
+// The same with the stage-3 operand written transposed: outputs o with
+// osel[o] = 0 go to out[ooff[o] + b*ldo + x] (rows (m, b), columns (w, a')),
+// through an LDS tile of 32 x-values by 64 b-values so both the reads (along
+// b) and the writes (along x) are coalesced.  X must be a multiple of 32.
+hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
+                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
 
 // Small device -> host transfer without a stream synchronisation: copies n
 // (<= TN_PUBLISH_MAX) doubles to host-mapped coherent memory, then

@@ -45,6 +45,11 @@ __device__ __forceinline__ double to_d(bf16 v) { return (double)bf2f(v); }
 __device__ __forceinline__ float to_f(double v) { return (float)v; }
 __device__ __forceinline__ float to_f(float v) { return v; }
 __device__ __forceinline__ float to_f(bf16 v) { return bf2f(v); }
+template <typename A, typename T> __device__ __forceinline__ A to_acc(T v)
+{
+    if constexpr (std::is_same<A, double>::value) return to_d(v);
+    else return to_f(v);
+}
 
 // ---------------------------------------------------------------------------
 // MFMA traits: 16x16 output block per wave instruction
@@ -606,6 +611,83 @@ __global__ void __launch_bounds__(256) wmix_id_k(const TI* __restrict__ in, long
             else out[x * osx + ooff[o] + b] = cvt_from_f<TO>(acc);
         }
     }
+}
+
+template <typename TI, typename TX, typename TO>
+__global__ void __launch_bounds__(256) wmix_idt_k(const TI* __restrict__ in, long isx, const TX* __restrict__ in2,
+                                                  long isx2, TO* __restrict__ out, long ldo,
+                                                  double* __restrict__ out2, long osx2, int Bn, int nout,
+                                                  const int* __restrict__ optr, const long* __restrict__ ooff,
+                                                  const long* __restrict__ eoff, const int* __restrict__ esel,
+                                                  const int* __restrict__ osel, const double* __restrict__ eval,
+                                                  const float* __restrict__ evalf)
+{
+    typedef typename std::conditional<std::is_same<TI, double>::value, double, float>::type acc_t;
+    __shared__ acc_t tile[64][33];
+    const int tb = threadIdx.x & 63, tr = threadIdx.x >> 6;  // b within the tile, row group of 8
+    const int b = blockIdx.x * 64 + tb;
+    const long x0 = (long)blockIdx.y * 32 + tr * 8;
+    const bool bok = b < Bn;
+    // writer role: row wb of the tile (b), columns wc..wc+7 (x)
+    const int wb = threadIdx.x >> 2, wc = (threadIdx.x & 3) * 8;
+    const bool wok = blockIdx.x * 64 + wb < Bn;
+    for (int o = 0; o < nout; o++) {
+        const int e0 = optr[o], e1 = optr[o + 1];
+        acc_t acc[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) acc[r] = 0;
+        if (bok) {
+            for (int e = e0; e < e1; e++) {
+                const acc_t v = std::is_same<TI, double>::value ? (acc_t)eval[e] : (acc_t)evalf[e];
+                const long off = eoff[e] + b;
+                if (esel[e]) {
+#pragma unroll
+                    for (int r = 0; r < 8; r++) acc[r] += v * to_acc<acc_t>(in2[(x0 + r) * isx2 + off]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 8; r++) acc[r] += v * to_acc<acc_t>(in[(x0 + r) * isx + off]);
+                }
+            }
+        }
+        if (osel[o]) {  // identity channel of L: straight into y[x][m][b]
+            if (bok) {
+#pragma unroll
+                for (int r = 0; r < 8; r++) out2[(x0 + r) * osx2 + ooff[o] + b] = (double)acc[r];
+            }
+            continue;
+        }
+        __syncthreads();  // the previous output's tile has been written out
+#pragma unroll
+        for (int r = 0; r < 8; r++) tile[tb][tr * 8 + r] = acc[r];
+        __syncthreads();
+        if (wok) {
+            TO* dst = out + ooff[o] + (long)(blockIdx.x * 64 + wb) * ldo + (long)blockIdx.y * 32 + wc;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                if constexpr (std::is_same<TO, double>::value) dst[c] = (double)tile[wb][wc + c];
+                else dst[c] = cvt_from_f<TO>((float)tile[wb][wc + c]);
+            }
+        }
+    }
+}
+
+hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
+                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s)
+{
+    if (!t.esel || !t.osel || X % 32) return hipErrorInvalidValue;
+    dim3 grid((Bn + 63) / 64, X / 32);
+#define WT_CASE(TIc, TXc, TOc, TIt, TXt, TOt)                                                              \
+    if (tin == TIc && tx == TXc && tout == TOc) {                                                          \
+        wmix_idt_k<TIt, TXt, TOt><<<grid, 256, 0, s>>>((const TIt*)in, isx, (const TXt*)in2, isx2,         \
+                                                       (TOt*)out, ldo, out2, osx2, Bn, t.nout, t.optr,     \
+                                                       t.ooff, t.eoff, t.esel, t.osel, t.eval, t.evalf);   \
+        return hipGetLastError();                                                                          \
+    }
+    WT_CASE(T_F64, T_F64, T_F64, double, double, double)
+    WT_CASE(T_F32, T_F32, T_F32, float, float, float)
+    WT_CASE(T_F32, T_BF16, T_BF16, float, bf16, bf16)
+#undef WT_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
