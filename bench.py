@@ -61,6 +61,8 @@ def parse():
                         "precond: train the learned Davidson preconditioner on the device and compare local "
                         "solvers on the S=1/2 Heisenberg chain L=128, D=1024")
     p.add_argument("--chains", type=int, default=128, help="batch workload: number of chains")
+    p.add_argument("--no-sequential", action="store_true",
+                   help="batch workload: skip the chain-by-chain comparison (for profiling runs)")
     p.add_argument("--precond-out", default=None, help="precond workload: write the trained network (.npz) here")
     p.add_argument("--heff-only", action="store_true",
                    help="set up the chain and time only the bulk-site Heff stages (for counter runs), "
@@ -129,6 +131,10 @@ def run_batch(a, tn, ctx):
     E_b = [ch.expectation() for ch in chains]
     print(f"# timed {a.steps} batched sweep(s) {wall:.2f}s matvecs={st.n_matvec} qr={st.n_qr} "
           f"fallback={st.n_qr_fallback}", file=sys.stderr, flush=True)
+    if a.no_sequential:
+        print(json.dumps({"metric": "batched DMRG sweep time (profiling run, no chain-by-chain comparison)",
+                          "ms_per_step": wall / a.steps * 1e3}), flush=True)
+        return
     # the same sweep(s) chain by chain from the saved states
     for ch, sv in zip(chains, saved):
         for i, (d1, d2, M) in enumerate(sv):

@@ -499,9 +499,57 @@ int sturm_count(int k, const double* a, const double* b2, double x)
 }
 }  // namespace
 
+// Lowest eigenvalue of tridiag(b, a, b) by Laguerre's iteration from below
+// (the Gershgorin bound): for a polynomial with real roots it increases
+// monotonically to the smallest root and converges cubically.  G = f'/f and
+// H = -(ln f)'' come from the LDL^T pivots d_i of T - x, which stay positive
+// below the smallest root.  Returns false (and leaves x) if it does not settle.
+static bool laguerre_lowest(int k, const double* a, const double* b2, double lo, double span, double* x)
+{
+    double xv = lo;
+    for (int it = 0; it < 60; it++) {
+        double d = a[0] - xv, dp = -1.0, dpp = 0.0;
+        if (!(d > 0.0)) break;
+        double G = dp / d, H = (dp / d) * (dp / d) - dpp / d;
+        for (int i = 1; i < k; i++) {
+            const double q = b2[i - 1] / d;
+            const double dn = a[i] - xv - q;
+            const double dpn = -1.0 + q * dp / d;
+            const double dppn = q * (dpp * d - 2.0 * dp * dp) / (d * d);
+            d = dn;
+            dp = dpn;
+            dpp = dppn;
+            if (!(d > 0.0)) break;
+            G += dp / d;
+            H += (dp / d) * (dp / d) - dpp / d;
+        }
+        if (!(d > 0.0)) {
+            // at (rounding) the root: a pivot reached zero
+            *x = xv;
+            return true;
+        }
+        const double n = (double)k;
+        const double disc = std::max(0.0, (n - 1.0) * (n * H - G * G));
+        const double den = G - std::sqrt(disc);  // G < 0 below every root
+        if (!(den < 0.0)) break;
+        const double step = -n / den;
+        xv += step;
+        if (step <= 4e-16 * (std::fabs(xv) + 1e-3 * span)) {
+            *x = xv;
+            return true;
+        }
+    }
+    return false;
+}
+
 void tridiag_lowest(int k, const double* a, const double* b, double* t0, double* t1, double* z, bool want_t1)
 {
-    std::vector<double> b2(std::max(k, 1));
+    thread_local std::vector<double> work;
+    thread_local std::vector<unsigned char> piv;
+    work.resize(8 * (size_t)std::max(k, 1));
+    piv.resize(std::max(k, 1));
+    double* b2 = work.data();
+    double *dd = b2 + k, *du = dd + k, *dl = du + k, *du2 = dl + k, *rhs = du2 + k, *zz = rhs + k;
     for (int i = 0; i + 1 < k; i++) b2[i] = b[i] * b[i];
     double lo = a[0], hi = a[0];
     for (int i = 0; i < k; i++) {
@@ -517,12 +565,14 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
         for (int it = 0; it < 200; it++) {
             const double mid = 0.5 * (l + h);
             if (mid <= l || mid >= h) break;
-            if (sturm_count(k, a, b2.data(), mid) > m) h = mid;
+            if (sturm_count(k, a, b2, mid) > m) h = mid;
             else l = mid;
         }
         return 0.5 * (l + h);
     };
-    *t0 = eig(0);
+    double x0;
+    if (!laguerre_lowest(k, a, b2, lo, span, &x0)) x0 = eig(0);
+    *t0 = x0;
     if (t1) *t1 = (want_t1 && k > 1) ? eig(1) : *t0;
     if (k == 1) {
         z[0] = 1.0;
@@ -531,8 +581,6 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
     double scale = 0.0;
     for (int i = 0; i < k; i++) scale = std::max(scale, std::fabs(a[i]) + (i < k - 1 ? std::fabs(b[i]) : 0.0));
     const double pert = (scale > 0 ? scale : 1.0) * 1e-14;
-    std::vector<double> dd(k), du(k), dl(k), du2(k), rhs(k), zz(k);
-    std::vector<unsigned char> piv(k);
     for (int i = 0; i < k; i++) zz[i] = 1.0 / std::sqrt((double)k) * (1.0 + 0.01 * ((i * 7919) % 13) / 13.0);
     for (int iter = 0; iter < 3; iter++) {
         for (int i = 0; i < k; i++) {
