@@ -20,9 +20,14 @@
 // CholeskyQR misses the orthogonality bound takes the single-chain gauge path
 // (shifted passes, SVD fallback) for that site.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <mutex>
+#include <thread>
 
 #include "tn_chain.h"
 
@@ -41,45 +46,166 @@ struct tn_batch {
     char* ring_h = nullptr;
     DBuf ring_d;
     size_t ring_cap = 0, ring_off = 0;
+    size_t stage_begin = 0;  // start of the staged, not yet copied, descriptors
     std::vector<double> hcoef;
+    // host threads for the per-chain work between launches
+    struct Pool* pool = nullptr;
+};
+
+// A few host threads for the per-chain host work of a lockstep iteration
+// (tridiagonal Ritz solves of every active chain).  run(n, f) calls f(i) for
+// i < n on the workers and the calling thread and returns when all are done;
+// the first exception is rethrown on the caller.
+struct Pool {
+    std::vector<std::thread> th;
+    std::mutex m;
+    std::condition_variable cv, done;
+    std::function<void(int)> fn;
+    std::atomic<int> next{0};
+    int n = 0, busy = 0;
+    unsigned long long gen = 0;
+    bool stop = false;
+    std::exception_ptr err;
+
+    explicit Pool(int workers)
+    {
+        for (int t = 0; t < workers; t++) th.emplace_back([this] { loop(); });
+    }
+    ~Pool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : th) t.join();
+    }
+    void work()
+    {
+        for (int i; (i = next.fetch_add(1)) < n;) {
+            try {
+                fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(m);
+                if (!err) err = std::current_exception();
+            }
+        }
+    }
+    void loop()
+    {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return stop || gen != seen; });
+                if (stop) return;
+                seen = gen;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(m);
+            if (--busy == 0) done.notify_one();
+        }
+    }
+    void run(int items, const std::function<void(int)>& f)
+    {
+        if (items < 16 || th.empty()) {
+            for (int i = 0; i < items; i++) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            fn = f;
+            n = items;
+            next = 0;
+            busy = (int)th.size();
+            err = nullptr;
+            gen++;
+        }
+        cv.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m);
+        done.wait(lk, [&] { return busy == 0; });
+        if (err) std::rethrow_exception(err);
+    }
 };
 
 namespace {
 
-void batch_sync(tn_batch* b)
+// copy the staged descriptors [stage_begin, ring_off) to the device, as one
+// transfer ordered before the launches that follow on the stream
+void flush(tn_batch* b)
 {
-    sync(b->ctx);
-    b->ring_off = 0;
+    if (b->ring_off > b->stage_begin)
+        TN_HIP(hipMemcpyAsync((char*)b->ring_d.p + b->stage_begin, b->ring_h + b->stage_begin,
+                              b->ring_off - b->stage_begin, hipMemcpyHostToDevice, b->ctx->stream));
+    b->stage_begin = b->ring_off;
 }
 
+// the ring may be reused from its start: the caller has just synchronised
+// with the stream, after every staged list was flushed
+void ring_reset(tn_batch* b)
+{
+    TN_REQUIRE(b->stage_begin == b->ring_off, TN_ERR_STATE, "batch: descriptors staged but not copied");
+    b->ring_off = b->stage_begin = 0;
+}
+
+void batch_sync(tn_batch* b)
+{
+    flush(b);
+    sync(b->ctx);
+    b->ring_off = b->stage_begin = 0;
+}
+
+// place a descriptor list in the ring; the device copy is valid after the next
+// flush (stage several lists, flush once, then launch)
 template <typename T>
-const T* push(tn_batch* b, const std::vector<T>& v)
+const T* stage(tn_batch* b, const std::vector<T>& v)
 {
     const size_t bytes = v.size() * sizeof(T);
     const size_t span = (bytes + 255) & ~(size_t)255;
     TN_REQUIRE(span <= b->ring_cap, TN_ERR_ARG, "batch: descriptor list larger than the ring");
     if (b->ring_off + span > b->ring_cap) batch_sync(b);
     char* h = b->ring_h + b->ring_off;
-    char* d = (char*)b->ring_d.p + b->ring_off;
     std::memcpy(h, v.data(), bytes);
-    TN_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, b->ctx->stream));
+    const T* d = (const T*)((char*)b->ring_d.p + b->ring_off);
     b->ring_off += span;
-    return (const T*)d;
+    return d;
+}
+
+template <typename T>
+const T* push(tn_batch* b, const std::vector<T>& v)
+{
+    const T* d = stage(b, v);
+    flush(b);
+    return d;
 }
 
 // read n doubles back; waits for the stream to reach the read (polled
 // completion flag up to TN_PUBLISH_MAX values), which also frees the ring
 void readback(tn_batch* b, double* dst, const double* src, size_t n)
 {
+    flush(b);
     if (n <= (size_t)TN_PUBLISH_MAX) d2h_small(b->ctx, dst, src, (int)n);
     else d2h(b->ctx, dst, src, n * 8);
-    b->ring_off = 0;
+    b->ring_off = b->stage_begin = 0;
 }
 
-// A list of GEMMs sharing types and layouts, as one grouped launch per tile.
-void gemm_group(tn_batch* b, const std::vector<GemmDesc>& list)
+// A list of GEMMs sharing types and layouts, as one grouped launch per tile:
+// gemm_group_stage places the descriptors, gemm_group_launch launches them
+// (after a flush).
+struct GroupLaunch {
+    int tin, tout;
+    bool akm, bkm;
+    int tile, bk;
+    const GemmProb* probs;
+    const int* off;
+    int nprob, total;
+};
+
+std::vector<GroupLaunch> gemm_group_stage(tn_batch* b, const std::vector<GemmDesc>& list)
 {
-    if (list.empty()) return;
+    std::vector<GroupLaunch> out;
+    if (list.empty()) return out;
     tn_ctx* c = b->ctx;
     std::map<std::pair<int, int>, std::vector<int>> groups;
     for (int q = 0; q < (int)list.size(); q++) {
@@ -102,11 +228,26 @@ void gemm_group(tn_batch* b, const std::vector<GemmDesc>& list)
             probs.push_back(GemmProb{g.A, g.B, g.C, g.lda, g.ldb, g.ldc, g.M, g.N, g.K, g.alpha, g.accumulate});
             off.push_back(off.back() + (g.M / bm) * (g.N / bn));
         }
-        const GemmProb* dp = push(b, probs);
-        const int* doff = push(b, off);
-        TN_HIP(gemm_grouped(list[0].tin, list[0].tout, list[0].a_kmajor, list[0].b_kmajor, tile, bk, dp, doff,
-                            (int)probs.size(), off.back(), c->stream));
+        const GemmProb* dp = stage(b, probs);
+        const int* doff = stage(b, off);
+        out.push_back(GroupLaunch{list[0].tin, list[0].tout, list[0].a_kmajor, list[0].b_kmajor, tile, bk, dp, doff,
+                                  (int)probs.size(), off.back()});
     }
+    return out;
+}
+
+void gemm_group_launch(tn_batch* b, const std::vector<GroupLaunch>& L)
+{
+    for (const GroupLaunch& g : L)
+        TN_HIP(gemm_grouped(g.tin, g.tout, g.akm, g.bkm, g.tile, g.bk, g.probs, g.off, g.nprob, g.total,
+                            b->ctx->stream));
+}
+
+void gemm_group(tn_batch* b, const std::vector<GemmDesc>& list)
+{
+    std::vector<GroupLaunch> L = gemm_group_stage(b, list);
+    flush(b);
+    gemm_group_launch(b, L);
 }
 
 // one active chain of a round
@@ -122,8 +263,17 @@ struct Slot {
 // a copy, as in heff_apply: stage 1 skips the right one, the mixing stage
 // reads x for it and writes the left one's slice into y, stage 3 skips the
 // left one and accumulates.
-void heff_batch(tn_batch* b, const std::vector<Slot>& S, const std::vector<const double*>& x,
-                const std::vector<double*>& y)
+struct HeffLaunch {
+    const ConvJob* conv = nullptr;
+    int nconv = 0;
+    long nmax = 0;
+    std::vector<GroupLaunch> g1, g3;
+    const MixJob* mix = nullptr;
+    int nmix = 0, Xmax = 0, Bnmax = 0;
+};
+
+HeffLaunch heff_batch_stage(tn_batch* b, const std::vector<Slot>& S, const std::vector<const double*>& x,
+                            const std::vector<double*>& y)
 {
     std::vector<ConvJob> cj;
     std::vector<GemmDesc> g1s, g3s;
@@ -166,11 +316,26 @@ void heff_batch(tn_batch* b, const std::vector<Slot>& S, const std::vector<const
         g3.C = y[k]; g3.ldc = (long)p * D2p; g3.alpha = 1.0f; g3.accumulate = idl ? 1 : 0;
         g3s.push_back(g3);
     }
+    HeffLaunch h;
+    h.conv = stage(b, cj);
+    h.nconv = (int)cj.size();
+    h.nmax = nmax;
+    h.g1 = gemm_group_stage(b, g1s);
+    h.mix = stage(b, mj);
+    h.nmix = (int)mj.size();
+    h.Xmax = Xmax;
+    h.Bnmax = Bnmax;
+    h.g3 = gemm_group_stage(b, g3s);  // the accumulate flag travels per problem
+    return h;
+}
+
+void heff_batch_launch(tn_batch* b, const HeffLaunch& h)
+{
     hipStream_t s = b->ctx->stream;
-    TN_HIP(bconvert(push(b, cj), (int)cj.size(), nmax, s));
-    gemm_group(b, g1s);
-    TN_HIP(bwmix(T_F32, push(b, mj), (int)mj.size(), Xmax, Bnmax, s));
-    gemm_group(b, g3s);  // the accumulate flag travels per problem
+    TN_HIP(bconvert(h.conv, h.nconv, h.nmax, s));
+    gemm_group_launch(b, h.g1);
+    TN_HIP(bwmix(T_F32, h.mix, h.nmix, h.Xmax, h.Bnmax, s));
+    gemm_group_launch(b, h.g3);
 }
 
 // squared norms of the vectors v_s (length n_s), read back
@@ -230,6 +395,10 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             TN_HIP(bscale(push(b, sj), (int)sj.size(), nmax, s));
         }
         std::vector<int> act = todo;
+        std::vector<ScaleJob> pend;  // v_j = w / beta of the chains going on, run before the next Heff
+        long pend_nmax = 1;
+        std::vector<double> resid(ns, 0.0);
+        std::vector<char> fin(ns, 0);
         for (int j = 0; j < kmax && !act.empty(); j++) {
             std::vector<Slot> sa;
             std::vector<const double*> xin;
@@ -241,12 +410,12 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 yout.push_back(V(k) + (long)(j + 1) * S[k].N);
                 nmax = std::max(nmax, S[k].N);
             }
-            double t0 = now_ms();
-            if (st) TN_HIP(hipEventRecord(c->ev[0], s));
-            heff_batch(b, sa, xin, yout);
-            if (st) TN_HIP(hipEventRecord(c->ev[1], s));
-            // alpha = v_j . w; three-term step fused with the Gram-Schmidt dots;
-            // Gram-Schmidt update fused with the norm (the kernels of tn::lanczos)
+            // every descriptor of the iteration in one transfer: the pending
+            // scaling, the Heff stages, then the Lanczos step (alpha = v_j . w;
+            // three-term step fused with the Gram-Schmidt dots; Gram-Schmidt
+            // update fused with the norm, the kernels of tn::lanczos)
+            const ScaleJob* dsc = pend.empty() ? nullptr : stage(b, pend);
+            HeffLaunch hl = heff_batch_stage(b, sa, xin, yout);
             std::vector<DotJob> d1;
             std::vector<StepJob> sj3;
             std::vector<UpdJob> uj;
@@ -259,11 +428,20 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 sj3.push_back(StepJob{w, Vk, N, j + 1, h, j > 0 ? be[k][j - 1] : 0.0, N, h + 1});
                 uj.push_back(UpdJob{w, Vk, N, j + 1, h + 1, N, h + j + 2});
             }
+            const DotJob* dd1 = stage(b, d1);
+            const StepJob* dsj = stage(b, sj3);
+            const UpdJob* duj = stage(b, uj);
+            flush(b);
             const int na = (int)act.size();
             double* part = b->part.as<double>();
-            TN_HIP(bmdot(push(b, d1), na, 1, nmax, part, s));
-            TN_HIP(bstep_dots(push(b, sj3), na, j + 1, nmax, part, s));
-            TN_HIP(bupdate_norm(push(b, uj), na, nmax, part, s));
+            double t0 = now_ms();
+            if (dsc) TN_HIP(bscale(dsc, (int)pend.size(), pend_nmax, s));
+            if (st) TN_HIP(hipEventRecord(c->ev[0], s));
+            heff_batch_launch(b, hl);
+            if (st) TN_HIP(hipEventRecord(c->ev[1], s));
+            TN_HIP(bmdot(dd1, na, 1, nmax, part, s));
+            TN_HIP(bstep_dots(dsj, na, j + 1, nmax, part, s));
+            TN_HIP(bupdate_norm(duj, na, nmax, part, s));
             readback(b, b->hcoef.data(), coef, (size_t)ns * str);
             if (st) {
                 float ms = 0;
@@ -271,31 +449,38 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 st->heff_ms += ms;
                 st->lanczos_ms += now_ms() - t0;
             }
-            std::vector<int> next;
-            std::vector<ScaleJob> sj;
-            for (int k : act) {
+            // the tridiagonal Ritz problems of the active chains, in parallel
+            b->pool->run((int)act.size(), [&](int t) {
+                const int k = act[t];
                 const double* h = b->hcoef.data() + (size_t)k * str;
                 al[k][j] = h[0] + h[j + 1];
                 be[k][j] = std::sqrt(std::max(h[j + 2], 0.0));
                 tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], nullptr, z[k].data(), false);
                 kk[k] = j + 1;
+                resid[k] = std::fabs(be[k][j] * z[k][j]);
+                const bool done = resid[k] < tol || be[k][j] < 1e-14 * std::max(1.0, std::fabs(e0[k]));
+                fin[k] = done ? 1 : 0;
+                if (done || j + 1 == kmax)  // the second Ritz value is only reported
+                    tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], &e1[k], z[k].data(), true);
+            });
+            std::vector<int> next;
+            pend.clear();
+            pend_nmax = 1;
+            for (int k : act) {
                 if (st) {
                     st->n_matvec++;
                     const double D1 = b->ch[S[k].q]->D1[S[k].site], D2 = b->ch[S[k].q]->D2[S[k].site];
                     st->heff_flops += S[k].op.flops_alg((int)D1, (int)D2);
                 }
                 TN_REQUIRE(std::isfinite(e0[k]), TN_ERR_STATE, "batch Lanczos: non-finite Ritz value");
-                const double resid = std::fabs(be[k][j] * z[k][j]);
-                const bool done = resid < tol || be[k][j] < 1e-14 * std::max(1.0, std::fabs(e0[k]));
-                if (done || j + 1 == kmax)  // the second Ritz value is only reported
-                    tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], &e1[k], z[k].data(), true);
-                if (done) continue;
+                if (fin[k]) continue;
                 next.push_back(k);
-                if (j + 1 < kmax)
-                    sj.push_back(ScaleJob{1.0 / be[k][j], V(k) + (long)(j + 1) * S[k].N,
-                                          V(k) + (long)(j + 1) * S[k].N, S[k].N});
+                if (j + 1 < kmax) {
+                    pend.push_back(ScaleJob{1.0 / be[k][j], V(k) + (long)(j + 1) * S[k].N,
+                                            V(k) + (long)(j + 1) * S[k].N, S[k].N});
+                    pend_nmax = std::max(pend_nmax, S[k].N);
+                }
             }
-            if (!sj.empty()) TN_HIP(bscale(push(b, sj), (int)sj.size(), nmax, s));
             act.swap(next);
         }
         // Ritz vectors x = V z, normalised, for every chain of this cycle
@@ -304,7 +489,7 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             for (int k : todo)
                 for (int i = 0; i < kk[k]; i++) zh[(size_t)k * str + i] = z[k][i];
             h2d(c, coef, zh.data(), zh.size() * 8);
-            b->ring_off = 0;
+            ring_reset(b);
             std::vector<GemvJob> gj;
             std::vector<const double*> xv;
             std::vector<long> nv;
@@ -503,7 +688,7 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
     {
         std::vector<int> z(b->ch.size(), 0);
         h2d(c, b->flags.p, z.data(), z.size() * sizeof(int));
-        b->ring_off = 0;
+        ring_reset(b);
     }
     qr_pass_batch(b, P1);
     qr_pass_batch(b, P2);
@@ -523,7 +708,7 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
     std::vector<double> dev = qr_check_batch(b, P2);
     std::vector<int> flag(b->ch.size(), 0);
     d2h(c, flag.data(), b->flags.p, flag.size() * sizeof(int));
-    b->ring_off = 0;
+    ring_reset(b);
     // accepted slots: site <- Q, neighbour <- R M (left) or M L (right)
     std::vector<GemmDesc> nb0, nb1;
     std::vector<ScaleJob> cp;
@@ -725,6 +910,8 @@ int tn_batch_create(tn_ctx* c, int n, tn_chain* const* chains, int max_krylov, t
         b->ring_cap = 8u << 20;
         TN_HIP(hipHostMalloc((void**)&b->ring_h, b->ring_cap, hipHostMallocDefault));
         b->ring_d.ensure(b->ring_cap);
+        const unsigned hw = std::thread::hardware_concurrency();
+        b->pool = new Pool((int)std::min(7u, hw > 1 ? hw - 1 : 0u));
         *out = b;
         return TN_OK;
     } catch (const TnError& e) {
@@ -746,6 +933,7 @@ int tn_batch_destroy(tn_batch* b)
     b->flags.release();
     b->ring_d.release();
     if (b->ring_h) (void)hipHostFree(b->ring_h);
+    delete b->pool;
     delete b;
     return TN_OK;
 }
