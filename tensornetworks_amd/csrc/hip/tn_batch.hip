@@ -543,8 +543,16 @@ void qr_pass_batch(tn_batch* b, const std::vector<QrSlot>& P)
     gemm_group(b, gram);
     TN_HIP(bscale(push(b, zero), (int)zero.size(), zmax, s));
     // identity on the zero padding of every Gram matrix
-    for (const QrSlot& t : P)
-        if (t.nl < t.n) TN_HIP(add_diag(b->qG[t.q].as<double>(), t.n, t.nl, t.n, 1.0, s));
+    {
+        std::vector<DiagJob> dj;
+        int lenmax = 0;
+        for (const QrSlot& t : P)
+            if (t.nl < t.n) {
+                dj.push_back(DiagJob{b->qG[t.q].as<double>(), t.n, t.nl, t.n, 1.0});
+                lenmax = std::max(lenmax, t.n - t.nl);
+            }
+        if (!dj.empty()) TN_HIP(badd_diag(push(b, dj), (int)dj.size(), lenmax, s));
+    }
     int maxnb = 0;
     for (const QrSlot& t : P) maxnb = std::max(maxnb, (t.n + 63) / 64);
     for (int kb = 0; kb < maxnb; kb++) {
@@ -573,9 +581,12 @@ void qr_pass_batch(tn_batch* b, const std::vector<QrSlot>& P)
             u.C = G + (size_t)r0 * t.n + r0; u.ldc = t.n; u.alpha = -1.0f; u.accumulate = 1;
             trail.push_back(u);
         }
-        TN_HIP(chol_diag_batch(push(b, cj), (int)cj.size(), s));
-        gemm_group(b, panel);
-        gemm_group(b, trail);
+        const CholJob* dcj = stage(b, cj);
+        std::vector<GroupLaunch> lp = gemm_group_stage(b, panel), lt = gemm_group_stage(b, trail);
+        flush(b);
+        TN_HIP(chol_diag_batch(dcj, (int)cj.size(), s));
+        gemm_group_launch(b, lp);
+        gemm_group_launch(b, lt);
     }
     // L^-1 by recursive doubling of block pairs (as in cholqr_device)
     int maxn = 0;
@@ -636,9 +647,9 @@ std::vector<double> qr_check_batch(tn_batch* b, const std::vector<QrSlot>& P)
     }
     gemm_group(b, gram);
     double* out = b->coef.as<double>();
-    double* scratch = b->part.as<double>();
-    for (size_t k = 0; k < P.size(); k++)
-        TN_HIP(eye_deviation(b->qG[P[k].q].as<double>(), P[k].n, P[k].nl, out + k, scratch + (size_t)k * 1024, s));
+    std::vector<EyeJob> ej;
+    for (size_t k = 0; k < P.size(); k++) ej.push_back(EyeJob{b->qG[P[k].q].as<double>(), P[k].n, P[k].nl, out + k});
+    TN_HIP(beye_deviation(push(b, ej), (int)ej.size(), b->part.as<double>(), s));
     std::vector<double> dev(P.size());
     readback(b, dev.data(), out, P.size());
     return dev;
@@ -665,6 +676,8 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
     }
     if (bat.empty()) return;
     std::vector<QrSlot> P1, P2;
+    std::vector<Copy2Job> xcp;
+    int xro = 0, xco = 0;
     for (int k : bat) {
         tn_chain* ch = b->ch[S[k].q];
         const int q = S[k].q, i = S[k].site, d = ch->d;
@@ -675,7 +688,10 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
             t.m = D1p * d; t.n = D2p; t.nl = ch->D2[i]; t.X = ch->A[i]; t.ldx = D2p;
         } else {
             t.m = d * D2p; t.n = D1p; t.nl = ch->D1[i];
-            TN_HIP(copy2(ch->A[i], (long)d * D2p, 1, D1p, d * D2p, b->qX[q].as<double>(), 1, D1p, D1p, d * D2p, s));
+            xcp.push_back(Copy2Job{ch->A[i], (long)d * D2p, 1, D1p, d * D2p, b->qX[q].as<double>(), 1, D1p, D1p,
+                                   d * D2p});
+            xro = std::max(xro, D1p);
+            xco = std::max(xco, d * D2p);
             t.X = b->qX[q].as<double>(); t.ldx = D1p;
         }
         t.Q = b->qQ1[q].as<double>();
@@ -685,6 +701,7 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
         t2.X = t.Q; t2.ldx = t.n; t2.Q = b->qQ[q].as<double>(); t2.L = b->qLk[q].as<double>();
         P2.push_back(t2);
     }
+    if (!xcp.empty()) TN_HIP(bcopy2(push(b, xcp), (int)xcp.size(), xro, xco, s));
     {
         std::vector<int> z(b->ch.size(), 0);
         h2d(c, b->flags.p, z.data(), z.size() * sizeof(int));
@@ -712,6 +729,8 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
     // accepted slots: site <- Q, neighbour <- R M (left) or M L (right)
     std::vector<GemmDesc> nb0, nb1;
     std::vector<ScaleJob> cp;
+    std::vector<Copy2Job> qcp;
+    int qro = 0, qco = 0;
     std::vector<int> ok, resc;
     long cmax = 1;
     for (size_t t = 0; t < bat.size(); t++) {
@@ -739,7 +758,9 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
             resc.push_back(j);
         } else {
             const int D1p = P.n, D2p = pad32(ch->D2[i]);
-            TN_HIP(copy2(P.Q, 1, P.n, D1p, d * D2p, ch->A[i], (long)d * D2p, 1, D1p, d * D2p, s));
+            qcp.push_back(Copy2Job{P.Q, 1, P.n, D1p, d * D2p, ch->A[i], (long)d * D2p, 1, D1p, d * D2p});
+            qro = std::max(qro, D1p);
+            qco = std::max(qco, d * D2p);
             const int j = i - 1, D0p = pad32(ch->D1[j]);
             GemmDesc g{};
             g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
@@ -751,6 +772,7 @@ void gauge_batch(tn_batch* b, const std::vector<Slot>& S, int dir, tn_sweep_stat
         }
     }
     if (!cp.empty()) TN_HIP(bscale(push(b, cp), (int)cp.size(), cmax, s));
+    if (!qcp.empty()) TN_HIP(bcopy2(push(b, qcp), (int)qcp.size(), qro, qco, s));
     if (ok.empty()) return;
     gemm_group(b, nb0);
     gemm_group(b, nb1);

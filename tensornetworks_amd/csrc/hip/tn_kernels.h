@@ -276,6 +276,22 @@ struct EyeJob {  // out[0] = max |G - I| over the leading n x n block (row strid
 };
 // part: 64 * n doubles
 hipError_t beye_deviation(const EyeJob* jobs, int n, double* part, hipStream_t s);
+struct Copy2Job {  // copy2 of one problem (strided 2-D copy with zero fill)
+    const double* in;
+    long si0, si1;
+    int r, c;
+    double* out;
+    long so0, so1;
+    int ro, co;
+};
+hipError_t bcopy2(const Copy2Job* jobs, int n, int romax, int comax, hipStream_t s);
+struct DiagJob {  // G[i][i] += v, i0 <= i < i1
+    double* G;
+    long ld;
+    int i0, i1;
+    double v;
+};
+hipError_t badd_diag(const DiagJob* jobs, int n, int lenmax, hipStream_t s);
 hipError_t bwmix(int tout, const MixJob* jobs, int n, int Xmax, int Bnmax, hipStream_t s);
 hipError_t add_diag(double* G, long ldg, int i0, int i1, double v, hipStream_t s);
 hipError_t eye_deviation(const double* G, long ldg, int n, double* out, double* part, hipStream_t s);

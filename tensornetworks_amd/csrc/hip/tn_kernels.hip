@@ -1910,6 +1910,35 @@ hipError_t beye_deviation(const EyeJob* jobs, int n, double* part, hipStream_t s
     return hipGetLastError();
 }
 
+// strided 2-D copies, one job per problem: grid (column blocks, rows, problem)
+__global__ void __launch_bounds__(256) bcopy2_k(const Copy2Job* __restrict__ jobs)
+{
+    const Copy2Job& j = jobs[blockIdx.z];
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const int row = blockIdx.y;
+    if (col >= j.co || row >= j.ro) return;
+    j.out[row * j.so0 + col * j.so1] = (row < j.r && col < j.c) ? j.in[row * j.si0 + col * j.si1] : 0.0;
+}
+hipError_t bcopy2(const Copy2Job* jobs, int n, int romax, int comax, hipStream_t s)
+{
+    if (n <= 0 || romax <= 0 || comax <= 0) return hipSuccess;
+    bcopy2_k<<<dim3((comax + 255) / 256, romax, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) badd_diag_k(const DiagJob* __restrict__ jobs)
+{
+    const DiagJob& j = jobs[blockIdx.y];
+    const int i = j.i0 + blockIdx.x * 256 + threadIdx.x;
+    if (i < j.i1) j.G[(long)i * j.ld + i] += j.v;
+}
+hipError_t badd_diag(const DiagJob* jobs, int n, int lenmax, hipStream_t s)
+{
+    if (n <= 0 || lenmax <= 0) return hipSuccess;
+    badd_diag_k<<<dim3((lenmax + 255) / 256, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
 // w = beta w + cscale sum_i coef[i] V_i
 __global__ void __launch_bounds__(256) bgemv_k(const GemvJob* __restrict__ jobs)
 {
