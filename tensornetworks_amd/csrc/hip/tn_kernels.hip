@@ -208,6 +208,9 @@ __device__ __forceinline__ void gemm_tile(const TI* __restrict__ A, const TI* __
             tile_gload<TI, BM, BK, AKM>(ra, A, lda, m0, (t + 1) * BK, tid);
             tile_gload<TI, BN, BK, BKM>(rb, B, ldb, n0, (t + 1) * BK, tid);
         }
+        // the MFMA section of a slice at raised wave priority: a wave that has
+        // its operands in LDS issues ahead of waves still loading or storing
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += T::MK) {
             frag_t a[FM], b[FN];
@@ -222,6 +225,7 @@ __device__ __forceinline__ void gemm_tile(const TI* __restrict__ A, const TI* __
 #pragma unroll
                 for (int j = 0; j < FN; j++) acc[i][j] = mma(a[i], b[j], acc[i][j]);
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (t + 1 < nk) {
             tile_sstore<TI, BM, BK, AKM>(ra, As, tid);
