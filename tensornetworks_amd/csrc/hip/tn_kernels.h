@@ -76,15 +76,14 @@ struct MixTable {
 };
 hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx, int X,
                 int Bn, const MixTable& t, hipStream_t s);
-// The same mixing with a second input and a second output: entries e with
-// esel[e] read in2[x*isx2 + eoff[e] + b] (type tx), outputs o with osel[o]
-// write out2[x*osx2 + ooff[o] + b] (fp64).  Used when an environment channel is
-// the identity: its stage-1 slice is the input vector itself (in2) and its
+// The Heff mixing with a second input and a second output, its stage-3
+// operand written transposed: entries e with esel[e] read
+// in2[x*isx2 + eoff[e] + b] (type tx), outputs o with osel[o] write
+// out2[x*osx2 + ooff[o] + b] (fp64).  They serve an environment channel that
+// is the identity: its stage-1 slice is the input vector itself (in2) and its
 // stage-3 contribution is a copy of the mixed slice into y (out2).
-hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
-                   long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
-// The same with the stage-3 operand written transposed: outputs o with
-// osel[o] = 0 go to out[ooff[o] + b*ldo + x] (rows (m, b), columns (w, a')),
+// Outputs o with osel[o] = 0 go to out[ooff[o] + b*ldo + x] (rows (m, b),
+// columns (w, a')),
 // through an LDS tile of 32 x-values by 64 b-values so both the reads (along
 // b) and the writes (along x) are coalesced.  X must be a multiple of 32.
 hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,

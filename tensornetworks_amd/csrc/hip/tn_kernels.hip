@@ -588,36 +588,6 @@ hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx
 }
 
 template <typename TI, typename TX, typename TO>
-__global__ void __launch_bounds__(256) wmix_id_k(const TI* __restrict__ in, long isx, const TX* __restrict__ in2,
-                                                 long isx2, TO* __restrict__ out, long osx,
-                                                 double* __restrict__ out2, long osx2, int Bn, int nout,
-                                                 const int* __restrict__ optr, const long* __restrict__ ooff,
-                                                 const long* __restrict__ eoff, const int* __restrict__ esel,
-                                                 const int* __restrict__ osel, const double* __restrict__ eval,
-                                                 const float* __restrict__ evalf)
-{
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= Bn) return;
-    const long x = blockIdx.y;
-    const TI* ip = in + x * isx + b;
-    const TX* ip2 = in2 + x * isx2 + b;
-    for (int o = 0; o < nout; o++) {
-        const int e0 = optr[o], e1 = optr[o + 1];
-        if constexpr (std::is_same<TI, double>::value) {
-            double acc = 0.0;
-            for (int e = e0; e < e1; e++) acc += eval[e] * (esel[e] ? to_d(ip2[eoff[e]]) : to_d(ip[eoff[e]]));
-            if (osel[o]) out2[x * osx2 + ooff[o] + b] = acc;
-            else out[x * osx + ooff[o] + b] = cvt_from_d<TO>(acc);
-        } else {
-            float acc = 0.0f;
-            for (int e = e0; e < e1; e++) acc += evalf[e] * (esel[e] ? to_f(ip2[eoff[e]]) : to_f(ip[eoff[e]]));
-            if (osel[o]) out2[x * osx2 + ooff[o] + b] = (double)acc;
-            else out[x * osx + ooff[o] + b] = cvt_from_f<TO>(acc);
-        }
-    }
-}
-
-template <typename TI, typename TX, typename TO>
 __global__ void __launch_bounds__(256) wmix_idt_k(const TI* __restrict__ in, long isx, const TX* __restrict__ in2,
                                                   long isx2, TO* __restrict__ out, long ldo,
                                                   double* __restrict__ out2, long osx2, int Bn, int nout,
@@ -691,25 +661,6 @@ hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const v
     WT_CASE(T_F32, T_F32, T_F32, float, float, float)
     WT_CASE(T_F32, T_BF16, T_BF16, float, bf16, bf16)
 #undef WT_CASE
-    return hipErrorInvalidValue;
-}
-
-hipError_t wmix_id(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
-                   long osx, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s)
-{
-    if (!t.esel || !t.osel) return hipErrorInvalidValue;
-    dim3 grid((Bn + 255) / 256, X);
-#define WI_CASE(TIc, TXc, TOc, TIt, TXt, TOt)                                                             \
-    if (tin == TIc && tx == TXc && tout == TOc) {                                                         \
-        wmix_id_k<TIt, TXt, TOt><<<grid, 256, 0, s>>>((const TIt*)in, isx, (const TXt*)in2, isx2,         \
-                                                      (TOt*)out, osx, out2, osx2, Bn, t.nout, t.optr,     \
-                                                      t.ooff, t.eoff, t.esel, t.osel, t.eval, t.evalf);   \
-        return hipGetLastError();                                                                         \
-    }
-    WI_CASE(T_F64, T_F64, T_F64, double, double, double)
-    WI_CASE(T_F32, T_F32, T_F32, float, float, float)
-    WI_CASE(T_F32, T_BF16, T_BF16, float, bf16, bf16)
-#undef WI_CASE
     return hipErrorInvalidValue;
 }
 
