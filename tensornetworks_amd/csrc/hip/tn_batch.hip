@@ -394,54 +394,67 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             }
             TN_HIP(bscale(push(b, sj), (int)sj.size(), nmax, s));
         }
+        // Pipelined lockstep iterations.  After the read-back of iteration j
+        // the host launches iteration j+1 for every chain still active
+        // (v_{j+1} = w / beta_j with beta_j taken from the device copy of
+        // |w|^2), then solves the tridiagonal problems of iteration j while the
+        // device works.  A chain found converged at j has run one iteration
+        // more than it needs; its results are ignored.
         std::vector<int> act = todo;
-        std::vector<ScaleJob> pend;  // v_j = w / beta of the chains going on, run before the next Heff
-        long pend_nmax = 1;
         std::vector<double> resid(ns, 0.0);
         std::vector<char> fin(ns, 0);
-        for (int j = 0; j < kmax && !act.empty(); j++) {
+        auto launch_iter = [&](int j, const std::vector<int>& A) {
             std::vector<Slot> sa;
             std::vector<const double*> xin;
             std::vector<double*> yout;
+            std::vector<ScaleJob> sc;
             long nmax = 1;
-            for (int k : act) {
+            for (int k : A) {
+                const long N = S[k].N;
+                double* vj = V(k) + (long)j * N;
+                double* h = coef + (size_t)k * str;
                 sa.push_back(S[k]);
-                xin.push_back(V(k) + (long)j * S[k].N);
-                yout.push_back(V(k) + (long)(j + 1) * S[k].N);
-                nmax = std::max(nmax, S[k].N);
+                xin.push_back(vj);
+                yout.push_back(V(k) + (long)(j + 1) * N);
+                nmax = std::max(nmax, N);
+                // |w_{j-1}|^2 of iteration j-1 sits at h[j+1]
+                if (j > 0) sc.push_back(ScaleJob{0.0, vj, vj, N, h + j + 1});
             }
-            // every descriptor of the iteration in one transfer: the pending
-            // scaling, the Heff stages, then the Lanczos step (alpha = v_j . w;
-            // three-term step fused with the Gram-Schmidt dots; Gram-Schmidt
-            // update fused with the norm, the kernels of tn::lanczos)
-            const ScaleJob* dsc = pend.empty() ? nullptr : stage(b, pend);
+            // every descriptor of the iteration in one transfer: the scaling,
+            // the Heff stages, then the Lanczos step (alpha = v_j . w; three-term
+            // step fused with the Gram-Schmidt dots; Gram-Schmidt update fused
+            // with the norm, the kernels of tn::lanczos)
+            const ScaleJob* dsc = sc.empty() ? nullptr : stage(b, sc);
             HeffLaunch hl = heff_batch_stage(b, sa, xin, yout);
             std::vector<DotJob> d1;
             std::vector<StepJob> sj3;
             std::vector<UpdJob> uj;
-            for (int k : act) {
+            for (int k : A) {
                 double* Vk = V(k);
                 const long N = S[k].N;
                 double* w = Vk + (long)(j + 1) * N;
                 double* h = coef + (size_t)k * str;
                 d1.push_back(DotJob{Vk + (long)j * N, N, 1, w, N, h});
-                sj3.push_back(StepJob{w, Vk, N, j + 1, h, j > 0 ? be[k][j - 1] : 0.0, N, h + 1});
+                sj3.push_back(StepJob{w, Vk, N, j + 1, h, 0.0, N, h + 1, j > 0 ? h + j + 1 : nullptr});
                 uj.push_back(UpdJob{w, Vk, N, j + 1, h + 1, N, h + j + 2});
             }
             const DotJob* dd1 = stage(b, d1);
             const StepJob* dsj = stage(b, sj3);
             const UpdJob* duj = stage(b, uj);
             flush(b);
-            const int na = (int)act.size();
+            const int na = (int)A.size();
             double* part = b->part.as<double>();
-            double t0 = now_ms();
-            if (dsc) TN_HIP(bscale(dsc, (int)pend.size(), pend_nmax, s));
+            if (dsc) TN_HIP(bscale(dsc, (int)sc.size(), nmax, s));
             if (st) TN_HIP(hipEventRecord(c->ev[0], s));
             heff_batch_launch(b, hl);
             if (st) TN_HIP(hipEventRecord(c->ev[1], s));
             TN_HIP(bmdot(dd1, na, 1, nmax, part, s));
             TN_HIP(bstep_dots(dsj, na, j + 1, nmax, part, s));
             TN_HIP(bupdate_norm(duj, na, nmax, part, s));
+        };
+        double t0 = now_ms();
+        if (kmax > 0 && !act.empty()) launch_iter(0, act);
+        for (int j = 0; j < kmax && !act.empty(); j++) {
             readback(b, b->hcoef.data(), coef, (size_t)ns * str);
             if (st) {
                 float ms = 0;
@@ -449,7 +462,10 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 st->heff_ms += ms;
                 st->lanczos_ms += now_ms() - t0;
             }
+            t0 = now_ms();
+            if (j + 1 < kmax) launch_iter(j + 1, act);
             // the tridiagonal Ritz problems of the active chains, in parallel
+            // with the device and on the host threads
             b->pool->run((int)act.size(), [&](int t) {
                 const int k = act[t];
                 const double* h = b->hcoef.data() + (size_t)k * str;
@@ -464,8 +480,6 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                     tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], &e1[k], z[k].data(), true);
             });
             std::vector<int> next;
-            pend.clear();
-            pend_nmax = 1;
             for (int k : act) {
                 if (st) {
                     st->n_matvec++;
@@ -473,13 +487,7 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                     st->heff_flops += S[k].op.flops_alg((int)D1, (int)D2);
                 }
                 TN_REQUIRE(std::isfinite(e0[k]), TN_ERR_STATE, "batch Lanczos: non-finite Ritz value");
-                if (fin[k]) continue;
-                next.push_back(k);
-                if (j + 1 < kmax) {
-                    pend.push_back(ScaleJob{1.0 / be[k][j], V(k) + (long)(j + 1) * S[k].N,
-                                            V(k) + (long)(j + 1) * S[k].N, S[k].N});
-                    pend_nmax = std::max(pend_nmax, S[k].N);
-                }
+                if (!fin[k]) next.push_back(k);
             }
             act.swap(next);
         }

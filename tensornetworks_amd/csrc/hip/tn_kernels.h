@@ -218,11 +218,12 @@ struct ThreeJob {  // w -= alpha[0] v + beta u
     long n;
 };
 hipError_t bthree(const ThreeJob* jobs, int n, long nmax, hipStream_t s);
-struct ScaleJob {  // y = a x
+struct ScaleJob {  // y = a x; with a_sq set, a = 1 / sqrt(*a_sq) read on the device
     double a;
     const double* x;
     double* y;
     long n;
+    const double* a_sq;
 };
 hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s);
 struct ConvJob {  // y = (float) x
@@ -246,8 +247,8 @@ struct MixJob {  // FP32 in; with t.esel/t.osel set, the identity-channel form o
 // Fused Lanczos step for many problems (n even for every job), the batched
 // form of lz_step_dots / lz_update_norm.  part: bmdot_part_size(n, kmax, nmax)
 // doubles for bstep_dots, n * ceil(nmax / 2048) for bupdate_norm.
-struct StepJob {  // w -= alpha[0] V_{k-1} + beta V_{k-2} (k >= 2), then out[i] = V_i . w, i < k
-    double* w;
+struct StepJob {  // w -= alpha[0] V_{k-1} + beta V_{k-2} (k >= 2), then out[i] = V_i . w, i < k;
+    double* w;      // with beta_sq set, beta = sqrt(*beta_sq) read on the device
     const double* V;
     long ldv;
     int k;
@@ -255,6 +256,7 @@ struct StepJob {  // w -= alpha[0] V_{k-1} + beta V_{k-2} (k >= 2), then out[i] 
     double beta;
     long n;
     double* out;
+    const double* beta_sq;
 };
 hipError_t bstep_dots(const StepJob* jobs, int n, int kmax, long nmax, double* part, hipStream_t s);
 struct UpdJob {  // w -= sum_{i<k} coef[i] V_i, then out[0] = |w|^2

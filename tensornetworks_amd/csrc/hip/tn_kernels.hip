@@ -1689,6 +1689,7 @@ __global__ void __launch_bounds__(BV_BLOCK) bstep_dots_k(const StepJob* __restri
     const long c0 = (long)blockIdx.x * BV_CHUNK;
     if (c0 >= j.n) return;  // the finalising sum reads only this job's chunks
     const double a = j.alpha[0];
+    const double beta = j.beta_sq ? sqrt(fmax(*j.beta_sq, 0.0)) : j.beta;
     const double* vj = j.V + (long)(j.k - 1) * j.ldv;
     const double* vm = j.k >= 2 ? j.V + (long)(j.k - 2) * j.ldv : nullptr;
     double2 wr[BV_PAIRS];
@@ -1702,8 +1703,8 @@ __global__ void __launch_bounds__(BV_BLOCK) bstep_dots_k(const StepJob* __restri
             x.y -= a * p.y;
             if (vm) {
                 const double2 q = *(const double2*)(vm + idx);
-                x.x -= j.beta * q.x;
-                x.y -= j.beta * q.y;
+                x.x -= beta * q.x;
+                x.y -= beta * q.y;
             }
             *(double2*)(j.w + idx) = x;
             wr[e] = x;
@@ -1940,7 +1941,8 @@ hipError_t bthree(const ThreeJob* jobs, int n, long nmax, hipStream_t s)
 __global__ void __launch_bounds__(256) bscale_k(const ScaleJob* __restrict__ jobs)
 {
     const ScaleJob& j = jobs[blockIdx.y];
-    for (long i = blockIdx.x * 256L + threadIdx.x; i < j.n; i += (long)gridDim.x * 256) j.y[i] = j.a * j.x[i];
+    const double a = j.a_sq ? 1.0 / sqrt(*j.a_sq) : j.a;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < j.n; i += (long)gridDim.x * 256) j.y[i] = a * j.x[i];
 }
 hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s)
 {
