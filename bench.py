@@ -187,8 +187,9 @@ def run_precond(a, tn, ctx):
     backward / Adam kernels), then one sweep of the S=1/2 Heisenberg chain
     (L=128, D=1024 by default; --L/--D override) from the same warm state with
     each local solver: Lanczos, Davidson without / with the diagonal / with
-    the learned correction.  Reported: matvecs (Krylov iterations), sweep time
-    and |dE|/L against the Lanczos sweep."""
+    the learned correction at every step / with the learned correction on the
+    first step only.  Reported: matvecs (Krylov iterations), the solves that
+    fell back to Lanczos, sweep time and |dE|/L against the Lanczos sweep."""
     from tensornetworks_amd import precond
     from tensornetworks_amd.models import xxz_mpo
     t0 = time.perf_counter()
@@ -215,7 +216,7 @@ def run_precond(a, tn, ctx):
     saved = [(ch.dims()[0][i], ch.dims()[1][i], ch.site(i)) for i in range(L)]
     res = {}
     for name, method, kind in [("lanczos", 0, 0), ("davidson_none", 1, 0), ("davidson_diag", 1, 1),
-                               ("davidson_learned", 1, 2)]:
+                               ("davidson_learned", 1, 2), ("davidson_learned_first", 1, 3)]:
         o = tn.SweepOpts.default(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=3, eig_method=method,
                                  precond=kind, **base)
         for i, (d1, d2, M) in enumerate(saved):
@@ -225,16 +226,18 @@ def run_precond(a, tn, ctx):
         t1 = time.perf_counter()
         ch.sweep(0, o, st)
         ch.sweep(1, o, st)
-        res[name] = {"n_matvec": st.n_matvec, "sweep_s": time.perf_counter() - t1, "E_per_site": ch.expectation() / L}
+        res[name] = {"n_matvec": st.n_matvec, "sweep_s": time.perf_counter() - t1, "E_per_site": ch.expectation() / L,
+                     "n_lanczos_fallback": st.n_precond_fallback}
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
     e_ref = res["lanczos"]["E_per_site"]
     for r in res.values():
         r["dE_per_site_vs_lanczos"] = abs(r["E_per_site"] - e_ref)
     out = {
         "metric": "Krylov matvecs per sweep and |dE|/L of the learned Davidson preconditioner vs plain Lanczos",
-        "value": res["lanczos"]["n_matvec"] / max(res["davidson_learned"]["n_matvec"], 1),
-        "unit": "x fewer matvecs than Lanczos (learned correction)",
-        "n_gpus": 1, "steps": 1, "warmup": max(a.warmup, 1), "ms_per_step": res["davidson_learned"]["sweep_s"] * 1e3,
+        "value": res["lanczos"]["n_matvec"] / max(res["davidson_learned_first"]["n_matvec"], 1),
+        "unit": "x fewer matvecs than Lanczos (learned first correction, diagonal after)",
+        "n_gpus": 1, "steps": 1, "warmup": max(a.warmup, 1),
+        "ms_per_step": res["davidson_learned_first"]["sweep_s"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
         "data": "synthetic (training chains from a seed; evaluation from a seeded random MPS)",
         "config": {"workload": f"S=1/2 Heisenberg open chain L={L}, D={D}, one sweep per local solver from the same "

@@ -138,7 +138,11 @@ typedef struct tn_sweep_opts {
                             max_krylov*(1+max_restarts) iterations), an alternative to
                             the reference's SparseEigenSolver back ends */
     int precond;         /* Davidson correction: 0 none, 1 diagonal (Heff diagonal),
-                            2 learned (network set by tn_ctx_set_mlp) */
+                            2 learned (network set by tn_ctx_set_mlp) at every step,
+                            3 learned on the first correction from the site tensor and
+                            diagonal after; with 2 and 3 Davidson gets max_krylov
+                            iterations, then plain Lanczos continues from its Ritz
+                            vector if it has not converged */
     double precond_delta;  /* safeguard of the diagonal shift |diag - theta| >= delta */
     int skip_identity;   /* 1 = an environment channel equal to the identity (the
                             "nothing placed yet" / "all placed" MPO channel of an
@@ -168,6 +172,7 @@ typedef struct tn_sweep_stats {
     int64_t n_qr_shifted;      /* QR gauge moves whose first pass needed the shift */
     int64_t n_ident_left;      /* local solves whose left environment had an identity channel */
     int64_t n_ident_right;     /* ... right environment */
+    int64_t n_precond_fallback;  /* learned-correction Davidson solves finished by Lanczos */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);

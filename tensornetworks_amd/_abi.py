@@ -71,6 +71,7 @@ class SweepStats(ctypes.Structure):
         ("n_qr_shifted", ctypes.c_int64),
         ("n_ident_left", ctypes.c_int64),
         ("n_ident_right", ctypes.c_int64),
+        ("n_precond_fallback", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -205,7 +206,7 @@ class Context:
 
     # -- learned Davidson preconditioner ---------------------------------------
     def set_mlp(self, params, norm):
-        """Network used by SweepOpts(eig_method=1, precond=2) on this context."""
+        """Network used by SweepOpts(eig_method=1, precond=2 or 3) on this context."""
         p = np.ascontiguousarray(params, dtype=np.float32)
         nm = np.ascontiguousarray(norm, dtype=np.float32)
         self.check(lib.tn_ctx_set_mlp(self.h, FP(p), FP(nm)), "tn_ctx_set_mlp")

@@ -270,16 +270,25 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     // rounding (~2^-8 for BF16, ~2^-23 for FP32 operands, relative to |theta|)
     // when a refinement follows, stop there and let the refinement finish
     if (o.eig_method == 1) {
+        TN_REQUIRE(o.precond >= 0 && o.precond <= 3, TN_ERR_ARG, "Davidson: precond must be 0..3");
         const double* Wd = w_diag(ch, i);
+        // a learned correction gets one basis (no restart); when that does not
+        // converge, plain Lanczos continues from its Ritz vector
+        const bool learned = o.precond >= 2;
+        const int cap = learned ? o.max_krylov : o.max_krylov * (1 + o.max_restarts);
         DavidsonResult dr = davidson(c, heff_cb, &hc, make_heff(ch, i, TN_FP64, o.gemm_tile, wl, wr), ch->D1[i],
-                                     ch->D2[i],
-                                     Wd, ch->A[i],
-                                     o.eig_tol, o.max_krylov, o.max_krylov * (1 + o.max_restarts), o.precond,
+                                     ch->D2[i], Wd, ch->A[i], o.eig_tol, o.max_krylov, cap, o.precond,
                                      o.precond_delta);
-        heff_timing_flush(c);
-        if (st) st->lanczos_ms += now_ms() - t0;
         *e0 = dr.e0;
         *e1 = dr.e1;
+        if (learned && !dr.converged) {
+            if (st) st->n_precond_fallback++;
+            LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts);
+            *e0 = r.e0;
+            *e1 = r.e1;
+        }
+        heff_timing_flush(c);
+        if (st) st->lanczos_ms += now_ms() - t0;
         return;
     }
     const bool refining = o.refine && o.refine_precision < o.precision;

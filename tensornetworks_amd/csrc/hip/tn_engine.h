@@ -181,7 +181,8 @@ struct DavidsonResult {
 // geo: the operator's shapes and FP64 environments (for the Heff diagonal),
 // D1 x D2 the logical bond dimensions;
 // Wd: device [Dw1][Dw2][p] diagonal of the W blocks; kind 0 none, 1 diagonal,
-// 2 network (ctx->mlp); converged when |H u - theta u| < tol.
+// 2 network (ctx->mlp), 3 network on the first correction and diagonal after;
+// converged when |H u - theta u| < tol.
 DavidsonResult davidson(tn_ctx* c, ApplyFn fn, const void* op, const HeffOp& geo, int D1, int D2, const double* Wd,
                         double* x, double tol, int mmax, int max_iter, int kind, double delta);
 void heff_diag(tn_ctx* c, const HeffOp& op, const double* Wd, double* diag);

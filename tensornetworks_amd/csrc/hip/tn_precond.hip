@@ -461,11 +461,11 @@ DavidsonResult davidson(tn_ctx* c, ApplyFn fn, const void* op_, const HeffOp& ge
     const long pcap = mdot_part_size(N, mmax + 1);
     c->part.ensure(pcap * 8);
     double* part = c->part.as<double>();
-    if (kind == 2) TN_REQUIRE(c->mlp.p != nullptr, TN_ERR_STATE, "Davidson: no preconditioner network loaded");
+    if (kind >= 2) TN_REQUIRE(c->mlp.p != nullptr, TN_ERR_STATE, "Davidson: no preconditioner network loaded");
     double dmu = 0.0, dsd = 1.0;
     if (kind >= 1) {
         heff_diag(c, geo, Wd, diag);
-        if (kind == 2) {
+        if (kind >= 2) {
             double st[2];
             diag_stats_k<<<1, 256, 0, s>>>(diag, geo.p, D1, D2, geo.D2p, h);
             TN_HIP(hipGetLastError());
@@ -521,11 +521,14 @@ DavidsonResult davidson(tn_ctx* c, ApplyFn fn, const void* op_, const HeffOp& ge
             m = 1;
         }
         double* t = V + (long)m * N;
-        if (kind == 0) {
+        // kind 3: the network on the first correction from the start vector
+        // (the step its training samples come from), the diagonal one after
+        const int ks = kind == 3 ? (it == 0 ? 2 : 1) : kind;
+        if (ks == 0) {
             TN_HIP(scale_copy(1.0, rt, t, N, s));
         } else {
-            if (kind == 2) bond_weights(c, u, geo.p, geo.D1p, geo.D2p, laml, lamr);
-            precond_apply(c, kind, rt, diag, res.e0, (res.e0 - dmu) / dsd, delta, laml, lamr, geo.p, geo.D2p, N, t);
+            if (ks == 2) bond_weights(c, u, geo.p, geo.D1p, geo.D2p, laml, lamr);
+            precond_apply(c, ks, rt, diag, res.e0, (res.e0 - dmu) / dsd, delta, laml, lamr, geo.p, geo.D2p, N, t);
         }
         for (int pass = 0; pass < 2; pass++) {
             TN_HIP(mdot(V, N, m, t, N, h, part, (int)pcap, s));
