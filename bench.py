@@ -46,6 +46,10 @@ def parse():
     p.add_argument("--max-krylov", type=int, default=24)
     p.add_argument("--max-restarts", type=int, default=0)
     p.add_argument("--gauge", default="qr", choices=["qr", "svd"])
+    p.add_argument("--reorth", default="partial", choices=["partial", "full"],
+                   help="Lanczos re-orthogonalisation: partial (Simon's omega recurrence, a pass against the "
+                        "whole basis only when needed) or full (every iteration); the FP64 comparison sweep "
+                        "uses the same mode")
     p.add_argument("--no-skip-identity", action="store_true",
                    help="contract identity environment channels like the others instead of copying them")
     p.add_argument("--no-fp64-compare", action="store_true")
@@ -294,7 +298,7 @@ def main():
     else:
         ch.normalize(1)
     common = dict(eig_tol=a.eig_tol, max_krylov=a.max_krylov, max_restarts=a.max_restarts, gauge=gauge,
-                  skip_identity=0 if a.no_skip_identity else 1)
+                  skip_identity=0 if a.no_skip_identity else 1, reorth=1 if a.reorth == "partial" else 0)
     if a.precision == "mixed":
         o_mixed = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
                                        refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
@@ -339,8 +343,8 @@ def main():
     # ---------------------------------------------------------
     E_mixed = ch.expectation()
     print(f"# timed {a.steps} sweep(s) {wall:.2f}s E/L={E_mixed / a.L:.12f} matvecs={st.n_matvec} "
-          f"(fp64 {st.n_matvec_fp64}) qr={st.n_qr} shifted={st.n_qr_shifted} fallback={st.n_qr_fallback}",
-          file=sys.stderr, flush=True)
+          f"(fp64 {st.n_matvec_fp64}) full-basis passes={st.n_full_reorth} qr={st.n_qr} "
+          f"shifted={st.n_qr_shifted} fallback={st.n_qr_fallback}", file=sys.stderr, flush=True)
 
     def timed_from_saved(opts, tune=True):
         """K sweeps from the saved state (after one untimed sweep that tunes
@@ -382,7 +386,8 @@ def main():
         st64, wall64, E64 = timed_from_saved(o_fp64)
         res_fp64 = {"sweep_s": wall64 / a.steps, "E_per_site": E64 / a.L,
                     "heff_tflops": st64.heff_flops / max(st64.heff_ms, 1e-12) / 1e9,
-                    "n_matvec": st64.n_matvec, "dE_per_site": abs(E_mixed - E64) / a.L,
+                    "n_matvec": st64.n_matvec, "n_full_reorth": st64.n_full_reorth,
+                    "dE_per_site": abs(E_mixed - E64) / a.L,
                     "gauge_ms": st64.svd_ms, "env_ms": st64.env_ms, "lanczos_ms": st64.lanczos_ms,
                     "heff_ms": st64.heff_ms, "n_qr_fallback": st64.n_qr_fallback,
                     "ms_per_matvec": st64.heff_ms / max(st64.n_matvec, 1)}
@@ -436,7 +441,7 @@ def main():
                    "precision": a.precision,
                    "L": a.L, "D": a.D, "d": d, "Dw": 5, "eig_tol": a.eig_tol, "max_krylov": a.max_krylov,
                    "max_restarts": a.max_restarts, "gauge": a.gauge, "autotune": not a.no_autotune,
-                   "skip_identity": not a.no_skip_identity},
+                   "skip_identity": not a.no_skip_identity, "reorth": a.reorth},
         "sweep_s": wall / a.steps,
         "E_per_site": E_mixed / a.L,
         "breakdown_ms": {"lanczos": st.lanczos_ms, "env": st.env_ms, "gauge": st.svd_ms, "wall": st.wall_ms},
@@ -445,7 +450,7 @@ def main():
                                "note": "the timed sweeps again from the same state with HIP events around the "
                                        "Heff stages; value and ms_per_matvec come from it"},
         "n_qr": st.n_qr, "n_qr_fallback": st.n_qr_fallback, "n_qr_shifted": st.n_qr_shifted,
-        "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64,
+        "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64, "n_full_reorth": st.n_full_reorth,
         "heff_gemm_tflops_launched": sti.heff_gemm_flops / max(sti.heff_gemm_ms, 1e-12) / 1e9,
         "n_ident_left": st.n_ident_left, "n_ident_right": st.n_ident_right,
         "ms_per_matvec": sti.heff_ms / max(sti.n_matvec, 1),

@@ -152,6 +152,11 @@ typedef struct tn_sweep_opts {
     int time_heff;       /* 1 = HIP events around the stages of every Heff apply
                             (tn_sweep_stats.heff_ms / heff_gemm_ms); 0 = counters only,
                             no events in the stream */
+    int reorth;          /* Lanczos re-orthogonalisation: 0 = against the whole Krylov
+                            basis every iteration; 1 = partial (Simon's omega recurrence):
+                            against the last two vectors every iteration and against the
+                            whole basis only when the estimated loss of orthogonality
+                            exceeds sqrt(u), u the unit roundoff of the Heff precision */
 } tn_sweep_opts;
 
 typedef struct tn_sweep_stats {
@@ -173,6 +178,8 @@ typedef struct tn_sweep_stats {
     int64_t n_ident_left;      /* local solves whose left environment had an identity channel */
     int64_t n_ident_right;     /* ... right environment */
     int64_t n_precond_fallback;  /* learned-correction Davidson solves finished by Lanczos */
+    int64_t n_full_reorth;     /* Lanczos iterations with a Gram-Schmidt pass against the whole
+                                  Krylov basis (every iteration with reorth = 0) */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);

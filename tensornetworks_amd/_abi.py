@@ -39,6 +39,7 @@ class SweepOpts(ctypes.Structure):
         ("precond_delta", ctypes.c_double),
         ("skip_identity", ctypes.c_int),
         ("time_heff", ctypes.c_int),
+        ("reorth", ctypes.c_int),
     ]
 
     @classmethod
@@ -72,6 +73,7 @@ class SweepStats(ctypes.Structure):
         ("n_ident_left", ctypes.c_int64),
         ("n_ident_right", ctypes.c_int64),
         ("n_precond_fallback", ctypes.c_int64),
+        ("n_full_reorth", ctypes.c_int64),
     ]
 
     def as_dict(self):

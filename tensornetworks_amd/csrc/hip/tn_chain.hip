@@ -248,6 +248,12 @@ const double* w_diag(tn_chain* ch, int i)
     return c->dav_wd.as<double>();
 }
 
+// unit roundoff of the Heff operands: the relative rounding of an apply
+double unit_roundoff(int precision)
+{
+    return precision == TN_FP64 ? 0x1p-53 : precision == TN_FP32 ? 0x1p-24 : 0x1p-8;
+}
+
 // local eigensolve of site i; returns (e0, e1)
 void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1)
 {
@@ -283,7 +289,9 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
         *e1 = dr.e1;
         if (learned && !dr.converged) {
             if (st) st->n_precond_fallback++;
-            LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts);
+            LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, 0.0,
+                                      o.reorth == 1, unit_roundoff(TN_FP64));
+            if (st) st->n_full_reorth += r.n_full;
             *e0 = r.e0;
             *e1 = r.e1;
         }
@@ -293,12 +301,16 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     }
     const bool refining = o.refine && o.refine_precision < o.precision;
     const double rel = refining ? (o.precision == TN_BF16 ? 4e-3 : 4e-7) : 0.0;
-    LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel);
+    LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel,
+                              o.reorth == 1, unit_roundoff(o.precision));
+    if (st) st->n_full_reorth += r.n_full;
     if (refining) {
         // polish the reduced-precision Ritz vector with a more accurate operator
         HeffCall hr = hc;
         hr.op = make_heff(ch, i, o.refine_precision, o.gemm_tile, wl, wr);
-        LanczosResult r2 = lanczos(c, heff_cb, &hr, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts);
+        LanczosResult r2 = lanczos(c, heff_cb, &hr, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts, 0.0,
+                                   o.reorth == 1, unit_roundoff(o.refine_precision));
+        if (st) st->n_full_reorth += r2.n_full;
         r.e0 = r2.e0;
         r.e1 = r2.e1;
     }
@@ -747,6 +759,7 @@ void tn_sweep_opts_default(tn_sweep_opts* o)
     o->precond_delta = 1e-2;
     o->skip_identity = 1;
     o->time_heff = 1;
+    o->reorth = 0;
 }
 
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded, double* rescale)
@@ -1389,13 +1402,17 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.D1 = D1;
         hc.D2 = D3;
         double tl = now_ms();
-        LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->theta.as<double>(), o.eig_tol, o.max_krylov, o.max_restarts);
+        LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->theta.as<double>(), o.eig_tol, o.max_krylov, o.max_restarts,
+                                  0.0, o.reorth == 1, unit_roundoff(o.precision));
+        if (st) st->n_full_reorth += r.n_full;
         if (o.refine && o.refine_precision < o.precision) {
             HeffCall h64 = hc;
             h64.op.precision = o.refine_precision;
             h64.op.Llo = lo_left(ch, o.refine_precision, i);
             h64.op.Rlo = lo_right(ch, o.refine_precision, j);
-            LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->theta.as<double>(), o.refine_tol, o.max_krylov, o.max_restarts);
+            LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->theta.as<double>(), o.refine_tol, o.max_krylov,
+                                       o.max_restarts, 0.0, o.reorth == 1, unit_roundoff(o.refine_precision));
+            if (st) st->n_full_reorth += r2.n_full;
             r.e0 = r2.e0;
             r.e1 = r2.e1;
         }

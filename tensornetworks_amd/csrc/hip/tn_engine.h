@@ -161,6 +161,7 @@ void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n);
 struct LanczosResult {
     double e0 = 0, e1 = 0, residual = 0;
     int n_matvec = 0;
+    int n_full = 0;  // iterations with a Gram-Schmidt pass against the whole basis
     bool converged = false;
 };
 // lowest two eigenvalues of tridiag(b, a, b) (k x k) and the eigenvector z of the lowest
@@ -169,8 +170,14 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
 // generic operator callback: y = A x (device fp64)
 typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
 // converged when |H x - theta x| < tol or < tol_rel*|theta| (tol_rel = 0: off)
+// partial = true: local re-orthogonalisation against the last two Krylov
+// vectors every iteration, and a pass against the whole basis only when the
+// estimated loss of orthogonality exceeds sqrt(eps_op), eps_op the relative
+// rounding of the operator (Simon's omega recurrence); false: the pass
+// against the whole basis every iteration
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
-                      int max_krylov, int max_restarts, double tol_rel = 0.0);
+                      int max_krylov, int max_restarts, double tol_rel = 0.0, bool partial = false,
+                      double eps_op = 0x1p-52);
 
 // --- Davidson with a diagonal / learned preconditioner (tn_precond.hip) ---
 struct DavidsonResult {

@@ -640,8 +640,19 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
 // per step).  Krylov vectors, Ritz vector and residual stay in FP64 on the
 // device; per iteration the host reads the coefficients and the norm (one
 // sync) and solves the tridiagonal problem itself.
+//
+// Partial re-orthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115-142):
+// each step removes the components along v_j and v_{j-1} only, and the host
+// propagates estimates omega_{j+1,k} of v_{j+1}.v_k through the recurrence
+//   beta_j w'_k = beta_k w_{k+1} + (alpha_k - alpha_j) w_k + beta_{k-1} w_{k-1}
+//                 - beta_{j-1} omega_{j-1,k}  (+ rounding, eps_op |T|),
+// with w = omega_{j,.}.  When an estimate exceeds sqrt(eps_op) the new vector
+// gets a Gram-Schmidt pass against the whole basis, and so does the next one
+// (the recurrence couples consecutive vectors).  Semi-orthogonality at
+// sqrt(eps_op) keeps the tridiagonal matrix the projection of the operator to
+// O(eps_op |H|), the operator's own accuracy.
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
-                      int max_krylov, int max_restarts, double tol_rel)
+                      int max_krylov, int max_restarts, double tol_rel, bool partial, double eps_op)
 {
     hipStream_t s = c->stream;
     LanczosResult res;
@@ -658,6 +669,9 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
     std::vector<double> al(kmax), be(kmax), z(kmax), hh(kmax + 4);
     // fused step kernels: 16-byte vector access needs an even length
     const bool fused = N % 2 == 0 && kmax <= 512;
+    const bool pro = partial && fused;
+    const double eta = std::sqrt(eps_op);
+    std::vector<double> om_old(kmax + 2, 0.0), om(kmax + 2, 0.0), om_new(kmax + 2, 0.0);
 
     auto read = [&](double* dst, const double* src, int n) {
         if (n <= TN_PUBLISH_MAX) d2h_small(c, dst, src, n);
@@ -690,6 +704,9 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
         TN_HIP(scale_copy(1.0 / nx, x, V, N, s));
         int k = 0;
         bool conv = false;
+        bool force = false;     // the previous vector had a full pass: this one gets one too
+        double tnorm = 0.0;     // Gershgorin bound on |T|
+        om[0] = 1.0;
         for (int j = 0; j < kmax; j++) {
             double* vj = V + (long)j * N;
             double* w = V + (long)(j + 1) * N;
@@ -699,7 +716,44 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
             // then one classical Gram-Schmidt pass against every Krylov vector
             // (h[1..j+1]) removes what rounding left; h[j+2] = |w|^2
             TN_HIP(mdot(vj, N, 1, w, N, h, part, (int)pcap, s));
-            if (fused) {
+            if (pro) {
+                // the same step and pass against v_{j-1}, v_j only
+                const int kk = j >= 1 ? 2 : 1;
+                const double* Vl = V + (long)(j + 1 - kk) * N;
+                TN_HIP(lz_step_dots(w, Vl, N, kk, h, j > 0 ? be[j - 1] : 0.0, N, h + 1, part, pcap, s));
+                TN_HIP(lz_update_norm(w, Vl, N, kk, h + 1, N, h + kk + 1, part, pcap, s));
+                read(hh.data(), h, kk + 2);
+                al[j] = hh[0] + hh[kk];
+                be[j] = std::sqrt(std::max(hh[kk + 1], 0.0));
+                tnorm = std::max(tnorm, std::fabs(al[j]) + be[j] + (j > 0 ? be[j - 1] : 0.0));
+                const double noise = eps_op * tnorm;
+                double wmax = 0.0;
+                if (be[j] > 0.0) {
+                    for (int q = 0; q + 1 < j; q++) {
+                        double t = be[q] * om[q + 1] + (al[q] - al[j]) * om[q] - be[j - 1] * om_old[q];
+                        if (q > 0) t += be[q - 1] * om[q - 1];
+                        t = (t + std::copysign(noise, t)) / be[j];
+                        om_new[q] = t;
+                        wmax = std::max(wmax, std::fabs(t));
+                    }
+                    for (int q = std::max(j - 1, 0); q <= j; q++) om_new[q] = noise / be[j];
+                }
+                om_new[j + 1] = 1.0;
+                if (j > 0 && (force || wmax > eta || !std::isfinite(wmax))) {
+                    TN_HIP(mdot(V, N, j + 1, w, N, h + 1, part, (int)pcap, s));
+                    TN_HIP(lz_update_norm(w, V, N, j + 1, h + 1, N, h + j + 2, part, pcap, s));
+                    read(hh.data(), h, j + 3);
+                    al[j] += hh[j + 1];
+                    be[j] = std::sqrt(std::max(hh[j + 2], 0.0));
+                    for (int q = 0; q <= j; q++) om_new[q] = be[j] > 0.0 ? noise / be[j] : 0.0;
+                    force = !force;
+                    res.n_full++;
+                } else {
+                    force = false;
+                }
+                std::swap(om_old, om);
+                std::swap(om, om_new);
+            } else if (fused) {
                 // three-term step fused with the Gram-Schmidt dot products, then
                 // the Gram-Schmidt update fused with the norm
                 TN_HIP(lz_step_dots(w, V, N, j + 1, h, j > 0 ? be[j - 1] : 0.0, N, h + 1, part, pcap, s));
@@ -711,11 +765,12 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
                 TN_HIP(gemv_acc(V, N, j + 1, h + 1, -1.0, 1.0, w, N, s));
                 TN_HIP(mdot(w, N, 1, w, N, h + j + 2, part, (int)pcap, s));
             }
-            read(hh.data(), h, j + 3);
-            const double alpha = hh[0] + hh[j + 1];
-            const double nw2 = hh[j + 2];
-            al[j] = alpha;
-            be[j] = std::sqrt(std::max(nw2, 0.0));
+            if (!pro) {
+                read(hh.data(), h, j + 3);
+                al[j] = hh[0] + hh[j + 1];
+                be[j] = std::sqrt(std::max(hh[j + 2], 0.0));
+                res.n_full++;
+            }
             tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, nullptr, z.data(), false);
             k = j + 1;
             res.residual = std::fabs(be[j] * z[j]);

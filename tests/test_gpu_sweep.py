@@ -143,3 +143,60 @@ def test_identity_channels_same_sweeps(ctx, prec, tol):
     solves = 6 * L
     assert stats[1].n_ident_left >= solves - 12 and stats[1].n_ident_right >= solves - 12, \
         (stats[1].n_ident_left, stats[1].n_ident_right)
+
+
+def test_partial_reorth_matches_reference_value(ctx):
+    # PyTensorNetworks/test_Unittest1.py:22 with partial re-orthogonalisation
+    st = tn.SweepStats()
+    o = tn.SweepOpts.default(eig_tol=1e-12, reorth=1)
+    ch = tn.Chain(ctx, xxz_mpo(10, 0.5), 8)
+    ch.initialize(1, 20261202)
+    ch.normalize(1)
+    ch.load_caches(o)
+    for _ in range(10):
+        ch.sweep(0, o, st)
+        ch.sweep(1, o, st)
+    assert abs(ch.expectation() / 9 - (-0.473113399629092)) < 1e-11
+    assert st.n_full_reorth < st.n_matvec
+
+
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-12), (tn.TN_FP32, 1e-6)])
+def test_partial_reorth_same_sweeps(ctx, prec, tol):
+    # semi-orthogonality at sqrt(unit roundoff) of the operator gives the
+    # sweeps of full re-orthogonalisation (FP64: to rounding; FP32: to the
+    # FP32 operator's accuracy) with far fewer passes against the whole basis
+    L, D = 16, 48
+    energies, stats = [], []
+    for reorth in (0, 1):
+        ch = tn.Chain(ctx, xxz_mpo(L, 1.0, Dz=0.1), D)
+        ch.initialize(1, 99)
+        ch.normalize_qr(1)
+        o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1, precision=prec, env_precision=prec, refine=0,
+                                 max_krylov=40, reorth=reorth)
+        ch.load_caches(o)
+        st = tn.SweepStats()
+        for _ in range(3):
+            ch.sweep(0, o, st)
+            ch.sweep(1, o, st)
+        energies.append(ch.expectation())
+        stats.append(st)
+        ch.close()
+    assert abs(energies[1] - energies[0]) < tol * abs(energies[0]), energies
+    # every Lanczos iteration of the full mode has a pass against the whole basis
+    assert stats[0].n_full_reorth == stats[0].n_matvec
+    assert stats[1].n_full_reorth < 0.5 * stats[1].n_matvec, (stats[1].n_full_reorth, stats[1].n_matvec)
+    assert abs(stats[1].n_matvec - stats[0].n_matvec) <= 0.1 * stats[0].n_matvec
+
+
+def test_partial_reorth_two_site(ctx):
+    # L=10 D=8 two-site ramp with partial re-orthogonalisation
+    L = 10
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), 2, Dcap=8)
+    ch.initialize(1, 5)
+    ch.normalize(1)
+    o = tn.SweepOpts.default(eig_tol=1e-12, reorth=1)
+    ch.load_caches(o)
+    for Dmax in (4, 8, 8, 8, 8, 8, 8, 8):
+        ch.sweep2(0, Dmax, 0.0, o)
+        ch.sweep2(1, Dmax, 0.0, o)
+    assert abs(ch.expectation() / 9 - (-0.473113399629092)) < 1e-7
