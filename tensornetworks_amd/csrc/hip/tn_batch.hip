@@ -357,13 +357,25 @@ std::vector<double> norms2(tn_batch* b, const std::vector<const double*>& v, con
 // Lockstep Lanczos (the algorithm of tn::lanczos: three-term step, one
 // Gram-Schmidt pass against all Krylov vectors, host tridiagonal solve) on
 // every slot; the Ritz vector replaces the slot's site tensor.
+//
+// partial: Simon's partial re-orthogonalisation as in tn::lanczos (eps_op the
+// operator's unit roundoff).  Each step works against v_{j-1}, v_j only; after
+// the read-back of iteration j the host propagates each chain's omega
+// estimates and, before launching iteration j+1, queues a pass against the
+// whole basis for the chains above sqrt(eps_op).  Coefficient slots per chain
+// (h = coef + k*stride): h[0] alpha dot, h[1..2] local dots, h[3 + (j & 1)]
+// |w_j|^2 (the pass overwrites it with the corrected value, which the device
+// reads as beta_j in iteration j+1), h[8..8+j] the pass's dots.  A chain with
+// a pass at j gets its corrected alpha_j, beta_j with the read-back of j+1,
+// so its tridiagonal solve of j is skipped.
 void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_restarts, std::vector<double>& e0,
-                   std::vector<double>& e1, tn_sweep_stats* st)
+                   std::vector<double>& e1, tn_sweep_stats* st, bool partial, double eps_op)
 {
     tn_ctx* c = b->ctx;
     hipStream_t s = c->stream;
     const int ns = (int)S.size(), kmax = b->kmax;
     const size_t str = b->stride;
+    TN_REQUIRE(!partial || str >= (size_t)kmax + 8, TN_ERR_STATE, "batch Lanczos: coefficient stride too small");
     double* coef = b->coef.as<double>();
     e0.assign(ns, 0.0);
     e1.assign(ns, 0.0);
@@ -372,6 +384,16 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
     std::vector<std::vector<double>> al(ns, std::vector<double>(kmax)), be(ns, std::vector<double>(kmax)),
         z(ns, std::vector<double>(kmax));
     std::vector<int> kk(ns, 0);
+    // partial re-orthogonalisation state per chain
+    const double eta = std::sqrt(eps_op);
+    std::vector<std::vector<double>> om_old, om, om_new;
+    if (partial) {
+        om_old.assign(ns, std::vector<double>(kmax + 2, 0.0));
+        om.assign(ns, std::vector<double>(kmax + 2, 0.0));
+        om_new.assign(ns, std::vector<double>(kmax + 2, 0.0));
+    }
+    std::vector<char> force(ns, 0), pend(ns, 0), full(ns, 0);
+    std::vector<double> tnorm(ns, 0.0);
     auto V = [&](int k) { return b->kry[S[k].q].as<double>(); };
     auto site = [&](int k) { return b->ch[S[k].q]->A[S[k].site]; };
     for (int restart = 0; restart <= max_restarts && !todo.empty(); restart++) {
@@ -394,6 +416,11 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             }
             TN_HIP(bscale(push(b, sj), (int)sj.size(), nmax, s));
         }
+        for (int k : todo) {
+            force[k] = pend[k] = full[k] = 0;
+            tnorm[k] = 0.0;
+            if (partial) om[k][0] = 1.0;
+        }
         // Pipelined lockstep iterations.  After the read-back of iteration j
         // the host launches iteration j+1 for every chain still active
         // (v_{j+1} = w / beta_j with beta_j taken from the device copy of
@@ -403,6 +430,8 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
         std::vector<int> act = todo;
         std::vector<double> resid(ns, 0.0);
         std::vector<char> fin(ns, 0);
+        // slot of |w_j|^2 in a chain's coefficients
+        auto nslot = [&](int j) { return partial ? 3 + (j & 1) : j + 2; };
         auto launch_iter = [&](int j, const std::vector<int>& A) {
             std::vector<Slot> sa;
             std::vector<const double*> xin;
@@ -417,8 +446,8 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 xin.push_back(vj);
                 yout.push_back(V(k) + (long)(j + 1) * N);
                 nmax = std::max(nmax, N);
-                // |w_{j-1}|^2 of iteration j-1 sits at h[j+1]
-                if (j > 0) sc.push_back(ScaleJob{0.0, vj, vj, N, h + j + 1});
+                // |w_{j-1}|^2 of iteration j-1
+                if (j > 0) sc.push_back(ScaleJob{0.0, vj, vj, N, h + nslot(j - 1)});
             }
             // every descriptor of the iteration in one transfer: the scaling,
             // the Heff stages, then the Lanczos step (alpha = v_j . w; three-term
@@ -429,14 +458,17 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             std::vector<DotJob> d1;
             std::vector<StepJob> sj3;
             std::vector<UpdJob> uj;
+            // the step works against v_{j-1}, v_j (partial) or every Krylov vector
+            const int kl = partial ? std::min(j + 1, 2) : j + 1;
             for (int k : A) {
                 double* Vk = V(k);
                 const long N = S[k].N;
                 double* w = Vk + (long)(j + 1) * N;
                 double* h = coef + (size_t)k * str;
+                const double* Vl = Vk + (long)(j + 1 - kl) * N;
                 d1.push_back(DotJob{Vk + (long)j * N, N, 1, w, N, h});
-                sj3.push_back(StepJob{w, Vk, N, j + 1, h, 0.0, N, h + 1, j > 0 ? h + j + 1 : nullptr});
-                uj.push_back(UpdJob{w, Vk, N, j + 1, h + 1, N, h + j + 2});
+                sj3.push_back(StepJob{w, Vl, N, kl, h, 0.0, N, h + 1, j > 0 ? h + nslot(j - 1) : nullptr});
+                uj.push_back(UpdJob{w, Vl, N, kl, h + 1, N, h + nslot(j)});
             }
             const DotJob* dd1 = stage(b, d1);
             const StepJob* dsj = stage(b, sj3);
@@ -449,8 +481,29 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             heff_batch_launch(b, hl);
             if (st) TN_HIP(hipEventRecord(c->ev[1], s));
             TN_HIP(bmdot(dd1, na, 1, nmax, part, s));
-            TN_HIP(bstep_dots(dsj, na, j + 1, nmax, part, s));
+            TN_HIP(bstep_dots(dsj, na, kl, nmax, part, s));
             TN_HIP(bupdate_norm(duj, na, nmax, part, s));
+        };
+        // partial: the pass of w_j against v_0..v_j for the chains in F
+        auto launch_full = [&](int j, const std::vector<int>& F) {
+            std::vector<DotJob> dj;
+            std::vector<UpdJob> uj;
+            long nmax = 1;
+            for (int k : F) {
+                double* Vk = V(k);
+                const long N = S[k].N;
+                double* w = Vk + (long)(j + 1) * N;
+                double* h = coef + (size_t)k * str;
+                dj.push_back(DotJob{Vk, N, j + 1, w, N, h + 8});
+                uj.push_back(UpdJob{w, Vk, N, j + 1, h + 8, N, h + nslot(j)});
+                nmax = std::max(nmax, N);
+            }
+            const DotJob* ddj = stage(b, dj);
+            const UpdJob* duj = stage(b, uj);
+            flush(b);
+            double* part = b->part.as<double>();
+            TN_HIP(bmdot(ddj, (int)F.size(), j + 1, nmax, part, s));
+            TN_HIP(bupdate_norm(duj, (int)F.size(), nmax, part, s));
         };
         double t0 = now_ms();
         if (kmax > 0 && !act.empty()) launch_iter(0, act);
@@ -463,19 +516,93 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 st->lanczos_ms += now_ms() - t0;
             }
             t0 = now_ms();
+            const int kl = partial ? std::min(j + 1, 2) : j + 1;
+            if (partial) {
+                // chains whose w_{j-1} had a pass against the whole basis: its dot
+                // with v_{j-1} and the corrected |w_{j-1}|^2.  When the pass left
+                // only rounding (an invariant subspace), v_j was noise: the chain
+                // ends at iteration j-1 and the results of j are dropped.
+                std::vector<int> keep;
+                for (int k : act) {
+                    if (pend[k]) {
+                        const double* h = b->hcoef.data() + (size_t)k * str;
+                        al[k][j - 1] += h[8 + (j - 1)];
+                        const double bc = std::sqrt(std::max(h[nslot(j - 1)], 0.0));
+                        pend[k] = 0;
+                        if (bc < 1e-10 * be[k][j - 1] || bc < 1e-14 * std::max(1.0, std::fabs(e0[k]))) {
+                            be[k][j - 1] = 0.0;
+                            tridiag_lowest(j, al[k].data(), be[k].data(), &e0[k], &e1[k], z[k].data(), true);
+                            kk[k] = j;
+                            resid[k] = 0.0;
+                            continue;
+                        }
+                        be[k][j - 1] = bc;
+                    }
+                    keep.push_back(k);
+                }
+                act.swap(keep);
+                if (act.empty()) break;
+            }
+            for (int k : act) {
+                const double* h = b->hcoef.data() + (size_t)k * str;
+                al[k][j] = h[0] + h[kl];
+                be[k][j] = std::sqrt(std::max(h[nslot(j)], 0.0));
+            }
+            std::vector<int> F;
+            if (partial) {
+                // omega_{j+1,.} of every active chain; a pass for those above eta
+                // (none in the last iteration: w_j is not used)
+                for (int k : act) {
+                    std::vector<double>&on = om_new[k], &oc = om[k], &oo = om_old[k];
+                    const double* a = al[k].data();
+                    const double* bb = be[k].data();
+                    tnorm[k] = std::max(tnorm[k], std::fabs(a[j]) + bb[j] + (j > 0 ? bb[j - 1] : 0.0));
+                    const double noise = eps_op * tnorm[k];
+                    double wmax = 0.0;
+                    if (bb[j] > 0.0) {
+                        for (int q = 0; q + 1 < j; q++) {
+                            double t = bb[q] * oc[q + 1] + (a[q] - a[j]) * oc[q] - bb[j - 1] * oo[q];
+                            if (q > 0) t += bb[q - 1] * oc[q - 1];
+                            t = (t + std::copysign(noise, t)) / bb[j];
+                            on[q] = t;
+                            wmax = std::max(wmax, std::fabs(t));
+                        }
+                        for (int q = std::max(j - 1, 0); q <= j; q++) on[q] = noise / bb[j];
+                    }
+                    on[j + 1] = 1.0;
+                    full[k] = 0;
+                    if (j > 0 && j + 1 < kmax && (force[k] || wmax > eta || !std::isfinite(wmax))) {
+                        for (int q = 0; q <= j; q++) on[q] = bb[j] > 0.0 ? noise / bb[j] : 0.0;
+                        force[k] = !force[k];
+                        full[k] = 1;
+                        F.push_back(k);
+                    } else {
+                        force[k] = 0;
+                    }
+                    std::swap(oo, oc);
+                    std::swap(oc, on);
+                }
+                if (!F.empty()) {
+                    launch_full(j, F);
+                    if (st) st->n_full_reorth += (int64_t)F.size();
+                }
+            } else if (st) {
+                st->n_full_reorth += (int64_t)act.size();
+            }
             if (j + 1 < kmax) launch_iter(j + 1, act);
             // the tridiagonal Ritz problems of the active chains, in parallel
-            // with the device and on the host threads
+            // with the device and on the host threads.  A chain whose w_j had a
+            // pass against the whole basis is tested with the values before the
+            // pass (its beta_j is then an upper bound); if it goes on, the
+            // corrected alpha_j, beta_j arrive with the next read-back.
             b->pool->run((int)act.size(), [&](int t) {
                 const int k = act[t];
-                const double* h = b->hcoef.data() + (size_t)k * str;
-                al[k][j] = h[0] + h[j + 1];
-                be[k][j] = std::sqrt(std::max(h[j + 2], 0.0));
                 tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], nullptr, z[k].data(), false);
                 kk[k] = j + 1;
                 resid[k] = std::fabs(be[k][j] * z[k][j]);
                 const bool done = resid[k] < tol || be[k][j] < 1e-14 * std::max(1.0, std::fabs(e0[k]));
                 fin[k] = done ? 1 : 0;
+                pend[k] = full[k] && !done;
                 if (done || j + 1 == kmax)  // the second Ritz value is only reported
                     tridiag_lowest(j + 1, al[k].data(), be[k].data(), &e0[k], &e1[k], z[k].data(), true);
             });
@@ -1044,7 +1171,7 @@ int tn_batch_sweep(tn_batch* b, int dir, const tn_sweep_opts* opts, tn_sweep_sta
                 S[k].N = S[k].op.n();
             }
             std::vector<double> e0, e1;
-            lanczos_batch(b, S, o.eig_tol, o.max_restarts, e0, e1, st);
+            lanczos_batch(b, S, o.eig_tol, o.max_restarts, e0, e1, st, o.reorth == 1, unit_roundoff(TN_FP32));
             for (size_t k = 0; k < S.size(); k++) {
                 tn_chain* ch = b->ch[S[k].q];
                 const int i = S[k].site;

@@ -71,4 +71,6 @@ void rescale_site(tn_chain* ch, int i);
 void svd_normalize(tn_chain* ch, int i, int dir, tn_sweep_stats* st, int gauge);
 void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_stats* st);
 void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st, double* e0, double* e1);
+// unit roundoff of Heff operands of a precision (relative rounding of an apply)
+double unit_roundoff(int precision);
 }  // namespace tn

@@ -744,7 +744,9 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
                     TN_HIP(lz_update_norm(w, V, N, j + 1, h + 1, N, h + j + 2, part, pcap, s));
                     read(hh.data(), h, j + 3);
                     al[j] += hh[j + 1];
-                    be[j] = std::sqrt(std::max(hh[j + 2], 0.0));
+                    const double bc = std::sqrt(std::max(hh[j + 2], 0.0));
+                    // only rounding left: an invariant subspace (breakdown)
+                    be[j] = bc < 1e-10 * be[j] ? 0.0 : bc;
                     for (int q = 0; q <= j; q++) om_new[q] = be[j] > 0.0 ? noise / be[j] : 0.0;
                     force = !force;
                     res.n_full++;

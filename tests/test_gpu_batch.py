@@ -52,6 +52,39 @@ def test_batch_matches_single_chain_sweeps(ctx):
     b.close()
 
 
+def test_batch_partial_reorth_matches_single_chain_sweeps(ctx):
+    # partial re-orthogonalisation in the pipelined batch (a chain whose new
+    # vector needs a pass against the whole basis defers its tridiagonal
+    # solve by one iteration) against the one-chain sweeps with the same
+    # option, and against the batch with a pass every iteration
+    spec = [(8, 8, 1.0, 1.0, 0.0), (12, 16, 0.7, 1.6, 0.3), (6, 32, 1.3, 0.4, -0.2), (10, 12, 0.9, 1.1, 0.45),
+            (14, 40, 1.1, 0.8, -0.1)]
+    energies, stats = {}, {}
+    for reorth in (0, 1):
+        o = fp32_opts(reorth=reorth)
+        chains = [make_chain(ctx, *s, seed=100 + k, opts=o) for k, s in enumerate(spec)]
+        b = tn.Batch(ctx, chains, max_krylov=40)
+        st = tn.SweepStats()
+        for _ in range(3):
+            b.sweep(0, o, st)
+            b.sweep(1, o, st)
+        energies[reorth] = [ch.expectation() for ch in chains]
+        stats[reorth] = st
+        b.close()
+    o = fp32_opts(reorth=1)
+    solo = [make_chain(ctx, *s, seed=100 + k, opts=o) for k, s in enumerate(spec)]
+    for ch in solo:
+        for _ in range(3):
+            ch.sweep(0, o)
+            ch.sweep(1, o)
+    for (L, *_), eb, ef, cs in zip(spec, energies[1], energies[0], solo):
+        es = cs.expectation()
+        assert abs(eb - es) < 1e-8 * max(1.0, abs(es)), (L, eb, es)
+        assert abs(eb - ef) < 1e-8 * max(1.0, abs(ef)), (L, eb, ef)
+    assert stats[0].n_full_reorth == stats[0].n_matvec
+    assert stats[1].n_full_reorth < 0.5 * stats[1].n_matvec, (stats[1].n_full_reorth, stats[1].n_matvec)
+
+
 def test_batch_identity_channels_same_sweeps(ctx):
     # the batched operator with identity channels copied gives the sweeps of
     # the full contraction
