@@ -188,7 +188,7 @@ def run_batch(a, tn, ctx):
 
 
 def run_precond(a, tn, ctx):
-    """Learned preconditioner: samples from random XXZ chains at D <= 96
+    """Learned preconditioner: samples from random XXZ chains at D <= 128
     (couplings as the batched workload), device training (own forward /
     backward / Adam kernels), then one sweep of the S=1/2 Heisenberg chain
     (L=128, D=1024 by default; --L/--D override) from the same warm state with
@@ -199,7 +199,7 @@ def run_precond(a, tn, ctx):
     from tensornetworks_amd import precond
     from tensornetworks_amd.models import xxz_mpo
     t0 = time.perf_counter()
-    X, av, tv = precond.generate_samples(ctx, n_chains=24, seed=a.seed % 100000, L_range=(12, 32), D_range=(16, 96),
+    X, av, tv = precond.generate_samples(ctx, n_chains=24, seed=a.seed % 100000, L_range=(12, 32), D_range=(16, 128),
                                          iters=1, nsamp=1024, sites_per_chain=12)
     params, norm, (c0, c1) = precond.train(ctx, X, av, tv, 1024, rounds=10, steps=400, batch=4096, lr=2e-3)
     train_s = time.perf_counter() - t0
