@@ -60,10 +60,11 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--seed", type=int, default=20261202)
     p.add_argument("--out", default=None)
-    p.add_argument("--workload", default="chain", choices=["chain", "batch", "precond"],
+    p.add_argument("--workload", default="chain", choices=["chain", "batch", "precond", "twosite"],
                    help="chain: the D=1024 single-chain sweep (default); batch: 128 XXZ chains swept in lockstep; "
                         "precond: train the learned Davidson preconditioner on the device and compare local "
-                        "solvers on the S=1/2 Heisenberg chain L=128, D=1024")
+                        "solvers on the S=1/2 Heisenberg chain L=128, D=1024; twosite: two-site DMRG of the "
+                        "S=1/2 Heisenberg chain L=64 with D ramped 32 -> 256, FP64, tolerance 1e-12")
     p.add_argument("--chains", type=int, default=128, help="batch workload: number of chains")
     p.add_argument("--no-sequential", action="store_true",
                    help="batch workload: skip the chain-by-chain comparison (for profiling runs)")
@@ -261,6 +262,72 @@ def run_precond(a, tn, ctx):
         precond.save(a.precond_out, params, norm)
 
 
+def run_twosite(a, tn, ctx):
+    """Two-site DMRG: open S=1/2 Heisenberg chain L=64 (J=Jz=1, hz=0), seeded
+    random MPS at D=32, two-site sweeps (left and right pass) at Dmax = 32, 64,
+    128, 256, 256 with FP64 contractions and Lanczos tolerance 1e-12, no
+    singular-value cutoff.  One step = one two-site sweep at D=256 (the last
+    stage).  Then one-site sweeps at D=256 from the final state: their energy
+    may only fall below the two-site one by the discarded weight's effect."""
+    from tensornetworks_amd.models import xxz_mpo
+    L = 64
+    o = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, refine=0, eig_tol=1e-12,
+                             max_krylov=a.max_krylov, max_restarts=8, gauge=0,
+                             reorth=1 if a.reorth == "partial" else 0)
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), 32, Dcap=256)
+    ch.initialize(1, a.seed)
+    ch.normalize(1)
+    ch.load_caches(o)
+    stages = []
+    last = None
+    for Dmax in (32, 64, 128, 256, 256):
+        st = tn.SweepStats()
+        t1 = time.perf_counter()
+        _, tr0 = ch.sweep2(0, Dmax, 0.0, o, st)
+        _, tr1 = ch.sweep2(1, Dmax, 0.0, o, st)
+        dt = time.perf_counter() - t1
+        e = ch.expectation() / (L - 1)
+        stages.append({"Dmax": Dmax, "sweep_s": dt, "E_per_bond": e, "truncation": max(tr0, tr1),
+                       "n_matvec": st.n_matvec, "heff_tflops": st.heff_flops / max(st.heff_ms, 1e-12) / 1e9,
+                       "lanczos_ms": st.lanczos_ms, "svd_ms": st.svd_ms})
+        print(f"# two-site Dmax={Dmax}: {dt:.2f}s E/bond={e:.12f} truncation={max(tr0, tr1):.2e} "
+              f"matvecs={st.n_matvec} dims max {max(ch.dims()[1])}", file=sys.stderr, flush=True)
+        last = (dt, st)
+    e2 = ch.expectation() / (L - 1)
+    o1 = tn.SweepOpts.from_buffer_copy(o)
+    ch.load_caches(o1)
+    t1 = time.perf_counter()
+    for _ in range(2):
+        ch.sweep(0, o1)
+        ch.sweep(1, o1)
+    t_one = (time.perf_counter() - t1) / 2
+    e1 = ch.expectation() / (L - 1)
+    print(f"# one-site at D=256 from the two-site state: {t_one:.2f}s/sweep E/bond={e1:.12f} "
+          f"(two-site {e2:.12f})", file=sys.stderr, flush=True)
+    dt, st = last
+    out = {
+        "metric": "two-site DMRG sweep time at D=256, S=1/2 Heisenberg L=64 (D ramped 32 -> 256)",
+        "value": 1.0 / dt,
+        "unit": "sweeps/s",
+        "n_gpus": 1, "steps": 1, "warmup": 4, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic (seeded random initial MPS)",
+        "config": {"workload": "open S=1/2 Heisenberg chain L=64, J=Jz=1, d=2, Dw=5, two-site sweeps at Dmax 32, "
+                               "64, 128, 256, 256, FP64 contractions, Lanczos tolerance 1e-12, no cutoff",
+                   "L": L, "eig_tol": 1e-12, "max_krylov": a.max_krylov, "reorth": a.reorth, "seed": a.seed},
+        "stages": stages,
+        "E_per_bond_two_site": e2,
+        "E_per_bond_one_site_after": e1,
+        "one_site_sweep_s": t_one,
+        "heff_tflops": st.heff_flops / max(st.heff_ms, 1e-12) / 1e9,
+    }
+    line = json.dumps(out)
+    print(line, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(line + "\n")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -272,13 +339,13 @@ def main():
 
     P = {"bf16": tn.TN_BF16, "fp32": tn.TN_FP32, "fp64": tn.TN_FP64}
     prec = tn.TN_BF16 if a.precision == "mixed" else P[a.precision]
-    if a.workload in ("batch", "precond"):
+    if a.workload in ("batch", "precond", "twosite"):
         ctx = tn.Context(0)
         if not a.no_autotune:
             ctx.set_autotune(True)
             if a.tune_db and os.path.exists(a.tune_db):
                 ctx.tune_load(a.tune_db)
-        (run_batch if a.workload == "batch" else run_precond)(a, tn, ctx)
+        {"batch": run_batch, "precond": run_precond, "twosite": run_twosite}[a.workload](a, tn, ctx)
         if a.tune_save and not a.no_autotune:
             ctx.tune_save(a.tune_save)
         return

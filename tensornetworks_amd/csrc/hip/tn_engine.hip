@@ -844,11 +844,18 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     int* flag = c->svd_flag.as<int>();
     const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)p));
     int hflag = 1;
-    for (int sweep = 0; sweep < 40 && hflag; sweep++) {
+    // one sweep of rotations on each pair's Gram matrix per visit: the pairs
+    // come back every outer sweep, and a launch has only nb/2 workgroups, so
+    // its latency (mostly the inner rounds) sets the cost.  Measured on the
+    // 512 x 512 two-site splits at D=256: one inner sweep halves the SVD time
+    // against diagonalising each pair (up to 20 inner sweeps), with more
+    // outer sweeps and the same singular values.
+    const int inner = 1;
+    for (int sweep = 0; sweep < 80 && hflag; sweep++) {
         TN_HIP(set_int(flag, 0, s));
         for (int rd = 0; rd < nb - 1; rd++)
             TN_HIP(jacobi_step(X, ldx, p, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb,
-                               nb / 2, tol, flag, s));
+                               nb / 2, tol, flag, s, inner));
         TN_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         sync(c);
     }

@@ -143,9 +143,10 @@ hipError_t dcopy_scalar(const double* src, double* dst, hipStream_t s);
 // dst[0] = sqrt(src[0])
 hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s);
 
-// one-sided block Jacobi SVD pieces (fp64, column-major X with ldx, q%32==0)
+// one-sided block Jacobi SVD pieces (fp64, column-major X with ldx, q%32==0);
+// each pair's 32 x 32 Gram matrix gets at most `inner` sweeps of rotations
 hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv,
-                       const int* pairs, int npairs, double tol, int* flag, hipStream_t s);
+                       const int* pairs, int npairs, double tol, int* flag, hipStream_t s, int inner);
 hipError_t col_norms(const double* X, long ldx, int p, int q, double* s, hipStream_t st);
 hipError_t set_int(int* p, int v, hipStream_t s);
 hipError_t set_identity(double* V, long ldv, int n, hipStream_t s);

@@ -1135,7 +1135,7 @@ hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s)
 __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx, int p,
                                                 double* __restrict__ V, long ldv, int qv,
                                                 const int* __restrict__ pairs, double tol,
-                                                int* __restrict__ flag)
+                                                int* __restrict__ flag, int inner)
 {
     __shared__ double G[JW][JW + 1];
     __shared__ double J[JW][JW + 1];
@@ -1193,7 +1193,7 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
     if (tid == 0) flag[0] = 1;
 
     // 2. cyclic parallel Jacobi on G (tournament ordering, 31 rounds per sweep)
-    for (int sweep = 0; sweep < 20; sweep++) {
+    for (int sweep = 0; sweep < inner; sweep++) {
         if (tid == 0) any_rot = 0;
         __syncthreads();
         for (int round = 0; round < JW - 1; round++) {
@@ -1272,10 +1272,10 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
 }
 
 hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv, const int* pairs,
-                       int npairs, double tol, int* flag, hipStream_t s)
+                       int npairs, double tol, int* flag, hipStream_t s, int inner)
 {
     if (npairs <= 0) return hipSuccess;
-    jacobi_k<<<npairs, 256, 0, s>>>(X, ldx, p, V, ldv, qv, pairs, tol, flag);
+    jacobi_k<<<npairs, 256, 0, s>>>(X, ldx, p, V, ldv, qv, pairs, tol, flag, inner);
     return hipGetLastError();
 }
 
