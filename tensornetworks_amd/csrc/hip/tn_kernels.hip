@@ -1131,6 +1131,7 @@ hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s)
 // Jacobi and the rotation applied to the 32 columns of X and of V.
 #define JB 16
 #define JW 32
+#define JR 128  // rows per Gram chunk
 
 __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx, int p,
                                                 double* __restrict__ V, long ldv, int qv,
@@ -1139,7 +1140,7 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
 {
     __shared__ double G[JW][JW + 1];
     __shared__ double J[JW][JW + 1];
-    __shared__ double tile[JW][JW + 1];
+    __shared__ double tile[JR][JW + 1];
     __shared__ double rc[JW / 2], rs[JW / 2];
     __shared__ int rp[JW / 2], rq[JW / 2];
     __shared__ int any_rot;
@@ -1148,15 +1149,22 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
     const int pb = pairs[2 * blockIdx.x], qb = pairs[2 * blockIdx.x + 1];
     auto colidx = [&](int j) { return j < JB ? pb * JB + j : qb * JB + (j - JB); };
 
-    // 1. Gram matrix over row chunks staged in LDS
+    // 1. Gram matrix over row chunks of JR rows staged in LDS (all loads of a
+    // chunk in flight together: few load-then-barrier rounds per launch)
     double acc[4] = {0, 0, 0, 0};
-    for (int r0 = 0; r0 < p; r0 += JW) {
+    for (int r0 = 0; r0 < p; r0 += JR) {
+        double ld[JR * JW / 256];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < JR * JW / 256; u++) {
             const int idx = tid + 256 * u;
-            const int row = idx & (JW - 1), c = idx >> 5;
+            const int row = idx % JR, c = idx / JR;
             const int gr = r0 + row;
-            tile[row][c] = (gr < p) ? X[(long)colidx(c) * ldx + gr] : 0.0;
+            ld[u] = (gr < p) ? X[(long)colidx(c) * ldx + gr] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < JR * JW / 256; u++) {
+            const int idx = tid + 256 * u;
+            tile[idx % JR][idx / JR] = ld[u];
         }
         __syncthreads();
 #pragma unroll
@@ -1165,7 +1173,7 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
             const int gi = e >> 5, gj = e & 31;
             double s = 0.0;
 #pragma unroll 8
-            for (int rr = 0; rr < JW; rr++) s += tile[rr][gi] * tile[rr][gj];
+            for (int rr = 0; rr < JR; rr++) s += tile[rr][gi] * tile[rr][gj];
             acc[u] += s;
         }
         __syncthreads();
