@@ -1138,9 +1138,10 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
                                                 const int* __restrict__ pairs, double tol,
                                                 int* __restrict__ flag, int inner)
 {
+    // even row strides: column pairs are 16-byte aligned for double2 reads
     __shared__ double G[JW][JW + 1];
-    __shared__ double J[JW][JW + 1];
-    __shared__ double tile[JR][JW + 1];
+    __shared__ __align__(16) double J[JW][JW + 2];
+    __shared__ __align__(16) double tile[JR][JW + 2];
     __shared__ double rc[JW / 2], rs[JW / 2];
     __shared__ int rp[JW / 2], rq[JW / 2];
     __shared__ int any_rot;
@@ -1150,8 +1151,11 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
     auto colidx = [&](int j) { return j < JB ? pb * JB + j : qb * JB + (j - JB); };
 
     // 1. Gram matrix over row chunks of JR rows staged in LDS (all loads of a
-    // chunk in flight together: few load-then-barrier rounds per launch)
-    double acc[4] = {0, 0, 0, 0};
+    // chunk in flight together: few load-then-barrier rounds per launch).
+    // Thread (bi, bj) forms the 2 x 2 block G[2bi..2bi+1][2bj..2bj+1] from
+    // two 16-byte LDS reads per row.
+    const int bi = tid >> 4, bj = tid & 15;
+    double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0;
     for (int r0 = 0; r0 < p; r0 += JR) {
         double ld[JR * JW / 256];
 #pragma unroll
@@ -1167,21 +1171,24 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
             tile[idx % JR][idx / JR] = ld[u];
         }
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int e = tid + 256 * u;
-            const int gi = e >> 5, gj = e & 31;
-            double s = 0.0;
 #pragma unroll 8
-            for (int rr = 0; rr < JR; rr++) s += tile[rr][gi] * tile[rr][gj];
-            acc[u] += s;
+        for (int rr = 0; rr < JR; rr++) {
+            const double2 a = *(const double2*)&tile[rr][2 * bi];
+            const double2 b = *(const double2*)&tile[rr][2 * bj];
+            g00 = fma(a.x, b.x, g00);
+            g01 = fma(a.x, b.y, g01);
+            g10 = fma(a.y, b.x, g10);
+            g11 = fma(a.y, b.y, g11);
         }
         __syncthreads();
     }
+    G[2 * bi][2 * bj] = g00;
+    G[2 * bi][2 * bj + 1] = g01;
+    G[2 * bi + 1][2 * bj] = g10;
+    G[2 * bi + 1][2 * bj + 1] = g11;
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const int e = tid + 256 * u;
-        G[e >> 5][e & 31] = acc[u];
         J[e >> 5][e & 31] = ((e >> 5) == (e & 31)) ? 1.0 : 0.0;
     }
     if (tid == 0) sh_off = 0;
@@ -1252,29 +1259,40 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
         __syncthreads();
     }
 
-    // 3. X[:, cols] <- X[:, cols] J ; V[:, cols] <- V[:, cols] J
+    // 3. X[:, cols] <- X[:, cols] J ; V[:, cols] <- V[:, cols] J (J read in
+    // 16-byte column pairs, the same for every lane)
     for (int row = tid; row < p; row += 256) {
         double x[JW];
 #pragma unroll
         for (int c = 0; c < JW; c++) x[c] = X[(long)colidx(c) * ldx + row];
-#pragma unroll 4
-        for (int c2 = 0; c2 < JW; c2++) {
-            double s = 0.0;
+#pragma unroll 2
+        for (int c2 = 0; c2 < JW; c2 += 2) {
+            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int c = 0; c < JW; c++) s += x[c] * J[c][c2];
-            X[(long)colidx(c2) * ldx + row] = s;
+            for (int c = 0; c < JW; c++) {
+                const double2 jj = *(const double2*)&J[c][c2];
+                s0 = fma(x[c], jj.x, s0);
+                s1 = fma(x[c], jj.y, s1);
+            }
+            X[(long)colidx(c2) * ldx + row] = s0;
+            X[(long)colidx(c2 + 1) * ldx + row] = s1;
         }
     }
     for (int row = tid; row < qv; row += 256) {
         double x[JW];
 #pragma unroll
         for (int c = 0; c < JW; c++) x[c] = V[(long)colidx(c) * ldv + row];
-#pragma unroll 4
-        for (int c2 = 0; c2 < JW; c2++) {
-            double s = 0.0;
+#pragma unroll 2
+        for (int c2 = 0; c2 < JW; c2 += 2) {
+            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int c = 0; c < JW; c++) s += x[c] * J[c][c2];
-            V[(long)colidx(c2) * ldv + row] = s;
+            for (int c = 0; c < JW; c++) {
+                const double2 jj = *(const double2*)&J[c][c2];
+                s0 = fma(x[c], jj.x, s0);
+                s1 = fma(x[c], jj.y, s1);
+            }
+            V[(long)colidx(c2) * ldv + row] = s0;
+            V[(long)colidx(c2 + 1) * ldv + row] = s1;
         }
     }
 }
