@@ -44,9 +44,14 @@ class SweepOpts(ctypes.Structure):
 
     @classmethod
     def default(cls, **kw):
+        """tn_sweep_opts_default, then the given fields; an unknown field name
+        raises (a ctypes structure would keep it as a plain attribute)."""
         o = cls()
         lib.tn_sweep_opts_default(ctypes.byref(o))
+        names = {f for f, _ in cls._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise TypeError(f"tn_sweep_opts has no field {k!r}")
             setattr(o, k, v)
         return o
 

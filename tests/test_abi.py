@@ -53,3 +53,8 @@ def test_svcompress_rule_matches_oracle(seed):
 def test_sweep_opts_defaults():
     o = tn.SweepOpts.default()
     assert o.precision == tn.TN_FP64 and o.max_krylov >= 16 and o.eig_tol > 0
+    assert o.reorth == 0 and o.skip_identity == 1
+    o = tn.SweepOpts.default(reorth=1, precond=3)
+    assert o.reorth == 1 and o.precond == 3
+    with pytest.raises(TypeError):
+        tn.SweepOpts.default(reorht=1)
