@@ -12,10 +12,11 @@
 //   Enums                  TensorNetworks/Enums.H:8-16
 //   SVCompressor           TensorNetworks/SVCompressor.H:10-24
 //   TNSLogger              TensorNetworks/TNSLogger.H:14-50
-//   EigenSolver / SparseEigenSolver / SVDSolver  NumericalMethods/*.H (solver seam)
+//   EigenSolver / SparseEigenSolver / SVDSolver / QRSolver  NumericalMethods/*.H (solver seam)
 #ifndef TN_TENSORNETWORKS_H
 #define TN_TENSORNETWORKS_H
 
+#include <algorithm>
 #include <cstdarg>
 #include <string>
 #include <tuple>
@@ -30,6 +31,7 @@ enum Position { PLeft, PBulk, PRight };
 enum Direction { DLeft, DRight };
 enum State { Product, Random, Neel, Constant };
 enum MPOForm { FUnknown, RegularUpper, RegularLower, expH, FUnit };
+enum CompressType { CNone, Std, Parker };
 // arithmetic of the contractions (extension: the reference is FP64 complex only)
 enum Precision { FP64 = TN_FP64, FP32 = TN_FP32, BF16 = TN_BF16 };
 
@@ -97,7 +99,13 @@ public:
     void Insert(const IterationScheduleLine& l) { itsLines.push_back(l); }
     void begin() const { itsCurrentLine = 0; }
     bool end() const { return itsCurrentLine == itsLines.size(); }
-    const IterationScheduleLine& operator++(int) const { return itsLines[itsCurrentLine++]; }
+    // moves to the next line and returns it (IterationSchedule.H:45-49); past
+    // the last line it returns the last line instead of reading beyond the end
+    const IterationScheduleLine& operator++(int) const
+    {
+        itsCurrentLine++;
+        return itsLines[std::min<size_t>(itsCurrentLine, itsLines.size() - 1)];
+    }
     const IterationScheduleLine& operator*() const { return itsLines[itsCurrentLine]; }
 private:
     mutable unsigned int itsCurrentLine;
@@ -136,15 +144,24 @@ private:
 class MPO {
 public:
     MPO() {}
-    explicit MPO(std::vector<SiteOperator> ops) : itsOps(std::move(ops)) {}
+    explicit MPO(std::vector<SiteOperator> ops, MPOForm f = FUnknown) : itsOps(std::move(ops)), itsForm(f) {}
     virtual ~MPO() {}
     int GetL() const { return (int)itsOps.size(); }
     const SiteOperator* GetSiteOperator(int isite) const { return &itsOps[isite - 1]; }  // 1-based
     int GetMaxDw() const;
-    // MPO::Product (TensorNetworks/MPO.C:73-79), uncompressed
+    MPOForm GetForm() const { return itsForm; }
+    // MPO::Product (TensorNetworks/MPO.C:73-79), uncompressed: a new MPO
     MPO* Product(const MPO* other) const;
+    // MPO::CanonicalForm (MPO.C:120-128): block QR sweeps left then right;
+    // needs a regular form (RegularUpper / RegularLower)
+    void CanonicalForm();
+    // MPO::Compress(CompressType, Dmax, eps) (MPO.C:93-118): Parker compression
+    // of the interior channels (Std is not provided: throws); returns the
+    // truncation error per bond
+    double Compress(CompressType, int Dmax, double epsSV);
 protected:
     std::vector<SiteOperator> itsOps;
+    MPOForm itsForm = FUnknown;
 };
 
 class MPS;
@@ -152,12 +169,17 @@ class MPS;
 // ---- Hamiltonian.H ----------------------------------------------------------
 class Hamiltonian : public MPO {
 public:
+    // nearest-neighbour XXZ chain (Hamiltonian_1D_NN_Heisenberg.C:35-79) + Dz Sz^2
     Hamiltonian(int L, double S, MPOForm f, double Jxy, double Jz, double hz, double Dz);
+    // any model from its bulk operator-valued matrix Wb (Dw x Dw blocks of d x d,
+    // [((w1*Dw + w2)*d + m)*d + n]), edges kept as SiteOperatorImp::Init_lr does
+    Hamiltonian(int L, double S, MPOForm f, int Dw, const std::vector<double>& Wb);
     int Getd() const { return itsd; }
     double GetS() const { return itsS; }
     MPS* CreateMPS(int D, double normEps = 1e-12, double epsSV = 1e-12) const;
     MPO* CreateUnitOperator() const;
-    MPO* CreateH2Operator() const;  // Hamiltonian.C:8-16 (product, no Parker compression)
+    // Hamiltonian.C:8-16: H*H, CanonicalForm, Compress(Parker, 0, 1e-13)
+    MPO* CreateH2Operator() const;
 private:
     int itsL, itsd;
     double itsS;
@@ -193,6 +215,16 @@ public:
     virtual ~SparseEigenSolver() {}
     virtual void Reset() = 0;
     virtual UdType Solve(const SparseEigenSolverClient<T>*, double eps, int NumEigenValues) = 0;
+};
+
+// QR seam (NumericalMethods/QRSolver.H; LapackQRSolver.C): thin QR (A = Q R,
+// m >= n) and thin LQ (A = L Q, m <= n), as MPSSite::NormalizeQR uses them
+template <class T> class QRSolver {
+public:
+    typedef std::tuple<Matrix<T>, Matrix<T>> QRType;
+    virtual ~QRSolver() {}
+    virtual QRType SolveThinQR(const Matrix<T>& A) = 0;  // returns (Q, R)
+    virtual QRType SolveThinLQ(const Matrix<T>& A) = 0;  // returns (L, Q)
 };
 
 template <class T> class SVDSolver {
@@ -231,6 +263,16 @@ private:
     std::vector<double> itsGuess;
 };
 
+// Device CholeskyQR behind the QRSolver seam (replaces LapackQRSolver).
+class GpuQRSolver : public QRSolver<double> {
+public:
+    explicit GpuQRSolver(tn_ctx* ctx) : itsCtx(ctx) {}
+    QRType SolveThinQR(const Matrix<double>& A) override;
+    QRType SolveThinLQ(const Matrix<double>& A) override;
+private:
+    tn_ctx* itsCtx;
+};
+
 // Device one-sided Jacobi SVD behind the SVDSolver seam (replaces LapackSVDSolver).
 class GpuSVDSolver : public SVDSolver<double> {
 public:
@@ -250,8 +292,19 @@ class MPS {
 public:
     MPS(const Hamiltonian* H, int D, double normEps, double epsSV, int Dcap = 0);
     ~MPS();
+    MPS(const MPS&) = delete;  // owns device memory
+    MPS& operator=(const MPS&) = delete;
     void InitializeWith(State, uint64_t seed = 20261202);
     void Normalize(Direction LR);
+    // MPS::NormalizeQR (MPSSite_Normalization.C:80-107): QR gauge moves
+    void NormalizeQR(Direction LR);
+    // MPS::GetNormStatus (MPSImp.C:180-192, MPSSite.C:173-211) with normEps
+    std::string GetNormStatus() const;
+    char GetNormStatus(int isite) const;  // 1-based
+    // checkpoint / resume (tn_chain_save / tn_chain_load): the state, its
+    // bond values and optionally the environment caches
+    void Save(const std::string& path, bool withEnvironments = false) const;
+    static MPS* Load(const std::string& path, int Dcap = 0);
     // MPSImp::FindVariationalGroundState (MPSImp_Variational.C:16-62); returns <H^2>-<H>^2
     double FindVariationalGroundState(const Hamiltonian* H, const IterationSchedule&);
     double GetExpectation(const MPO* o) const;
@@ -265,7 +318,9 @@ public:
 private:
     double FindVariationalGroundState(const Hamiltonian* H, const IterationScheduleLine&);
     void Sweep(Direction lr, const Epsilons& eps);
+    MPS() : itsL(0), itsd(0), itsNormEps(1e-12), itsCtx(nullptr), itsChain(nullptr) {}
     int itsL, itsd;
+    double itsNormEps;
     tn_ctx* itsCtx;
     tn_chain* itsChain;
     tn_sweep_opts itsOpts;
@@ -281,6 +336,12 @@ public:
     // extension: single-ion anisotropy Dz*Sz^2 (the S=1 workload)
     Hamiltonian* Make1D_NN_HeisenbergHamiltonian(int L, double S, MPOForm, double Jxy, double Jz, double hz,
                                                  double Dz);
+    // Factory.H:17-20 (Hamiltonian_1D_NN_TransverseIsing.C, Hamiltonian_1D_2Body_LongRange.C,
+    // Hamiltonian_1D_3Body_LongRange.C, Hamiltonian_1D_3Body.C)
+    Hamiltonian* Make1D_NN_TransverseIsingHamiltonian(int L, double S, MPOForm, double J, double hx);
+    Hamiltonian* Make1D_2BodyLongRangeHamiltonian(int L, double S, MPOForm, double J, double hx, int NN);
+    Hamiltonian* Make1D_3BodyLongRangeHamiltonian(int L, double S, MPOForm, double J, double hx, int NN);
+    Hamiltonian* Make1D_3BodyHamiltonian(int L, double S, MPOForm, double J, double K, double hz);
     SVCompressorC* MakeMPSCompressor(int Dmax, double epsSV);
     TNSLogger* MakeSPDLogger(int level);
 };

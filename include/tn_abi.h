@@ -78,11 +78,20 @@ int tn_heff_apply(tn_ctx* ctx, const double* x, double* y, int64_t N);
  * (uses the bound R, writes (Dw1, D1, D1)).  M is the site tensor.
  * TensorNetworksImp/MPS/MPSSite_Contractions.C:292-400. */
 int tn_env_update(tn_ctx* ctx, int dir, const double* M, double* Fout);
+/* The same product with x and y in device memory of the context's device
+ * (reference layout, N doubles each).  The data must be complete before the
+ * call (the library's stream does not wait on other streams); returns when y
+ * is written.  No PCIe traffic: the layout change runs on the device. */
+int tn_heff_apply_device(tn_ctx* ctx, const double* x_dev, double* y_dev, int64_t N);
 /* Lowest eigenpair(s) of the bound Heff by Lanczos with full FP64
  * re-orthogonalisation; replaces LapackEigenSolver<dcmplx>::Solve(Heff, eps, 2)
- * (NumericalMethods/LapackEigenSolver.C:62-109).  evec: in = start vector
- * (zero -> seeded random), out = Ritz vector (unit norm).  evals[0..nev-1]
- * ascending (nev <= 2).  Returns TN_WARN_NOT_CONVERGED if maxit is reached. */
+ * (NumericalMethods/LapackEigenSolver.C:62-109: IL=1, IU=2).  evec holds nev
+ * columns of N doubles (column k at evec + k*N): on input column 0 is the
+ * start vector (all zero or non-finite -> seeded random), on output column k
+ * is the eigenvector of evals[k] (unit norm), evals ascending, nev <= 2.  The
+ * second pair comes from Lanczos on H + sigma x0 x0^T (x0 the first vector),
+ * sigma above the spread of the first Ritz values.  n_matvec counts both
+ * solves.  Returns TN_WARN_NOT_CONVERGED if maxit is reached. */
 int tn_eig_lowest(tn_ctx* ctx, int64_t N, int nev, double tol, int maxit, double* evals,
                   double* evec, int* n_matvec);
 /* Thin SVD A = U diag(s) Vt of a column-major m x n matrix, k = min(m,n):
@@ -90,11 +99,41 @@ int tn_eig_lowest(tn_ctx* ctx, int64_t N, int nev, double tol, int maxit, double
  * orthonormal columns/rows even for zero singular values.  Replaces
  * LapackSVDSolver<T>::Solve (NumericalMethods/LapackSVDSolver.C:42-90). */
 int tn_svd(tn_ctx* ctx, int m, int n, const double* A, double* U, double* s, double* Vt);
+/* Thin QR / LQ of a column-major m x n matrix on the device (shifted
+ * CholeskyQR, FP64); replaces LapackQRSolver<T>::SolveThinQR / SolveThinLQ as
+ * MPSSite::NormalizeQR calls them (TensorNetworksImp/MPS/MPSSite_Normalization.C:80-107).
+ * mode 0: A = Q R with m >= n, Q (m x n) orthonormal columns, R (n x n) upper
+ * triangular with a non-negative diagonal; mode 1: A = L Q with m <= n,
+ * L (m x m) lower triangular, Q (m x n) orthonormal rows.  Column-major
+ * outputs.  Returns TN_WARN_NOT_CONVERGED when max|Q Q^T - I| (the
+ * orthogonality of the computed factor) stays above 1e-11, < 0 on error
+ * (including m < n for mode 0 and m > n for mode 1, the forms LAPACK returns
+ * with a rectangular triangular factor). */
+int tn_qr(tn_ctx* ctx, int mode, int m, int n, const double* A, double* Q, double* R);
 /* Truncation rule of SVCompressorImp<T>::Compress
  * (TensorNetworksImp/SVCompressorImp.C:19-73): returns the kept dimension,
  * the discarded weight sum(s^2) and the squared-norm ratio rescale factor. */
 int tn_svcompress_rank(int N, const double* s, int Dmax, double eps, double* discarded,
                        double* rescale);
+
+/* ---- MPO canonical form and compression (host) ---------------------------
+ * MPO::CanonicalForm() and MPO::Compress(Parker, Dmax, eps)
+ * (TensorNetworks/MPO.C:93-128; Operators/SiteOperatorImpCompress.C:16-100 ->
+ * MatrixO::BlockQX / BlockSVD, Operators/OperatorValuedMatrix.C:437-537,786-840),
+ * as Hamiltonian::CreateH2Operator applies them (TensorNetworks/Hamiltonian.C:8-16).
+ * The MPO is given like tn_chain_create's (dw1, dw2, W concatenated) in
+ * regular form (form 0 = RegularUpper, 1 = RegularLower: the identity
+ * channels first/last on every bond).  ops: 1 = canonical form (block QR
+ * sweeps left then right), 2 = compression (block QR + SVD of the interior
+ * block, left then right, singular values cut by the SVCompressor rule
+ * (Dmax, eps); Dmax = 0 no cap), 3 = both.  Writes the new bond dims into
+ * dw1_out/dw2_out (L each) and the blocks into w_out when cap (doubles) is
+ * large enough; returns the number of doubles the result needs (>= 0), or
+ * < 0 on error (also for an MPO not in regular form).  *terror (optional):
+ * sqrt(sum of discarded s^2) / (L-1), as MPO::Compress reports it.  Host
+ * only: needs no device. */
+int tn_mpo_reduce(int L, int d, const int* dw1, const int* dw2, const double* wcat, int form, int ops, int Dmax,
+                  double eps, int* dw1_out, int* dw2_out, double* w_out, int64_t cap, double* terror);
 
 /* ---- device-resident chain (the sweep) ---------------------------------- */
 /* Per-site MPO: dw1[i], dw2[i] and W blocks concatenated in site order.
@@ -208,6 +247,23 @@ int tn_chain_bond_sv(const tn_chain* chain, int b, double* sv, int cap);
  * for any MPO given like tn_chain_create's; FP64 left zipper on the device. */
 int tn_chain_expectation(tn_chain* chain, const int* dw1, const int* dw2, const double* wcat,
                          double* value);
+/* MPS::GetNormStatus() (TensorNetworksImp/MPS/MPSSite.C:173-211): per site
+ * 'A' left-normalised (sum_{m,i1} A^T A = 1), 'B' right-normalised, 'I' both,
+ * 'M' neither, within eps (elementwise); out receives L characters and a NUL. */
+int tn_chain_norm_status(tn_chain* chain, double eps, char* out);
+/* Checkpoint of a chain (no reference counterpart; the reference's MPS::Clone,
+ * TensorNetworksImp/MPS/MPSImp.C:38-53, copies in memory only): MPO, site
+ * tensors (reference layout), bond values and per-site energies, with
+ * flags & 1 also the FP64 environment caches, and `meta_bytes` bytes of the
+ * caller's own data (e.g. the position in a schedule).  Written to path.tmp,
+ * then renamed; a hash guards the content. */
+int tn_chain_save(tn_chain* chain, const char* path, const void* meta, int64_t meta_bytes, int flags);
+/* Recreate a chain from a checkpoint on ctx (bond capacity max(Dcap, the
+ * saved one)).  Without saved environments call tn_chain_load_caches before
+ * sweeping.  meta receives the first min(meta_cap, length) bytes of the
+ * caller's data and *meta_bytes its full length. */
+int tn_chain_load(tn_ctx* ctx, const char* path, int Dcap, tn_chain** chain, void* meta, int64_t meta_cap,
+                  int64_t* meta_bytes);
 /* Two-site update of bond (site, site+1) with the SVCompressor rule (Dmax, eps):
  * an extension (the reference sweep is one-site only). */
 int tn_chain_sweep2(tn_chain* chain, int dir, int Dmax, double eps_sv,
@@ -282,6 +338,11 @@ int tn_mlp_train(tn_ctx* ctx, float* params, const float* norm, int64_t n, const
  * iters*nsamp entries; *nout = entries written.  The chain is unchanged. */
 int tn_chain_precond_samples(tn_chain* ch, int site, int iters, int nsamp, uint64_t seed, double delta,
                              double* X, double* a, double* t, int* nout);
+
+/* Round trip of n (<= 16384) doubles host -> device -> host through the
+ * polled read-back the Lanczos iterations use (values published to host-mapped
+ * memory, then a completion flag), for a value test of that path. */
+int tn_publish_test(tn_ctx* ctx, int n, const double* in, double* out);
 
 /* Plain device copy of n doubles (read n*8 bytes, write n*8 bytes) by the
  * library's streaming kernel, once, into freshly allocated buffers; *ms = its

@@ -29,6 +29,7 @@ def _load():
     i, d, l, v, u64 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_void_p, ctypes.c_uint64
     sig = {
         "orc_version": (i, []),
+        "orc_ed_ground_mpo": (d, [i, i, _ip, _ip, _dp, d, i]),
         "orc_spin_ops": (None, [i, _dp, _dp, _dp]),
         "orc_xxz_bulk_W": (None, [i, d, d, d, d, i, _dp]),
         "orc_site_W": (None, [i, i, _dp, i, i, _dp, _ip, _ip]),
@@ -223,6 +224,15 @@ def svcompress(s, Dmax, eps):
     disc, resc = ctypes.c_double(), ctypes.c_double()
     D = lib.orc_svcompress(len(s), P(s), Dmax, eps, ctypes.byref(disc), ctypes.byref(resc))
     return D, disc.value, resc.value
+
+
+def ed_ground_mpo(Ws, tol=1e-12, maxit=300):
+    """Ground energy of an MPO (list of W[i], (Dw1, Dw2, d, d)) by Lanczos on the d^L space."""
+    dw1 = np.array([W.shape[0] for W in Ws], dtype=np.int32)
+    dw2 = np.array([W.shape[1] for W in Ws], dtype=np.int32)
+    wcat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(W, dtype=np.float64).ravel() for W in Ws]))
+    return lib.orc_ed_ground_mpo(len(Ws), Ws[0].shape[2], dw1.ctypes.data_as(_ip), dw2.ctypes.data_as(_ip), P(wcat),
+                                  tol, maxit)
 
 
 def ed_ground(L, d, Jxy=1.0, Jz=1.0, hz=0.0, Dz=0.0, tol=1e-12, maxit=300):

@@ -893,6 +893,79 @@ double orc_ed_ground(int L, int d, double Jxy, double Jz, double hz, double Dz,
     return e0;
 }
 
+/* exact diagonalisation of any MPO: y = H x site by site.  The state index
+ * has site 0 most significant (s = s_0 d^(L-1) + ... + s_(L-1)).  After site
+ * i the work tensor holds X[w][out_0..out_i][in_(i+1)..in_(L-1)]; each step
+ * contracts W_i[w][w'][m][n] over (w, n = in_i). */
+typedef struct {
+    int L, d;
+    long N;
+    const int *dw1, *dw2;
+    const double** W;
+    int maxdw;
+    double *X, *Y;
+} edm_ctx;
+
+static void edm_matvec(void* vctx, const double* x, double* y)
+{
+    edm_ctx* c = (edm_ctx*)vctx;
+    const int L = c->L, d = c->d;
+    const long N = c->N;
+    memcpy(c->X, x, sizeof(double) * N); /* w = 0 only (dw1[0] = 1) */
+    long head = 1;                       /* d^i: processed (output) sites */
+    for (int i = 0; i < L; i++) {
+        const int a = c->dw1[i], b = c->dw2[i];
+        const long tail = N / (head * d); /* remaining input sites after i */
+        const double* W = c->W[i];
+        memset(c->Y, 0, sizeof(double) * (size_t)b * N);
+        /* X[w][h][n][t] -> Y[w'][h][m][t] */
+#pragma omp parallel for schedule(static)
+        for (long h = 0; h < head; h++)
+            for (int w = 0; w < a; w++)
+                for (int w2 = 0; w2 < b; w2++)
+                    for (int m = 0; m < d; m++)
+                        for (int n = 0; n < d; n++) {
+                            const double v = W[((w * b + w2) * d + m) * d + n];
+                            if (v == 0.0) continue;
+                            const double* src = c->X + (size_t)w * N + (h * d + n) * tail;
+                            double* dst = c->Y + (size_t)w2 * N + (h * d + m) * tail;
+                            for (long t = 0; t < tail; t++) dst[t] += v * src[t];
+                        }
+        double* tmp = c->X;
+        c->X = c->Y;
+        c->Y = tmp;
+        head *= d;
+    }
+    memcpy(y, c->X, sizeof(double) * N); /* dw2[L-1] = 1 */
+}
+
+double orc_ed_ground_mpo(int L, int d, const int* dw1, const int* dw2, const double* wcat, double tol,
+                         int maxit)
+{
+    edm_ctx c;
+    c.L = L; c.d = d; c.dw1 = dw1; c.dw2 = dw2;
+    c.N = 1;
+    for (int i = 0; i < L; i++) c.N *= d;
+    c.W = (const double**)xcalloc(L, sizeof(double*));
+    c.maxdw = 1;
+    size_t off = 0;
+    for (int i = 0; i < L; i++) {
+        c.W[i] = wcat + off;
+        off += (size_t)dw1[i] * dw2[i] * d * d;
+        if (dw1[i] > c.maxdw) c.maxdw = dw1[i];
+        if (dw2[i] > c.maxdw) c.maxdw = dw2[i];
+    }
+    c.X = dalloc((size_t)c.maxdw * c.N);
+    c.Y = dalloc((size_t)c.maxdw * c.N);
+    double* x = dalloc(c.N);
+    uint64_t st = 20261202;
+    for (long i = 0; i < c.N; i++) x[i] = orc_rng_uniform(&st);
+    double e0 = 0, e1 = 0;
+    orc_lanczos(c.N, edm_matvec, &c, x, tol, maxit, &e0, &e1);
+    free(x); free(c.X); free(c.Y); free((void*)c.W);
+    return e0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* MPS + sweep: MPSImp.C, MPSSite.C, Bond.C, MPSImp_Variational.C            */
 struct orc_mps {

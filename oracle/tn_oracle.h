@@ -117,6 +117,14 @@ int orc_lanczos(long N, orc_matvec_fn mv, void* ctx, double* x0, double tol, int
 double orc_ed_ground(int L, int d, double Jxy, double Jz, double hz, double Dz,
                      double tol, int maxit);
 
+/* Ground energy of any MPO (given like the chain's: dw1, dw2, W concatenated)
+ * by Lanczos on the d^L space, the MPO applied site by site.  Pinned against
+ * orc_ed_ground on the XXZ chain, numpy eigvalsh of the contracted MPO, the
+ * free-fermion closed form of the transverse-field Ising chain and brute force
+ * over classical configurations (tests/test_oracle.py, tests/test_mpo.py). */
+double orc_ed_ground_mpo(int L, int d, const int* dw1, const int* dw2, const double* wcat, double tol,
+                         int maxit);
+
 /* ---- the sweep (MPSImp_Variational.C:16-173) ---------------------------- */
 typedef struct orc_mps orc_mps;
 /* MPSImp.C:69-90 shapes; init 0=Product 1=Random(seeded U(-1,1)) 2=Neel. */

@@ -17,13 +17,17 @@ from ._abi import (  # noqa: F401
     SweepStats,
     TnError,
     lib,
+    mpo_reduce,
+    read_checkpoint_mpo,
     svcompress_rank,
 )
-from .models import xxz_mpo, mpo_square, identity_mpo  # noqa: F401
+from .models import (  # noqa: F401
+    xxz_mpo, tfim_mpo, long_range_mpo, three_body_mpo, mpo_square, identity_mpo, mpo_contract)
 from .dmrg import FindVariationalGroundState, ScheduleLine  # noqa: F401
 
 __all__ = [
     "Context", "Chain", "Batch", "SweepOpts", "SweepStats", "TnError", "lib", "LIB_PATH",
-    "TN_FP64", "TN_FP32", "TN_BF16", "MLP_NP", "svcompress_rank", "xxz_mpo", "mpo_square",
-    "identity_mpo", "FindVariationalGroundState", "ScheduleLine",
+    "TN_FP64", "TN_FP32", "TN_BF16", "MLP_NP", "svcompress_rank", "mpo_reduce", "read_checkpoint_mpo",
+    "xxz_mpo", "tfim_mpo", "long_range_mpo", "three_body_mpo", "mpo_square", "identity_mpo", "mpo_contract",
+    "FindVariationalGroundState", "ScheduleLine",
 ]

@@ -138,6 +138,13 @@ def _load():
         "tn_chain_bond_sv": (i, [v, i, _dp, i]),
         "tn_chain_expectation": (i, [v, _ip, _ip, _dp, _dp]),
         "tn_chain_sweep2": (i, [v, i, i, d, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
+        "tn_heff_apply_device": (i, [v, v, v, i64]),
+        "tn_qr": (i, [v, i, i, i, _dp, _dp, _dp]),
+        "tn_publish_test": (i, [v, i, _dp, _dp]),
+        "tn_chain_norm_status": (i, [v, d, ctypes.c_char_p]),
+        "tn_chain_save": (i, [v, ctypes.c_char_p, ctypes.c_char_p, i64, i]),
+        "tn_chain_load": (i, [v, ctypes.c_char_p, i, vp, ctypes.c_char_p, i64, ctypes.POINTER(ctypes.c_int64)]),
+        "tn_mpo_reduce": (i, [i, i, _ip, _ip, _dp, i, i, i, d, _ip, _ip, _dp, i64, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -286,12 +293,42 @@ class Context:
         return F
 
     def eig_lowest(self, x0, tol=1e-12, maxit=200, nev=2):
-        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        """Lowest nev (1 or 2) eigenpairs of the bound operator from start
+        vector x0: returns (evals, X, n_matvec, rc) with X[:, k] the
+        eigenvector of evals[k] (X is 1-D, the first vector, when nev == 1)."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64).ravel()
+        N = x0.size
+        x = np.zeros(nev * N)
+        x[:N] = x0
         ev = np.zeros(2)
         nmv = ctypes.c_int()
-        rc = self.check(lib.tn_eig_lowest(self.h, x.size, nev, tol, maxit, P(ev), P(x), ctypes.byref(nmv)),
+        rc = self.check(lib.tn_eig_lowest(self.h, N, nev, tol, maxit, P(ev), P(x), ctypes.byref(nmv)),
                         "tn_eig_lowest")
-        return ev[:nev], x, nmv.value, rc
+        X = x[:N].copy() if nev == 1 else x.reshape(nev, N).T.copy()
+        return ev[:nev], X, nmv.value, rc
+
+    def heff_apply_device(self, x_ptr, y_ptr, n):
+        """y = Heff x with x, y device pointers (reference layout, n doubles)."""
+        self.check(lib.tn_heff_apply_device(self.h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), int(n)),
+                   "tn_heff_apply_device")
+
+    def qr(self, A, mode=0):
+        """Thin QR (mode 0, m >= n: A = Q R) or LQ (mode 1, m <= n: A = L Q) on
+        the device; returns (Q, R or L, rc)."""
+        A = np.asarray(A, dtype=np.float64)
+        m, n = A.shape
+        Af = np.ascontiguousarray(A.T).ravel()
+        k = n if mode == 0 else m
+        Q = np.zeros(m * n)
+        R = np.zeros(k * k)
+        rc = self.check(lib.tn_qr(self.h, int(mode), m, n, P(Af), P(Q), P(R)), "tn_qr")
+        return Q.reshape(n, m).T.copy(), R.reshape(k, k).T.copy(), rc
+
+    def publish_test(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.check(lib.tn_publish_test(self.h, x.size, P(x), P(y)), "tn_publish_test")
+        return y
 
     def svd(self, A):
         A = np.asarray(A, dtype=np.float64)
@@ -330,6 +367,50 @@ def _mpo_arrays(Ws):
     dw2 = np.array([W.shape[1] for W in Ws], dtype=np.int32)
     wcat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(W, dtype=np.float64).ravel() for W in Ws]))
     return dw1, dw2, wcat
+
+
+def read_checkpoint_mpo(path):
+    """The MPO stored in a chain checkpoint (list of W[i], (Dw1, Dw2, d, d))."""
+    with open(path, "rb") as f:
+        head = f.read(8)
+        if head != b"TNCHKPT1":
+            raise TnError(-2, f"{path}: not a chain checkpoint")
+        version, L, d, _dcap, _flags = np.frombuffer(f.read(20), dtype="<i4")
+        if version != 1:
+            raise TnError(-2, f"{path}: unknown checkpoint version {version}")
+        dw = np.frombuffer(f.read(8 * L), dtype="<i4").reshape(2, L)
+        Ws = []
+        for i in range(L):
+            n = int(dw[0, i] * dw[1, i] * d * d)
+            Ws.append(np.frombuffer(f.read(8 * n), dtype="<f8").reshape(dw[0, i], dw[1, i], d, d).copy())
+    return Ws
+
+
+def mpo_reduce(Ws, form=0, ops=3, Dmax=0, eps=1e-13):
+    """MPO::CanonicalForm (ops & 1) and MPO::Compress(Parker, Dmax, eps)
+    (ops & 2) of a regular MPO (form 0 upper, 1 lower) through tn_mpo_reduce;
+    returns (list of W[i], truncation error)."""
+    dw1, dw2, wcat = _mpo_arrays(Ws)
+    L = len(Ws)
+    d = Ws[0].shape[2]
+    o1 = np.zeros(L, dtype=np.int32)
+    o2 = np.zeros(L, dtype=np.int32)
+    te = ctypes.c_double(0.0)
+    need = lib.tn_mpo_reduce(L, d, IP(dw1), IP(dw2), P(wcat), form, ops, Dmax, eps, IP(o1), IP(o2), None, 0,
+                             ctypes.byref(te))
+    if need < 0:
+        raise TnError(need, "tn_mpo_reduce: bad MPO (not in regular form?)")
+    out = np.zeros(max(need, 1))
+    got = lib.tn_mpo_reduce(L, d, IP(dw1), IP(dw2), P(wcat), form, ops, Dmax, eps, IP(o1), IP(o2), P(out), out.size,
+                            ctypes.byref(te))
+    if got != need:
+        raise TnError(-3, "tn_mpo_reduce: size changed between calls")
+    Wo, off = [], 0
+    for i in range(L):
+        n = int(o1[i] * o2[i] * d * d)
+        Wo.append(out[off:off + n].reshape(o1[i], o2[i], d, d).copy())
+        off += n
+    return Wo, te.value
 
 
 class Chain:
@@ -447,6 +528,41 @@ class Chain:
                                                env_precision, *[ctypes.byref(x) for x in vals]),
                        "tn_chain_heff_bench")
         return [x.value for x in vals]
+
+    def norm_status(self, eps=1e-12):
+        """MPS::GetNormStatus: one of 'A', 'B', 'I', 'M' per site."""
+        buf = ctypes.create_string_buffer(self.L + 1)
+        self.ctx.check(lib.tn_chain_norm_status(self.h, float(eps), buf), "tn_chain_norm_status")
+        return buf.value.decode()
+
+    def save(self, path, meta=b"", environments=False):
+        """Checkpoint (tn_chain_save): MPO, sites, bond values, per-site
+        energies, optionally the FP64 environments, and `meta` bytes."""
+        meta = bytes(meta)
+        self.ctx.check(lib.tn_chain_save(self.h, os.fsencode(path), meta, len(meta), 1 if environments else 0),
+                       "tn_chain_save")
+
+    @classmethod
+    def load(cls, ctx, path, Dcap=0):
+        """Chain from a checkpoint; returns (chain, meta bytes)."""
+        Ws = read_checkpoint_mpo(path)
+        h = ctypes.c_void_p()
+        nb = ctypes.c_int64(0)
+        buf = ctypes.create_string_buffer(1 << 20)
+        ctx.check(lib.tn_chain_load(ctx.h, os.fsencode(path), int(Dcap), ctypes.byref(h), buf, len(buf),
+                                    ctypes.byref(nb)), "tn_chain_load")
+        if nb.value > len(buf):
+            lib.tn_chain_destroy(h)
+            raise TnError(-2, f"{path}: checkpoint meta data larger than {len(buf)} bytes")
+        ch = cls.__new__(cls)
+        ch.ctx = ctx
+        ch.Ws = Ws
+        ch.L = len(Ws)
+        ch.d = Ws[0].shape[2]
+        ch._mpo = _mpo_arrays(Ws)
+        ch.h = h
+        ctx._chains.add(ch)
+        return ch, buf.raw[:nb.value]
 
     def expectation(self, Ws=None):
         dw1, dw2, wcat = self._mpo if Ws is None else _mpo_arrays(Ws)

@@ -34,6 +34,9 @@ HIP_SOURCES = ["hip/tn_kernels.hip", "hip/tn_engine.hip", "hip/tn_chain.hip", "h
 HIP_HEADERS = ["hip/tn_kernels.h", "hip/tn_engine.h", "hip/tn_chain.h"]
 CXX_SOURCES = ["host/tensornetworks.cpp"]
 CXX_HEADERS = []
+# host-only C++ linked into the ABI library (no device code)
+ABI_HOST_SOURCES = ["host/tn_mpo.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE]
 
 
 def _digest(paths, flags):
@@ -63,11 +66,28 @@ def _compile_hip(src):
     return obj
 
 
+def _compile_host(src):
+    path = os.path.join(CSRC, src)
+    deps = [path, os.path.join(INCLUDE, "tn_abi.h")]
+    obj = os.path.join(BUILD, os.path.basename(src) + "." + _digest(deps, HOST_FLAGS) + ".o")
+    if not os.path.exists(obj):
+        r = subprocess.run(["g++"] + HOST_FLAGS + ["-c", path, "-o", obj], capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed on {src}")
+        base = os.path.basename(src) + "."
+        for f in os.listdir(BUILD):
+            if f.startswith(base) and os.path.join(BUILD, f) != obj:
+                os.remove(os.path.join(BUILD, f))
+    return obj
+
+
 def build(verbose=True):
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(_compile_hip, HIP_SOURCES))
+    objs += [_compile_host(s) for s in ABI_HOST_SOURCES]
     lib = os.path.join(LIBDIR, "libtn_hip.so")
     tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs

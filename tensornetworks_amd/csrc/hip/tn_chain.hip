@@ -690,6 +690,12 @@ int tn_ctx_tune_load(tn_ctx* c, const char* path)
     long v[11];
     while (fscanf(f, "%ld %ld %ld %ld %ld %ld %ld %ld %ld %ld %ld", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
                   &v[7], &v[8], &v[9], &v[10]) == 11) {
+        // a malformed line is skipped: element types in {f64, f32, bf16},
+        // layout and accumulate flags 0/1, positive shape, a known tile and k-split
+        const bool types_ok = v[0] >= 0 && v[0] <= 2 && v[1] >= 0 && v[1] <= 2;
+        const bool flags_ok = (v[2] == 0 || v[2] == 1) && (v[3] == 0 || v[3] == 1) && (v[8] == 0 || v[8] == 1);
+        const bool shape_ok = v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] > 0;
+        if (!types_ok || !flags_ok || !shape_ok) continue;
         if (v[9] < 1 || v[9] > gemm_num_tiles() || (v[10] != 1 && v[10] != 2 && v[10] != 4)) continue;
         const std::array<long, 9> key{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
         c->tune[key] = {(int)v[9], (int)v[10]};
@@ -726,7 +732,7 @@ int tn_ctx_destroy(tn_ctx* c)
                         &c->svd_flag, &c->svd_A, &c->spare, &c->pack, &c->qr_W, &c->qr_G, &c->qr_L,
                         &c->qr_T, &c->qr_linv, &c->qr_Q, &c->tune_C, &c->splitk, &c->b_L, &c->b_R,
                         &c->b_Llo, &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M, &c->dav_W, &c->dav_vec,
-                        &c->dav_wd, &c->mlp, &c->ident};
+                        &c->dav_wd, &c->mlp, &c->ident, &c->defl};
     for (auto* b : bufs) b->release();
     c->b_mix_heff.buf.release();
     c->b_mix_envl.buf.release();
@@ -948,6 +954,41 @@ static void bound_cb(tn_ctx* c, const void* vop, const double* x, double* y)
     heff_apply(c, *(const HeffOp*)vop, x, y, false);
 }
 
+// H' = H + sigma x0 x0^T: the lowest eigenpair of H' is the second of H when
+// sigma exceeds lambda_2 - lambda_1 (x0 the lowest eigenvector, unit norm)
+struct DeflatedOp {
+    const HeffOp* op;
+    const double* x0;
+    double sigma;
+    double* dot;
+    double* part;
+    int pcap;
+    long N;
+};
+static void deflated_cb(tn_ctx* c, const void* vop, const double* x, double* y)
+{
+    const DeflatedOp* d = (const DeflatedOp*)vop;
+    heff_apply(c, *d->op, x, y, false);
+    TN_HIP(mdot(d->x0, d->N, 1, x, d->N, d->dot, d->part, d->pcap, c->stream));
+    TN_HIP(gemv_acc(d->x0, d->N, 1, d->dot, d->sigma, 1.0, y, d->N, c->stream));
+}
+
+// seeded U(-1,1) start vector over the logical entries (reference layout);
+// the padding of the internal layout stays zero, so the operator's null
+// space on the padding never enters the Krylov space
+static void random_start(double* v, long n, uint64_t seed)
+{
+    uint64_t st = seed;
+    for (long i = 0; i < n; i++) {
+        st += 0x9e3779b97f4a7c15ULL;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        z ^= z >> 31;
+        v[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    }
+}
+
 int tn_eig_lowest(tn_ctx* c, int64_t N, int nev, double tol, int maxit, double* evals, double* evec,
                   int* n_matvec)
 {
@@ -956,15 +997,136 @@ int tn_eig_lowest(tn_ctx* c, int64_t N, int nev, double tol, int maxit, double* 
     TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_eig_lowest: no site bound");
     TN_REQUIRE(N == (int64_t)c->b_d * c->b_D1 * c->b_D2 && evals && evec && nev >= 1 && nev <= 2 && maxit >= 1,
                TN_ERR_ARG, "tn_eig_lowest: bad arguments");
+    TN_REQUIRE(nev == 1 || N >= 2, TN_ERR_ARG, "tn_eig_lowest: a second eigenpair needs N >= 2");
     HeffOp op = bound_op(c);
-    psi_to_dev(c, evec, c->b_x.as<double>());
-    LanczosResult r = lanczos(c, bound_cb, &op, op.n(), c->b_x.as<double>(), tol, maxit, 4);
+    const long n = op.n();
+    std::vector<double> start(evec, evec + N);
+    bool any_nonzero = false, any_bad = false;
+    for (long i = 0; i < N; i++) {
+        if (!std::isfinite(start[i])) any_bad = true;
+        else if (start[i] != 0.0) any_nonzero = true;
+    }
+    if (any_bad || !any_nonzero) random_start(start.data(), N, 0x5eedULL);
+    psi_to_dev(c, start.data(), c->b_x.as<double>());
+    LanczosResult r = lanczos(c, bound_cb, &op, n, c->b_x.as<double>(), tol, maxit, 4);
+    int nmv = r.n_matvec;
+    bool conv = r.converged;
     sync(c);
     psi_from_dev(c, c->b_x.as<double>(), evec);
     evals[0] = r.e0;
-    if (nev > 1) evals[1] = r.e1;
-    if (n_matvec) *n_matvec = r.n_matvec;
-    return r.converged ? TN_OK : TN_WARN_NOT_CONVERGED;
+    if (nev > 1) {
+        // second eigenpair (LapackEigenSolver.C:62-109 returns IL=1..IU=2): Lanczos
+        // on the deflated operator from a start orthogonal to x0 up to rounding;
+        // the shift is raised if the solve lands on the deflated vector
+        const long pcap = mdot_part_size(n, 1);
+        c->defl.ensure((size_t)(pcap + 8) * 8);
+        DBuf x1;
+        x1.ensure((size_t)n * 8);
+        std::vector<double> s1(N);
+        random_start(s1.data(), N, 0x5eed2ULL);
+        double sigma = 2.0 * std::max(r.e1 - r.e0, 0.0) + 2.0 * std::fabs(r.e0) + 1.0;
+        LanczosResult r2;
+        for (int attempt = 0; attempt < 4; attempt++) {
+            DeflatedOp dop{&op, c->b_x.as<double>(), sigma, c->defl.as<double>(), c->defl.as<double>() + 8, (int)pcap,
+                           n};
+            psi_to_dev(c, s1.data(), x1.as<double>());
+            r2 = lanczos(c, deflated_cb, &dop, n, x1.as<double>(), tol, maxit, 4);
+            nmv += r2.n_matvec;
+            // overlap with x0: a large one means the shift did not lift x0 above lambda_2
+            TN_HIP(mdot(c->b_x.as<double>(), n, 1, x1.as<double>(), n, c->defl.as<double>(), c->defl.as<double>() + 8,
+                        (int)pcap, c->stream));
+            double ov = 0;
+            d2h(c, &ov, c->defl.p, 8);
+            if (std::fabs(ov) < 1e-3) break;
+            sigma *= 10.0;
+        }
+        conv = conv && r2.converged;
+        sync(c);
+        psi_from_dev(c, x1.as<double>(), evec + N);
+        x1.release();
+        evals[1] = r2.e0;
+    }
+    if (n_matvec) *n_matvec = nmv;
+    return conv ? TN_OK : TN_WARN_NOT_CONVERGED;
+    ABI_END(c)
+}
+
+int tn_heff_apply_device(tn_ctx* c, const double* x, double* y, int64_t N)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_heff_apply_device: no site bound");
+    TN_REQUIRE(x && y && N == (int64_t)c->b_d * c->b_D1 * c->b_D2, TN_ERR_ARG,
+               "tn_heff_apply_device: N != d*D1*D2");
+    const int d = c->b_d, D1 = c->b_D1, D2 = c->b_D2, D1p = pad32(D1), D2p = pad32(D2);
+    TN_HIP(psi_ref_to_int(x, c->b_x.as<double>(), d, D1, D2, D1p, D2p, c->stream));
+    HeffOp op = bound_op(c);
+    heff_apply(c, op, c->b_x.as<double>(), c->b_y.as<double>(), false);
+    TN_HIP(psi_int_to_ref(c->b_y.as<double>(), y, d, D1, D2, D2p, c->stream));
+    sync(c);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_qr(tn_ctx* c, int mode, int m, int n, const double* A, double* Q, double* R)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(A && Q && R && m > 0 && n > 0 && (mode == 0 || mode == 1), TN_ERR_ARG, "tn_qr: bad arguments");
+    TN_REQUIRE(mode == 0 ? m >= n : m <= n, TN_ERR_ARG,
+               mode == 0 ? "tn_qr: the QR form needs m >= n (use the LQ form)"
+                         : "tn_qr: the LQ form needs m <= n (use the QR form)");
+    // X (rows x cols, row-major, padded) = A (QR) or A^T (LQ); X = Q' Lacc^T
+    const int rows = mode == 0 ? m : n, cols = mode == 0 ? n : m;
+    const int rp = pad32(rows), cp = pad32(cols);
+    TN_REQUIRE(rp >= cp, TN_ERR_ARG, "tn_qr: padded shapes");
+    std::vector<double> h((size_t)rp * cp, 0.0);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            const double v = A[i + (size_t)j * m];
+            if (mode == 0) h[(size_t)i * cp + j] = v;
+            else h[(size_t)j * cp + i] = v;
+        }
+    DBuf a, q, l;
+    a.ensure(h.size() * 8);
+    q.ensure(h.size() * 8);
+    l.ensure((size_t)cp * cp * 8);
+    h2d(c, a.p, h.data(), h.size() * 8);
+    int sh = 0;
+    const bool ok = cholqr_device(c, a.as<double>(), cp, rp, cp, cols, q.as<double>(), l.as<double>(), 1e-11, &sh, true);
+    d2h(c, h.data(), q.p, h.size() * 8);
+    std::vector<double> hl((size_t)cp * cp);
+    d2h(c, hl.data(), l.p, hl.size() * 8);
+    a.release();
+    q.release();
+    l.release();
+    if (mode == 0) {
+        // Q (m x n) column-major, R (n x n) = Lacc^T upper, column-major
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < n; j++) Q[i + (size_t)j * m] = h[(size_t)i * cp + j];
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) R[i + (size_t)j * n] = hl[(size_t)j * cp + i];
+    } else {
+        // A = L Q: Q (m x n) = Q'^T, L (m x m) = Lacc lower, column-major
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < n; j++) Q[i + (size_t)j * m] = h[(size_t)j * cp + i];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) R[i + (size_t)j * m] = hl[(size_t)i * cp + j];
+    }
+    return ok ? TN_OK : TN_WARN_NOT_CONVERGED;
+    ABI_END(c)
+}
+
+int tn_publish_test(tn_ctx* c, int n, const double* in, double* out)
+{
+    if (!c || !in || !out || n < 1 || n > TN_PUBLISH_MAX) return TN_ERR_ARG;
+    ABI_BEGIN
+    DBuf b;
+    b.ensure((size_t)n * 8);
+    h2d(c, b.p, in, (size_t)n * 8);
+    d2h_small(c, out, b.as<double>(), n);
+    b.release();
+    return TN_OK;
     ABI_END(c)
 }
 
@@ -1309,6 +1471,267 @@ int tn_chain_bond_sv(const tn_chain* ch, int b, double* sv, int cap)
     int n = (int)l.size();
     for (int k = 0; k < n && k < cap; k++) sv[k] = l[k];
     return n;
+}
+
+int tn_chain_norm_status(tn_chain* ch, double eps, char* out)
+{
+    if (!ch || !out || !(eps > 0.0)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    // MPSSite.C:173-211: 'A' sum_{m,a} A*A = 1, 'B' sum_{m,b} A A* = 1, 'I' both, 'M' neither
+    const int d = ch->d;
+    DBuf g, dev, part;
+    g.ensure((size_t)ch->Dcapp * ch->Dcapp * 8);
+    dev.ensure(2 * 8);
+    part.ensure(1024 * 8);
+    for (int i = 0; i < ch->L; i++) {
+        const int D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+        GemmDesc gl{};
+        gl.tin = T_F64; gl.tout = T_F64; gl.a_kmajor = false; gl.b_kmajor = false;
+        gl.M = D2p; gl.N = D2p; gl.K = D1p * d; gl.batch = 1;
+        gl.A = ch->A[i]; gl.lda = D2p; gl.B = ch->A[i]; gl.ldb = D2p; gl.C = g.p; gl.ldc = D2p; gl.alpha = 1.0f;
+        TN_HIP(gemm(gl, c->stream));
+        TN_HIP(eye_deviation(g.as<double>(), D2p, D2, dev.as<double>(), part.as<double>(), c->stream));
+        GemmDesc gr{};
+        gr.tin = T_F64; gr.tout = T_F64; gr.a_kmajor = true; gr.b_kmajor = true;
+        gr.M = D1p; gr.N = D1p; gr.K = d * D2p; gr.batch = 1;
+        gr.A = ch->A[i]; gr.lda = (long)d * D2p; gr.B = ch->A[i]; gr.ldb = (long)d * D2p; gr.C = g.p; gr.ldc = D1p;
+        gr.alpha = 1.0f;
+        TN_HIP(gemm(gr, c->stream));
+        TN_HIP(eye_deviation(g.as<double>(), D1p, D1, dev.as<double>() + 1, part.as<double>(), c->stream));
+        double e[2];
+        d2h(c, e, dev.p, 16);
+        const bool A = e[0] < eps, B = e[1] < eps;
+        out[i] = (A && B) ? 'I' : (A ? 'A' : (B ? 'B' : 'M'));
+    }
+    out[ch->L] = 0;
+    g.release();
+    dev.release();
+    part.release();
+    return TN_OK;
+    ABI_END(c)
+}
+
+// ---- checkpoint / resume ------------------------------------------------------
+// File layout (little-endian): "TNCHKPT1", int32 version, L, d, Dcap, flags,
+// dw1[L], dw2[L], float64 W (site order); per site int32 D1, D2 and the site
+// tensor in the reference layout; per bond int32 n, lambda[n], maxDelta,
+// compErr; per site Emin, IterDE, GapE; with flags & 1 the FP64 environments
+// (left env after site i: D2 x Dw2 x D2 as [bra][w][ket]; right env from site i:
+// Dw1 x D1 x D1 as [w][bra][ket]); int64 meta length and the caller's meta
+// bytes; uint64 FNV-1a hash of everything before it.
+namespace {
+struct Fnv {
+    uint64_t h = 1469598103934665603ULL;
+    void add(const void* p, size_t n)
+    {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; i++) {
+            h ^= b[i];
+            h *= 1099511628211ULL;
+        }
+    }
+};
+struct Writer {
+    FILE* f;
+    Fnv h;
+    void put(const void* p, size_t n)
+    {
+        TN_REQUIRE(fwrite(p, 1, n, f) == n, TN_ERR_STATE, "tn_chain_save: write failed");
+        h.add(p, n);
+    }
+    void i32(int v) { put(&v, 4); }
+    void f64(const double* p, size_t n) { put(p, n * 8); }
+};
+struct Reader {
+    FILE* f;
+    Fnv h;
+    void get(void* p, size_t n)
+    {
+        TN_REQUIRE(fread(p, 1, n, f) == n, TN_ERR_ARG, "tn_chain_load: truncated file");
+        h.add(p, n);
+    }
+    int i32()
+    {
+        int v;
+        get(&v, 4);
+        return v;
+    }
+    void f64(double* p, size_t n) { get(p, n * 8); }
+};
+struct FileCloser {
+    FILE* f;
+    ~FileCloser()
+    {
+        if (f) fclose(f);
+    }
+};
+const char kCkptMagic[8] = {'T', 'N', 'C', 'H', 'K', 'P', 'T', '1'};
+}  // namespace
+
+int tn_chain_save(tn_chain* ch, const char* path, const void* meta, int64_t meta_bytes, int flags)
+{
+    if (!ch || !path || meta_bytes < 0 || (meta_bytes > 0 && !meta) || (flags & ~1)) return TN_ERR_ARG;
+    tn_ctx* c = ch->ctx;
+    ABI_BEGIN
+    const std::string tmp = std::string(path) + ".tmp";
+    FileCloser fc{fopen(tmp.c_str(), "wb")};
+    TN_REQUIRE(fc.f, TN_ERR_ARG, std::string("tn_chain_save: cannot open ") + tmp);
+    Writer w{fc.f};
+    const int L = ch->L, d = ch->d;
+    w.put(kCkptMagic, 8);
+    w.i32(1);
+    w.i32(L);
+    w.i32(d);
+    w.i32(ch->Dcap);
+    w.i32(flags);
+    for (int i = 0; i < L; i++) w.i32(ch->dw1[i]);
+    for (int i = 0; i < L; i++) w.i32(ch->dw2[i]);
+    for (int i = 0; i < L; i++) w.f64(ch->W[i].data(), ch->W[i].size());
+    std::vector<double> M;
+    for (int i = 0; i < L; i++) {
+        w.i32(ch->D1[i]);
+        w.i32(ch->D2[i]);
+        download_site(ch, i, M);
+        w.f64(M.data(), M.size());
+    }
+    for (int b = 0; b < L - 1; b++) {
+        w.i32((int)ch->lam[b].size());
+        w.f64(ch->lam[b].data(), ch->lam[b].size());
+        w.f64(&ch->maxDelta[b], 1);
+        w.f64(&ch->compErr[b], 1);
+    }
+    for (int i = 0; i < L; i++) {
+        w.f64(&ch->Emin[i], 1);
+        w.f64(&ch->IterDE[i], 1);
+        w.f64(&ch->GapE[i], 1);
+    }
+    if (flags & 1) {
+        for (int i = 0; i < L; i++) {
+            // logical blocks only: left [D2][Dw2][D2], right [Dw1][D1][D1]
+            const int D1 = ch->D1[i], D2 = ch->D2[i], D1p = pad32(D1), D2p = pad32(D2);
+            const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
+            std::vector<double> buf((size_t)std::max(D2p * Dw2 * D2p, Dw1 * D1p * D1p));
+            std::vector<double> out;
+            d2h(c, buf.data(), ch->Lenv[i], (size_t)D2p * Dw2 * D2p * 8);
+            out.clear();
+            for (int a = 0; a < D2; a++)
+                for (int x = 0; x < Dw2; x++)
+                    for (int b = 0; b < D2; b++) out.push_back(buf[((size_t)a * Dw2 + x) * D2p + b]);
+            w.f64(out.data(), out.size());
+            d2h(c, buf.data(), ch->Renv[i], (size_t)Dw1 * D1p * D1p * 8);
+            out.clear();
+            for (int x = 0; x < Dw1; x++)
+                for (int a = 0; a < D1; a++)
+                    for (int b = 0; b < D1; b++) out.push_back(buf[((size_t)x * D1p + a) * D1p + b]);
+            w.f64(out.data(), out.size());
+        }
+    }
+    const int64_t mb = meta_bytes;
+    w.put(&mb, 8);
+    if (mb > 0) w.put(meta, (size_t)mb);
+    const uint64_t hash = w.h.h;
+    TN_REQUIRE(fwrite(&hash, 1, 8, fc.f) == 8, TN_ERR_STATE, "tn_chain_save: write failed");
+    TN_REQUIRE(fclose(fc.f) == 0, TN_ERR_STATE, "tn_chain_save: close failed");
+    fc.f = nullptr;
+    TN_REQUIRE(std::rename(tmp.c_str(), path) == 0, TN_ERR_STATE, std::string("tn_chain_save: cannot rename to ") + path);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_chain_load(tn_ctx* c, const char* path, int Dcap, tn_chain** out, void* meta, int64_t meta_cap,
+                  int64_t* meta_bytes)
+{
+    if (!c || !path || !out || meta_cap < 0 || (meta_cap > 0 && !meta)) return TN_ERR_ARG;
+    *out = nullptr;
+    tn_chain* ch = nullptr;
+    ABI_BEGIN
+    FileCloser fc{fopen(path, "rb")};
+    TN_REQUIRE(fc.f, TN_ERR_ARG, std::string("tn_chain_load: cannot open ") + path);
+    Reader r{fc.f};
+    char magic[8];
+    r.get(magic, 8);
+    TN_REQUIRE(std::memcmp(magic, kCkptMagic, 8) == 0, TN_ERR_ARG, "tn_chain_load: not a chain checkpoint");
+    TN_REQUIRE(r.i32() == 1, TN_ERR_ARG, "tn_chain_load: unknown version");
+    const int L = r.i32(), d = r.i32(), fDcap = r.i32(), flags = r.i32();
+    TN_REQUIRE(L >= 1 && L <= (1 << 20) && d >= 2 && d <= 64 && fDcap >= 1 && (flags & ~1) == 0, TN_ERR_ARG,
+               "tn_chain_load: bad header");
+    std::vector<int> dw1(L), dw2(L);
+    for (int i = 0; i < L; i++) dw1[i] = r.i32();
+    for (int i = 0; i < L; i++) dw2[i] = r.i32();
+    size_t nw = 0;
+    for (int i = 0; i < L; i++) {
+        TN_REQUIRE(dw1[i] >= 1 && dw2[i] >= 1 && dw1[i] <= 4096 && dw2[i] <= 4096, TN_ERR_ARG,
+                   "tn_chain_load: bad MPO dims");
+        nw += (size_t)dw1[i] * dw2[i] * d * d;
+    }
+    std::vector<double> wcat(nw);
+    r.f64(wcat.data(), nw);
+    std::vector<int> D1(L), D2(L);
+    std::vector<std::vector<double>> M(L);
+    int Dmax = 1;
+    for (int i = 0; i < L; i++) {
+        D1[i] = r.i32();
+        D2[i] = r.i32();
+        TN_REQUIRE(D1[i] >= 1 && D2[i] >= 1 && D1[i] <= fDcap && D2[i] <= fDcap, TN_ERR_ARG,
+                   "tn_chain_load: bad site dims");
+        Dmax = std::max(Dmax, std::max(D1[i], D2[i]));
+        M[i].resize((size_t)d * D1[i] * D2[i]);
+        r.f64(M[i].data(), M[i].size());
+    }
+    const int cap = std::max(std::max(Dcap, fDcap), Dmax);
+    int rc = tn_chain_create(c, L, d, dw1.data(), dw2.data(), wcat.data(), Dmax, cap, &ch);
+    if (rc != TN_OK) return rc;
+    for (int i = 0; i < L; i++) upload_site(ch, i, D1[i], D2[i], M[i].data());
+    for (int b = 0; b < L - 1; b++) {
+        const int n = r.i32();
+        TN_REQUIRE(n >= 0 && n <= fDcap, TN_ERR_ARG, "tn_chain_load: bad bond size");
+        ch->lam[b].resize(n);
+        r.f64(ch->lam[b].data(), n);
+        r.f64(&ch->maxDelta[b], 1);
+        r.f64(&ch->compErr[b], 1);
+    }
+    for (int i = 0; i < L; i++) {
+        r.f64(&ch->Emin[i], 1);
+        r.f64(&ch->IterDE[i], 1);
+        r.f64(&ch->GapE[i], 1);
+    }
+    if (flags & 1) {
+        for (int i = 0; i < L; i++) {
+            const int a1 = D1[i], a2 = D2[i], D1p = pad32(a1), D2p = pad32(a2), w1 = dw1[i], w2 = dw2[i];
+            std::vector<double> in((size_t)std::max(a2 * w2 * a2, w1 * a1 * a1));
+            std::vector<double> buf((size_t)D2p * w2 * D2p, 0.0);
+            r.f64(in.data(), (size_t)a2 * w2 * a2);
+            for (int a = 0; a < a2; a++)
+                for (int x = 0; x < w2; x++)
+                    for (int b = 0; b < a2; b++) buf[((size_t)a * w2 + x) * D2p + b] = in[((size_t)a * w2 + x) * a2 + b];
+            h2d(c, ch->Lenv[i], buf.data(), buf.size() * 8);
+            buf.assign((size_t)w1 * D1p * D1p, 0.0);
+            r.f64(in.data(), (size_t)w1 * a1 * a1);
+            for (int x = 0; x < w1; x++)
+                for (int a = 0; a < a1; a++)
+                    for (int b = 0; b < a1; b++) buf[((size_t)x * D1p + a) * D1p + b] = in[((size_t)x * a1 + a) * a1 + b];
+            h2d(c, ch->Renv[i], buf.data(), buf.size() * 8);
+        }
+    }
+    int64_t mb = 0;
+    r.get(&mb, 8);
+    TN_REQUIRE(mb >= 0 && mb <= ((int64_t)1 << 30), TN_ERR_ARG, "tn_chain_load: bad meta length");
+    std::vector<char> mbuf((size_t)mb);
+    if (mb > 0) r.get(mbuf.data(), (size_t)mb);
+    const uint64_t expect = r.h.h;
+    uint64_t hash = 0;
+    TN_REQUIRE(fread(&hash, 1, 8, fc.f) == 8 && hash == expect, TN_ERR_ARG, "tn_chain_load: checksum mismatch");
+    if (mb > 0 && meta_cap > 0) std::memcpy(meta, mbuf.data(), (size_t)std::min(mb, meta_cap));
+    if (meta_bytes) *meta_bytes = mb;
+    *out = ch;
+    return TN_OK;
+    }
+    catch (const TnError& e) {
+        int code = fail(c, e);
+        if (ch) tn_chain_destroy(ch);
+        return code;
+    }
 }
 
 int tn_chain_expectation(tn_chain* ch, const int* dw1, const int* dw2, const double* wcat, double* value)

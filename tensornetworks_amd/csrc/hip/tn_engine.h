@@ -95,6 +95,8 @@ struct tn_ctx {
     tn::DBuf dav_W, dav_vec, dav_wd, mlp;
     // identity-channel check of the environments (4 device doubles)
     tn::DBuf ident;
+    // deflation of the lowest eigenvector for the second eigenpair (dot + partials)
+    tn::DBuf defl;
     // stats whose Heff event timings are recorded but not yet read
     tn_sweep_stats* heff_pending = nullptr;
     // seam state (tn_site_bind)
@@ -140,9 +142,27 @@ struct HeffOp {
     // flops of the two GEMM stages as launched (identity channels excluded)
     double flops_gemm(int D1, int D2) const;
 };
-// y = Heff x (fp64 in/out).  gemm_ms/total_ms accumulate device times when
-// timing != nullptr (events recorded; read after the next sync).
-void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed);
+// Work space of one Heff apply supplied by the caller.  With it the apply
+// allocates nothing: the stage buffers come from here and every GEMM runs in
+// one launch with the tile of the tuning table (or the built-in choice),
+// never a k-split and never a tuning run.
+struct HeffScratch {
+    void* T1;
+    size_t T1_bytes;
+    void* T2;
+    size_t T2_bytes;
+    void* xlo;  // reduced-precision copy of x (unused in FP64)
+    size_t xlo_bytes;
+};
+// bytes of each part (256-byte multiples) and their sum
+size_t heff_scratch_bytes(const HeffOp& op, size_t* t1, size_t* t2, size_t* xlo);
+// GEMM with the table's tile (or the built-in one), one launch, no tuning
+void gemm_run_fixed(tn_ctx* c, const GemmDesc& g);
+// y = Heff x (fp64 in/out).  timed: HIP events around the stages (read by
+// heff_timing_read / heff_stage_times).  scr: caller work space (see HeffScratch),
+// nullptr = the context's buffers.
+void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed,
+                const HeffScratch* scr = nullptr);
 void heff_timing_read(tn_ctx* c, double* gemm_ms, double* total_ms);
 // per-stage times of the last timed apply (ms): s1, mix, s3
 void heff_stage_times(tn_ctx* c, double* s1, double* mix, double* s3);

@@ -291,7 +291,33 @@ double HeffOp::flops_gemm(int D1, int D2) const
 
 static inline int lo_type(int prec) { return prec == TN_FP32 ? T_F32 : T_BF16; }
 
-void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed)
+void gemm_run_fixed(tn_ctx* c, const GemmDesc& g)
+{
+    GemmDesc t = g;
+    if (t.tile == 0) {
+        const std::array<long, 9> key{g.tin, g.tout, g.a_kmajor, g.b_kmajor, g.M, g.N, g.K, g.batch, g.accumulate};
+        auto it = c->tune.find(key);
+        t.tile = it != c->tune.end() ? it->second.first : gemm_default_tile(g.M, g.N, g.batch);
+    }
+    TN_HIP(gemm(t, c->stream));
+}
+
+size_t heff_scratch_bytes(const HeffOp& op, size_t* t1, size_t* t2, size_t* xlo)
+{
+    const int nR = op.Dw2 - (op.wr_id >= 0 ? 1 : 0), nL = op.Dw1 - (op.wl_id >= 0 ? 1 : 0);
+    const size_t n1 = (size_t)op.D1p * op.p * nR * op.D2p, n2 = (size_t)nL * op.D1p * op.p * op.D2p;
+    const size_t es = op.precision == TN_FP64 ? 8 : (op.precision == TN_FP32 ? 4 : 2);
+    const size_t a = op.precision == TN_FP64 ? n1 * 8 : n1 * 4;
+    const size_t b = n2 * es;
+    const size_t x = op.precision == TN_FP64 ? 0 : (size_t)op.n() * es;
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    if (t1) *t1 = up(a);
+    if (t2) *t2 = up(b);
+    if (xlo) *xlo = up(x);
+    return up(a) + up(b) + up(x);
+}
+
+void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed, const HeffScratch* scr)
 {
     hipStream_t s = c->stream;
     const int p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
@@ -336,34 +362,59 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
     g3.accumulate = idl ? 1 : 0;  // y already holds the identity channel's slice
     const long roff = (long)r0 * D2p * D2p;  // element offset of the first kept R channel
     const long loff = (long)l0 * D1p;        // column offset of the first kept L channel
+    // work space: the caller's (allocation-free applies) or the context's
+    void *T1 = nullptr, *T2 = nullptr, *XL = nullptr;
+    if (scr) {
+        size_t b1, b2, bx;
+        heff_scratch_bytes(op, &b1, &b2, &bx);
+        TN_REQUIRE(scr->T1 && scr->T2 && scr->T1_bytes >= b1 && scr->T2_bytes >= b2 &&
+                       (bx == 0 || (scr->xlo && scr->xlo_bytes >= bx)),
+                   TN_ERR_ARG, "heff_apply: work space too small");
+        T1 = scr->T1;
+        T2 = scr->T2;
+        XL = scr->xlo;
+    }
+    auto run = [&](const GemmDesc& g) {
+        if (scr) gemm_run_fixed(c, g);
+        else gemm_run(c, g);
+    };
     if (op.precision == TN_FP64) {
-        c->T1.ensure(n1 * 8);
-        c->T2.ensure(n2 * 8);
+        if (!scr) {
+            c->T1.ensure(n1 * 8);
+            c->T2.ensure(n2 * 8);
+            T1 = c->T1.p;
+            T2 = c->T2.p;
+        }
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
-        g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R + roff; g1.C = c->T1.p;
-        gemm_run(c, g1);
+        g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R + roff; g1.C = T1;
+        run(g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        TN_HIP(wmix_idt(T_F64, T_F64, T_F64, c->T1.p, (long)p * nR * D2p, x, (long)p * D2p, c->T2.p,
+        TN_HIP(wmix_idt(T_F64, T_F64, T_F64, T1, (long)p * nR * D2p, x, (long)p * D2p, T2,
                         (long)nL * D1p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
-        g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L + loff; g3.B = c->T2.p; g3.C = y;
-        gemm_run(c, g3);
+        g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L + loff; g3.B = T2; g3.C = y;
+        run(g3);
     } else {
         const int tl = lo_type(op.precision);
         const size_t es = dtype_size(tl);
-        c->xlo.ensure(n * es);
-        c->T1.ensure(n1 * 4);
-        c->T2.ensure(n2 * es);
-        TN_HIP(convert(T_F64, x, tl, c->xlo.p, n, s));
+        if (!scr) {
+            c->xlo.ensure(n * es);
+            c->T1.ensure(n1 * 4);
+            c->T2.ensure(n2 * es);
+            XL = c->xlo.p;
+            T1 = c->T1.p;
+            T2 = c->T2.p;
+        }
+        TN_HIP(convert(T_F64, x, tl, XL, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
-        g1.tin = tl; g1.tout = T_F32; g1.A = c->xlo.p; g1.B = (const char*)op.Rlo + roff * es; g1.C = c->T1.p;
-        gemm_run(c, g1);
+        g1.tin = tl; g1.tout = T_F32; g1.A = XL; g1.B = (const char*)op.Rlo + roff * es; g1.C = T1;
+        run(g1);
         if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        TN_HIP(wmix_idt(T_F32, tl, tl, c->T1.p, (long)p * nR * D2p, c->xlo.p, (long)p * D2p, c->T2.p,
+        TN_HIP(wmix_idt(T_F32, tl, tl, T1, (long)p * nR * D2p, XL, (long)p * D2p, T2,
                         (long)nL * D1p, y, (long)p * D2p, D1p, D2p, op.mix->t, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
-        g3.tin = tl; g3.tout = T_F64; g3.A = (const char*)op.Llo + loff * es; g3.B = c->T2.p; g3.C = y;
-        gemm_run(c, g3);
+        g3.tin = tl; g3.tout = T_F64; g3.A = (const char*)op.Llo + loff * es; g3.B = T2; g3.C = y;
+        run(g3);
     }
     if (timed) TN_HIP(hipEventRecord(c->ev[4], s));
 }

@@ -101,6 +101,11 @@ hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStre
 hipError_t fill_zero(void* p, long bytes, hipStream_t s);
 hipError_t fill_value(double* p, double v, long n, hipStream_t s);
 
+// site tensor layouts: reference psi[m + d*(a + D1*b)] <-> internal
+// out[(a*d + m)*D2p + b] over [D1p][d][D2p] (padding written as zero)
+hipError_t psi_ref_to_int(const double* ref, double* out, int d, int D1, int D2, int D1p, int D2p, hipStream_t s);
+hipError_t psi_int_to_ref(const double* in, double* ref, int d, int D1, int D2, int D2p, hipStream_t s);
+
 // copy a 3-index tensor with a strided view: out[(i*n1 + j)*n2o + k] = in[i*si + j*sj + k*sk]
 // for i<n0, j<n1, k<n2; entries k in [n2, n2o) are zeroed.
 hipError_t copy3(const double* in, long si, long sj, long sk, double* out, int n0, int n1,

@@ -668,6 +668,10 @@ __global__ void __launch_bounds__(256) publish_k(const double* __restrict__ src,
                                                  unsigned long long* host_flag, unsigned long long seq)
 {
     for (int t = threadIdx.x; t < n; t += 256) host_dst[t] = src[t];
+    // every wave orders its own stores at system scope before the barrier;
+    // the barrier alone is a workgroup-scope fence, so without this the host
+    // could see the flag before the values of waves 1-3
+    __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence_system();
@@ -734,6 +738,48 @@ hipError_t fill_value(double* p, double v, long n, hipStream_t s)
     long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     fill_value_k<<<blocks, 256, 0, s>>>(p, v, n);
+    return hipGetLastError();
+}
+
+// site tensor layouts: reference psi[m + d*(a + D1*b)] <-> internal [D1p][d][D2p]
+// (zero padding written on the internal side; grid-stride over the written side)
+__global__ void psi_r2i_k(const double* __restrict__ ref, double* __restrict__ out, int d, int D1, int D2, int D1p,
+                          int D2p)
+{
+    const long n = (long)D1p * d * D2p;
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int b = (int)(t % D2p);
+        const long r = t / D2p;
+        const int m = (int)(r % d), a = (int)(r / d);
+        out[t] = (a < D1 && b < D2) ? ref[m + (long)d * (a + (long)D1 * b)] : 0.0;
+    }
+}
+__global__ void psi_i2r_k(const double* __restrict__ in, double* __restrict__ ref, int d, int D1, int D2, int D2p)
+{
+    const long n = (long)d * D1 * D2;
+    for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+        const int m = (int)(t % d);
+        const long r = t / d;
+        const int a = (int)(r % D1), b = (int)(r / D1);
+        ref[t] = in[((long)a * d + m) * D2p + b];
+    }
+}
+hipError_t psi_ref_to_int(const double* ref, double* out, int d, int D1, int D2, int D1p, int D2p, hipStream_t s)
+{
+    if (d <= 0 || D1 <= 0 || D2 <= 0 || D1p < D1 || D2p < D2) return hipErrorInvalidValue;
+    const long n = (long)D1p * d * D2p;
+    long blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    psi_r2i_k<<<blocks, 256, 0, s>>>(ref, out, d, D1, D2, D1p, D2p);
+    return hipGetLastError();
+}
+hipError_t psi_int_to_ref(const double* in, double* ref, int d, int D1, int D2, int D2p, hipStream_t s)
+{
+    if (d <= 0 || D1 <= 0 || D2 <= 0 || D2p < D2) return hipErrorInvalidValue;
+    const long n = (long)d * D1 * D2;
+    long blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    psi_i2r_k<<<blocks, 256, 0, s>>>(in, ref, d, D1, D2, D2p);
     return hipGetLastError();
 }
 

@@ -60,6 +60,45 @@ int MPO::GetMaxDw() const
     return m;
 }
 
+static void mpo_arrays(const MPO* o, std::vector<int>& dw1, std::vector<int>& dw2, std::vector<double>& w);
+
+void MPO::CanonicalForm()
+{
+    if (itsForm != RegularUpper && itsForm != RegularLower)
+        throw std::invalid_argument("MPO::CanonicalForm needs a regular (upper or lower) MPO");
+    Compress(CNone, 0, 0.0);
+}
+
+double MPO::Compress(CompressType ct, int Dmax, double eps)
+{
+    // ct == CNone is used by CanonicalForm (block QR sweeps only)
+    if (ct == Std) throw std::invalid_argument("MPO::Compress: only Parker compression is provided");
+    if (itsForm != RegularUpper && itsForm != RegularLower)
+        throw std::invalid_argument("MPO::Compress needs a regular (upper or lower) MPO");
+    const int L = GetL(), d = itsOps[0].Getd();
+    std::vector<int> dw1, dw2, o1(L), o2(L);
+    std::vector<double> w;
+    mpo_arrays(this, dw1, dw2, w);
+    const int form = itsForm == RegularUpper ? 0 : 1, ops = ct == CNone ? 1 : 2;
+    double te = 0.0;
+    const int need = tn_mpo_reduce(L, d, dw1.data(), dw2.data(), w.data(), form, ops, Dmax, eps, o1.data(), o2.data(),
+                                   nullptr, 0, &te);
+    if (need < 0) throw std::invalid_argument("MPO::Compress: MPO not in regular form");
+    std::vector<double> out(need);
+    if (tn_mpo_reduce(L, d, dw1.data(), dw2.data(), w.data(), form, ops, Dmax, eps, o1.data(), o2.data(), out.data(),
+                      need, &te) != need)
+        throw std::runtime_error("MPO::Compress: inconsistent sizes");
+    std::vector<SiteOperator> ops_new;
+    size_t off = 0;
+    for (int i = 0; i < L; i++) {
+        const size_t n = (size_t)o1[i] * o2[i] * d * d;
+        ops_new.emplace_back(d, o1[i], o2[i], std::vector<double>(out.begin() + off, out.begin() + off + n));
+        off += n;
+    }
+    itsOps = std::move(ops_new);
+    return te;
+}
+
 MPO* MPO::Product(const MPO* other) const
 {
     // W2[(w1,v1),(w2,v2)](m,n) = sum_k W(w1,w2)(m,k) V(v1,v2)(k,n)
@@ -81,39 +120,84 @@ MPO* MPO::Product(const MPO* other) const
                             }
         ops.emplace_back(d, A1 * B1, A2 * B2, std::move(W));
     }
-    return new MPO(std::move(ops));
+    // the product of two regular MPOs of one form is regular in that form
+    // (channel (w, v) -> w*Dw + v keeps the first and the last channel)
+    return new MPO(std::move(ops), itsForm == other->itsForm ? itsForm : FUnknown);
 }
 
 // ---- Hamiltonian (Hamiltonian_1D_NN_Heisenberg.C:35-79, SiteOperatorImp.C:33-134) ----
+// ---- spin operators (SpinCalculator.C:20-70): index n <-> Sz = n - S --------------
+namespace {
+struct SpinOps {
+    int d;
+    std::vector<double> Sp, Sm, Sz, Sx, I;
+    explicit SpinOps(double S) : d((int)std::lround(2 * S + 1))
+    {
+        Sp.assign(d * d, 0.0);
+        Sm.assign(d * d, 0.0);
+        Sz.assign(d * d, 0.0);
+        Sx.assign(d * d, 0.0);
+        I.assign(d * d, 0.0);
+        for (int m = 0; m < d; m++)
+            for (int n = 0; n < d; n++) {
+                const double sm = m - S, sn = n - S;
+                if (m == n + 1) Sp[m * d + n] = std::sqrt(S * (S + 1) - sm * sn);
+                if (m + 1 == n) Sm[m * d + n] = std::sqrt(S * (S + 1) - sm * sn);
+                if (m == n) {
+                    Sz[m * d + n] = sm;
+                    I[m * d + n] = 1.0;
+                }
+            }
+        for (int k = 0; k < d * d; k++) Sx[k] = 0.5 * (Sp[k] + Sm[k]);
+    }
+    std::vector<double> mul(const std::vector<double>& a, const std::vector<double>& b) const
+    {
+        std::vector<double> c(d * d, 0.0);
+        for (int i = 0; i < d; i++)
+            for (int k = 0; k < d; k++)
+                for (int j = 0; j < d; j++) c[i * d + j] += a[i * d + k] * b[k * d + j];
+        return c;
+    }
+};
+// bulk operator-valued matrix under construction: block (a, b) = c * M
+struct BulkW {
+    int Dw, d;
+    std::vector<double> W;
+    BulkW(int Dw_, int d_) : Dw(Dw_), d(d_), W((size_t)Dw_ * Dw_ * d_ * d_, 0.0) {}
+    void set(int a, int b, const std::vector<double>& M, double c = 1.0)
+    {
+        for (int k = 0; k < d * d; k++) W[(size_t)(a * Dw + b) * d * d + k] = c * M[k];
+    }
+};
+}  // namespace
+
 Hamiltonian::Hamiltonian(int L, double S, MPOForm f, double Jxy, double Jz, double hz, double Dz)
+    : Hamiltonian(L, S, f, 5, [&]() {
+          SpinOps o(S);
+          const int d = o.d;
+          std::vector<double> On(d * d, 0.0);
+          for (int m = 0; m < d; m++) On[m * d + m] = hz * (m - S) + Dz * (m - S) * (m - S);
+          BulkW w(5, d);
+          if (f == RegularUpper) {
+              w.set(0, 0, o.I); w.set(0, 1, o.Sp); w.set(0, 2, o.Sm); w.set(0, 3, o.Sz); w.set(0, 4, On);
+              w.set(1, 4, o.Sm, 0.5 * Jxy); w.set(2, 4, o.Sp, 0.5 * Jxy); w.set(3, 4, o.Sz, Jz); w.set(4, 4, o.I);
+          } else {
+              w.set(0, 0, o.I); w.set(1, 0, o.Sp); w.set(2, 0, o.Sm); w.set(3, 0, o.Sz); w.set(4, 0, On);
+              w.set(4, 1, o.Sm, 0.5 * Jxy); w.set(4, 2, o.Sp, 0.5 * Jxy); w.set(4, 3, o.Sz, Jz); w.set(4, 4, o.I);
+          }
+          return w.W;
+      }())
+{
+}
+
+Hamiltonian::Hamiltonian(int L, double S, MPOForm f, int Dw, const std::vector<double>& Wb)
     : itsL(L), itsd((int)std::lround(2 * S + 1)), itsS(S)
 {
     if (L < 1 || itsd < 2) throw std::invalid_argument("Hamiltonian: L >= 1 and S >= 1/2");
     if (f != RegularUpper && f != RegularLower) throw std::invalid_argument("Hamiltonian: MPO form");
-    const int d = itsd, Dw = 5;
-    std::vector<double> Sp(d * d, 0.0), Sm(d * d, 0.0), Sz(d * d, 0.0), I(d * d, 0.0), On(d * d, 0.0);
-    for (int m = 0; m < d; m++)
-        for (int n = 0; n < d; n++) {
-            const double sm = m - S, sn = n - S;
-            if (m == n + 1) Sp[m * d + n] = std::sqrt(S * (S + 1) - sm * sn);
-            if (m + 1 == n) Sm[m * d + n] = std::sqrt(S * (S + 1) - sm * sn);
-            if (m == n) {
-                Sz[m * d + n] = sm;
-                I[m * d + n] = 1.0;
-                On[m * d + n] = hz * sm + Dz * sm * sm;
-            }
-        }
-    std::vector<double> Wb((size_t)Dw * Dw * d * d, 0.0);
-    auto set = [&](int a, int b, const std::vector<double>& M, double c) {
-        for (int k = 0; k < d * d; k++) Wb[(size_t)(a * Dw + b) * d * d + k] = c * M[k];
-    };
-    if (f == RegularUpper) {
-        set(0, 0, I, 1); set(0, 1, Sp, 1); set(0, 2, Sm, 1); set(0, 3, Sz, 1); set(0, 4, On, 1);
-        set(1, 4, Sm, 0.5 * Jxy); set(2, 4, Sp, 0.5 * Jxy); set(3, 4, Sz, Jz); set(4, 4, I, 1);
-    } else {
-        set(0, 0, I, 1); set(1, 0, Sp, 1); set(2, 0, Sm, 1); set(3, 0, Sz, 1); set(4, 0, On, 1);
-        set(4, 1, Sm, 0.5 * Jxy); set(4, 2, Sp, 0.5 * Jxy); set(4, 3, Sz, Jz); set(4, 4, I, 1);
-    }
+    const int d = itsd;
+    if (Dw < 1 || Wb.size() != (size_t)Dw * Dw * d * d) throw std::invalid_argument("Hamiltonian: bulk W size");
+    itsForm = f;
     const int lindex = (f == RegularUpper) ? 0 : Dw - 1, rindex = (f == RegularUpper) ? Dw - 1 : 0;
     for (int i = 0; i < L; i++) {
         int r0 = 0, r1 = Dw, c0 = 0, c1 = Dw;
@@ -142,7 +226,13 @@ MPO* Hamiltonian::CreateUnitOperator() const
     return new MPO(std::move(ops));
 }
 
-MPO* Hamiltonian::CreateH2Operator() const { return Product(this); }
+MPO* Hamiltonian::CreateH2Operator() const
+{
+    MPO* H2 = Product(this);
+    H2->CanonicalForm();
+    H2->Compress(Parker, 0, 1e-13);
+    return H2;
+}
 
 // ---- SVCompressorImp.C:19-73 ---------------------------------------------------------
 template <class T> double SVCompressor<T>::Compress(Matrix<T>& U, DiagonalMatrixRT& s, Matrix<T>& Vd) const
@@ -188,16 +278,41 @@ SparseEigenSolver<double>::UdType GpuSparseEigenSolver::Solve(const SparseEigenS
     if (!hc) throw std::invalid_argument("GpuSparseEigenSolver: client must be a GpuHeffClient (no host fallback)");
     if (nev < 1 || nev > 2) throw std::invalid_argument("GpuSparseEigenSolver: 1 <= NumEigenValues <= 2");
     const long N = hc->GetSize();
+    if (nev == 2 && N < 2) throw std::invalid_argument("GpuSparseEigenSolver: two eigenpairs need N >= 2");
     if ((long)itsGuess.size() != N) itsGuess.assign(N, 0.0);
     double ev[2] = {0, 0};
     int nmv = 0;
-    const int rc = tn_eig_lowest(hc->ctx(), N, nev, eps, 200, ev, itsGuess.data(), &nmv);
+    std::vector<double> X((size_t)nev * N);
+    std::copy(itsGuess.begin(), itsGuess.end(), X.begin());  // warm start (Reset() clears it)
+    const int rc = tn_eig_lowest(hc->ctx(), N, nev, eps, 200, ev, X.data(), &nmv);
     check(hc->ctx(), rc, "tn_eig_lowest");
+    std::copy(X.begin(), X.begin() + N, itsGuess.begin());
     Matrix<double> U(N, nev, 0.0);
-    for (long i = 0; i < N; i++) U(i + 1, 1) = itsGuess[i];  // column 1: lowest Ritz vector
+    for (int k = 0; k < nev; k++)
+        for (long i = 0; i < N; i++) U(i + 1, k + 1) = X[(size_t)k * N + i];  // column k+1: eigenvector k+1
     Vector<double> d(nev);
     for (int k = 0; k < nev; k++) d(k + 1) = ev[k];
     return std::make_tuple(U, d);
+}
+
+QRSolver<double>::QRType GpuQRSolver::SolveThinQR(const Matrix<double>& A)
+{
+    const int m = A.GetNumRows(), n = A.GetNumCols();
+    Matrix<double> Q(m, n), R(n, n);
+    const int rc = tn_qr(itsCtx, 0, m, n, A.data(), Q.data(), R.data());
+    check(itsCtx, rc, "tn_qr");
+    if (rc != 0) std::fprintf(stderr, "GpuQRSolver: orthogonality of Q above 1e-11\n");
+    return std::make_tuple(Q, R);
+}
+
+QRSolver<double>::QRType GpuQRSolver::SolveThinLQ(const Matrix<double>& A)
+{
+    const int m = A.GetNumRows(), n = A.GetNumCols();
+    Matrix<double> Lm(m, m), Q(m, n);
+    const int rc = tn_qr(itsCtx, 1, m, n, A.data(), Q.data(), Lm.data());
+    check(itsCtx, rc, "tn_qr");
+    if (rc != 0) std::fprintf(stderr, "GpuQRSolver: orthogonality of Q above 1e-11\n");
+    return std::make_tuple(Lm, Q);
 }
 
 SVDSolver<double>::UsVType GpuSVDSolver::Solve(const Matrix<double>& A, double, int ns)
@@ -231,7 +346,8 @@ static void mpo_arrays(const MPO* o, std::vector<int>& dw1, std::vector<int>& dw
     }
 }
 
-MPS::MPS(const Hamiltonian* H, int D, double, double, int Dcap) : itsL(H->GetL()), itsd(H->Getd())
+MPS::MPS(const Hamiltonian* H, int D, double normEps, double, int Dcap)
+    : itsL(H->GetL()), itsd(H->Getd()), itsNormEps(normEps)
 {
     itsCtx = nullptr;
     itsChain = nullptr;
@@ -240,16 +356,19 @@ MPS::MPS(const Hamiltonian* H, int D, double, double, int Dcap) : itsL(H->GetL()
     std::vector<int> dw1, dw2;
     std::vector<double> w;
     mpo_arrays(H, dw1, dw2, w);
-    check(itsCtx, tn_chain_create(itsCtx, itsL, itsd, dw1.data(), dw2.data(), w.data(), D, Dcap > 0 ? Dcap : D,
-                                  &itsChain),
-          "tn_chain_create");
+    rc = tn_chain_create(itsCtx, itsL, itsd, dw1.data(), dw2.data(), w.data(), D, Dcap > 0 ? Dcap : D, &itsChain);
+    if (rc < 0) {
+        const std::string msg = std::string("tn_chain_create: ") + tn_last_error(itsCtx);
+        tn_ctx_destroy(itsCtx);  // the destructor does not run for a failed constructor
+        throw std::runtime_error(msg);
+    }
     tn_sweep_opts_default(&itsOpts);
 }
 
 MPS::~MPS()
 {
-    tn_chain_destroy(itsChain);
-    tn_ctx_destroy(itsCtx);
+    if (itsChain) tn_chain_destroy(itsChain);
+    if (itsCtx) tn_ctx_destroy(itsCtx);
 }
 
 void MPS::InitializeWith(State s, uint64_t seed)
@@ -260,6 +379,57 @@ void MPS::InitializeWith(State s, uint64_t seed)
 }
 
 void MPS::Normalize(Direction LR) { check(itsCtx, tn_chain_normalize(itsChain, LR == DLeft ? 0 : 1), "tn_chain_normalize"); }
+
+void MPS::NormalizeQR(Direction LR)
+{
+    check(itsCtx, tn_chain_normalize_qr(itsChain, LR == DLeft ? 0 : 1), "tn_chain_normalize_qr");
+}
+
+std::string MPS::GetNormStatus() const
+{
+    std::vector<char> buf(itsL + 1);
+    check(itsCtx, tn_chain_norm_status(itsChain, itsNormEps, buf.data()), "tn_chain_norm_status");
+    return std::string(buf.data());
+}
+
+char MPS::GetNormStatus(int isite) const
+{
+    if (isite < 1 || isite > itsL) throw std::invalid_argument("MPS::GetNormStatus: site out of range");
+    return GetNormStatus()[isite - 1];
+}
+
+void MPS::Save(const std::string& path, bool withEnvironments) const
+{
+    check(itsCtx, tn_chain_save(itsChain, path.c_str(), nullptr, 0, withEnvironments ? 1 : 0), "tn_chain_save");
+}
+
+MPS* MPS::Load(const std::string& path, int Dcap)
+{
+    // L and d from the checkpoint header ("TNCHKPT1", version, L, d, ...)
+    int head[5] = {0, 0, 0, 0, 0};
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::invalid_argument("MPS::Load: cannot open " + path);
+    char magic[8];
+    const bool ok = std::fread(magic, 1, 8, f) == 8 && std::fread(head, 4, 5, f) == 5;
+    std::fclose(f);
+    if (!ok) throw std::invalid_argument("MPS::Load: " + path + " is not a chain checkpoint");
+    MPS* m = new MPS();
+    if (tn_ctx_create(0, &m->itsCtx) != 0) {
+        delete m;
+        throw std::runtime_error("MPS::Load: no HIP device (tn_ctx_create failed)");
+    }
+    int64_t nb = 0;
+    const int rc = tn_chain_load(m->itsCtx, path.c_str(), Dcap, &m->itsChain, nullptr, 0, &nb);
+    if (rc < 0) {
+        const std::string msg = std::string("tn_chain_load: ") + tn_last_error(m->itsCtx);
+        delete m;
+        throw std::runtime_error(msg);
+    }
+    m->itsL = head[1];
+    m->itsd = head[2];
+    tn_sweep_opts_default(&m->itsOpts);
+    return m;
+}
 
 int MPS::GetMaxD() const
 {
@@ -339,6 +509,80 @@ Hamiltonian* Factory::Make1D_NN_HeisenbergHamiltonian(int L, double S, MPOForm f
 {
     return new Hamiltonian(L, S, f, Jxy, Jz, hz, Dz);
 }
+// Hamiltonian_1D_NN_TransverseIsing.C:26-46: H = J sum Sz Sz + hx sum Sx, Dw = 3
+Hamiltonian* Factory::Make1D_NN_TransverseIsingHamiltonian(int L, double S, MPOForm f, double J, double hx)
+{
+    SpinOps o(S);
+    BulkW w(3, o.d);
+    if (f == RegularUpper) {
+        w.set(0, 0, o.I); w.set(0, 1, o.Sz); w.set(0, 2, o.Sx, hx); w.set(1, 2, o.Sz, J); w.set(2, 2, o.I);
+    } else {
+        w.set(0, 0, o.I); w.set(1, 0, o.Sz); w.set(2, 0, o.Sx, hx); w.set(2, 1, o.Sz, J); w.set(2, 2, o.I);
+    }
+    return new Hamiltonian(L, S, f, 3, w.W);
+}
+
+// Hamiltonian_1D_2Body_LongRange.C:26-63 and Hamiltonian_1D_3Body_LongRange.C:26-63:
+// Dw = 2 + NN(NN+1)/2, for each range i a string of i channels
+static std::vector<double> long_range_W(const SpinOps& o, MPOForm f, double J, double hx, int NN, bool three)
+{
+    const int Dw = 2 + NN * (NN + 1) / 2;
+    BulkW w(Dw, o.d);
+    w.set(0, 0, o.I);
+    w.set(Dw - 1, Dw - 1, o.I);
+    const std::vector<double> Sz2 = o.mul(o.Sz, o.Sz);
+    const std::vector<double>& open = three ? Sz2 : o.Sz;
+    int ib = 0;
+    for (int i = 1; i <= NN; i++) {
+        for (int j = 1; j < i; j++) {
+            const std::vector<double>& link = three ? o.Sz : o.I;
+            const double c = three ? 1.0 / (j * j) : 1.0;
+            if (f == RegularUpper) w.set(ib + j, ib + 1 + j, link, c);
+            else w.set(ib + 1 + j, ib + j, link, c);
+        }
+        if (f == RegularUpper) {
+            w.set(0, ib + 1, open);
+            w.set(ib + i, Dw - 1, open, J / (i * i));
+        } else {
+            w.set(ib + 1, 0, open);
+            w.set(Dw - 1, ib + i, open, J / (i * i));
+        }
+        ib += i;
+    }
+    if (f == RegularUpper) w.set(0, Dw - 1, o.Sx, hx);
+    else w.set(Dw - 1, 0, o.Sx, hx);
+    return w.W;
+}
+
+Hamiltonian* Factory::Make1D_2BodyLongRangeHamiltonian(int L, double S, MPOForm f, double J, double hx, int NN)
+{
+    if (NN < 1) throw std::invalid_argument("Make1D_2BodyLongRangeHamiltonian: NN >= 1");
+    SpinOps o(S);
+    return new Hamiltonian(L, S, f, 2 + NN * (NN + 1) / 2, long_range_W(o, f, J, hx, NN, false));
+}
+
+Hamiltonian* Factory::Make1D_3BodyLongRangeHamiltonian(int L, double S, MPOForm f, double J, double hx, int NN)
+{
+    if (NN < 1) throw std::invalid_argument("Make1D_3BodyLongRangeHamiltonian: NN >= 1");
+    SpinOps o(S);
+    return new Hamiltonian(L, S, f, 2 + NN * (NN + 1) / 2, long_range_W(o, f, J, hx, NN, true));
+}
+
+// Hamiltonian_1D_3Body.C:26-56 (entries as given there), Dw = 5
+Hamiltonian* Factory::Make1D_3BodyHamiltonian(int L, double S, MPOForm f, double J, double K, double hz)
+{
+    SpinOps o(S);
+    BulkW w(5, o.d);
+    if (f == RegularUpper) {
+        w.set(0, 0, o.I); w.set(0, 1, o.Sx); w.set(0, 2, o.Sx); w.set(0, 4, o.Sz, hz);
+        w.set(3, 2, o.Sz); w.set(1, 4, o.Sx, J); w.set(3, 4, o.Sx, K); w.set(4, 4, o.I);
+    } else {
+        w.set(0, 0, o.I); w.set(1, 0, o.Sx); w.set(2, 0, o.Sx); w.set(4, 0, o.Sz, hz);
+        w.set(2, 3, o.Sz); w.set(4, 1, o.Sx, J); w.set(4, 3, o.Sx, K); w.set(4, 4, o.I);
+    }
+    return new Hamiltonian(L, S, f, 5, w.W);
+}
+
 SVCompressorC* Factory::MakeMPSCompressor(int Dmax, double epsSV) { return new SVCompressorC(Dmax, epsSV); }
 TNSLogger* Factory::MakeSPDLogger(int level) { return new TNSLogger(level); }
 

@@ -8,8 +8,8 @@ reached, and evaluate <H> and <H^2>.  Returns <H^2> - <H>^2 of the last line.
 """
 from dataclasses import dataclass, field
 
-from ._abi import SweepOpts, SweepStats
-from .models import mpo_square
+from ._abi import SweepOpts, SweepStats, mpo_reduce
+from .models import mpo_square, regular_form
 
 
 @dataclass
@@ -32,10 +32,21 @@ class GSResult:
     stats: SweepStats = None
 
 
+def create_h2(Ws):
+    """Hamiltonian::CreateH2Operator (TensorNetworks/Hamiltonian.C:8-16): H*H,
+    CanonicalForm and Compress(Parker, 0, 1e-13) when H has a regular form;
+    the plain product otherwise."""
+    H2 = mpo_square(Ws)
+    form = regular_form(Ws)
+    if form is None:
+        return H2
+    return mpo_reduce(H2, form, ops=3, Dmax=0, eps=1e-13)[0]
+
+
 def FindVariationalGroundState(chain, schedule, opts=None, H2=None, log=None):
     """Run the schedule on a Chain; the chain's own MPO is the Hamiltonian."""
     opts = opts or SweepOpts.default()
-    H2 = H2 if H2 is not None else mpo_square(chain.Ws)
+    H2 = H2 if H2 is not None else create_h2(chain.Ws)
     stats = SweepStats()
     res = GSResult(stats=stats)
     chain.normalize(1)

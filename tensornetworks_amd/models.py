@@ -65,6 +65,119 @@ def xxz_mpo(L, S=0.5, Jxy=1.0, Jz=1.0, hz=0.0, Dz=0.0, form=REGULAR_UPPER):
     return edge_trim(xxz_bulk_W(d, Jxy, Jz, hz, Dz, form), L, form)
 
 
+def sx_op(d):
+    """Sx = (S+ + S-)/2 (TensorNetworksImp/SpinCalculator.C:49-52)."""
+    Sp, Sm, _ = spin_ops(d)
+    return 0.5 * (Sp + Sm)
+
+
+def tfim_bulk_W(d, J=1.0, hx=0.0, form=REGULAR_UPPER):
+    """Transverse-field Ising chain H = J sum Sz Sz + hx sum Sx, Dw = 3
+    (TensorNetworksImp/Hamiltonians/Hamiltonian_1D_NN_TransverseIsing.C:26-46)."""
+    _, _, Sz = spin_ops(d)
+    I, Sx = np.eye(d), sx_op(d)
+    W = np.zeros((3, 3, d, d))
+    if form == REGULAR_UPPER:
+        W[0, 0], W[0, 1], W[0, 2], W[1, 2], W[2, 2] = I, Sz, hx * Sx, J * Sz, I
+    else:
+        W[0, 0], W[1, 0], W[2, 0], W[2, 1], W[2, 2] = I, Sz, hx * Sx, J * Sz, I
+    return W
+
+
+def long_range_bulk_W(d, J=1.0, hx=0.0, NN=1, three_body=False, form=REGULAR_UPPER):
+    """Long-range chains with Dw = 2 + NN(NN+1)/2: for each range i = 1..NN a
+    string of i channels carries the left operator i sites to the right.
+    Two-body (Hamiltonian_1D_2Body_LongRange.C:26-63): H = sum_i J/i^2 Sz_k Sz_{k+i}
+    + hx sum Sx.  Three-body (Hamiltonian_1D_3Body_LongRange.C:26-63): the string
+    opens with Sz^2, its links after the first multiply by Sz/j^2, it closes with
+    J/i^2 Sz^2; + hx sum Sx."""
+    _, _, Sz = spin_ops(d)
+    I, Sx = np.eye(d), sx_op(d)
+    Dw = 2 + NN * (NN + 1) // 2
+    W = np.zeros((Dw, Dw, d, d))
+    W[0, 0] = I
+    W[Dw - 1, Dw - 1] = I
+    opn = Sz @ Sz if three_body else Sz
+    ib = 0
+    for i in range(1, NN + 1):
+        for j in range(1, i):
+            link = (Sz / (j * j)) if three_body else I
+            if form == REGULAR_UPPER:
+                W[ib + j, ib + 1 + j] = link
+            else:
+                W[ib + 1 + j, ib + j] = link
+        if form == REGULAR_UPPER:
+            W[0, ib + 1] = opn
+            W[ib + i, Dw - 1] = J / (i * i) * opn
+        else:
+            W[ib + 1, 0] = opn
+            W[Dw - 1, ib + i] = J / (i * i) * opn
+        ib += i
+    if form == REGULAR_UPPER:
+        W[0, Dw - 1] = hx * Sx
+    else:
+        W[Dw - 1, 0] = hx * Sx
+    return W
+
+
+def three_body_bulk_W(d, J=1.0, K=1.0, hz=0.0, form=REGULAR_UPPER):
+    """The three-body chain of TensorNetworksImp/Hamiltonians/Hamiltonian_1D_3Body.C:26-56,
+    Dw = 5, entries as given there (channels 2 and 3 are only linked to each
+    other, so only J sum Sx Sx + hz sum Sz acts; the compression drops them)."""
+    _, _, Sz = spin_ops(d)
+    I, Sx = np.eye(d), sx_op(d)
+    W = np.zeros((5, 5, d, d))
+    if form == REGULAR_UPPER:
+        W[0, 0], W[0, 1], W[0, 2], W[0, 4] = I, Sx, Sx, hz * Sz
+        W[3, 2], W[1, 4], W[3, 4], W[4, 4] = Sz, J * Sx, K * Sx, I
+    else:
+        W[0, 0], W[1, 0], W[2, 0], W[4, 0] = I, Sx, Sx, hz * Sz
+        W[2, 3], W[4, 1], W[4, 3], W[4, 4] = Sz, J * Sx, K * Sx, I
+    return W
+
+
+def tfim_mpo(L, S=0.5, J=1.0, hx=0.0, form=REGULAR_UPPER):
+    d = int(round(2 * S + 1))
+    return edge_trim(tfim_bulk_W(d, J, hx, form), L, form)
+
+
+def long_range_mpo(L, S=0.5, J=1.0, hx=0.0, NN=1, three_body=False, form=REGULAR_UPPER):
+    d = int(round(2 * S + 1))
+    return edge_trim(long_range_bulk_W(d, J, hx, NN, three_body, form), L, form)
+
+
+def three_body_mpo(L, S=0.5, J=1.0, K=1.0, hz=0.0, form=REGULAR_UPPER):
+    d = int(round(2 * S + 1))
+    return edge_trim(three_body_bulk_W(d, J, K, hz, form), L, form)
+
+
+def mpo_contract(Ws):
+    """Dense operator of an MPO (d^L x d^L, site 1 the most significant
+    index): the brute-force check of small MPOs."""
+    T = Ws[0][0]  # (Dw2, d, d)
+    for W in Ws[1:]:
+        # T[w][M][N] x W[w][v][m][n] -> [v][(M,m)][(N,n)]
+        a, b, d, _ = W.shape
+        n = T.shape[1]
+        T = np.einsum("wMN,wvmn->vMmNn", T, W).reshape(b, n * d, n * d)
+    return T[0]
+
+
+def regular_form(Ws):
+    """REGULAR_UPPER or REGULAR_LOWER when the MPO has that regular form (its
+    edge blocks hold the identity where the form puts it), else None."""
+    first, last = Ws[0], Ws[-1]
+    d = first.shape[2]
+    I = np.eye(d)
+    if len(Ws) < 2 or first.shape[1] < 2:
+        return None
+    if np.array_equal(first[0, 0], I) and np.array_equal(last[-1, 0], I):
+        return REGULAR_UPPER
+    if np.array_equal(first[0, -1], I) and np.array_equal(last[0, 0], I):
+        return REGULAR_LOWER
+    return None
+
+
 def mpo_square(Ws):
     """Uncompressed product MPO H*H (OperatorValuedMatrix.H:410 TensorProduct):
     W2[(w1,v1),(w2,v2)] = W(w1,w2) @ W(v1,v2) in physical space."""

@@ -1,0 +1,235 @@
+"""GPU checks of the solver seam and the chain services added around it:
+the second eigenpair of tn_eig_lowest (LapackEigenSolver's IL=1..IU=2), the
+thin QR/LQ seam, the device-pointer Heff apply, the polled read-back, the
+norm status strings of MPSNormTests and checkpoint / resume."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tensornetworks_amd as tn
+from tensornetworks_amd.models import xxz_mpo, tfim_mpo
+
+pytestmark = pytest.mark.gpu
+
+
+def rand(shape, seed):
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, shape)
+
+
+def sym_env(Dw, D, seed):
+    F = rand((Dw, D, D), seed)
+    F = 0.5 * (F + F.transpose(0, 2, 1))
+    return np.ascontiguousarray(F.transpose(2, 1, 0)).ravel()
+
+
+def sym_W(Dw1, Dw2, d, seed):
+    W = rand((Dw1, Dw2, d, d), seed)
+    W = 0.5 * (W + W.transpose(0, 1, 3, 2))
+    W[np.abs(W) < 0.3] = 0.0
+    return np.ascontiguousarray(W)
+
+
+@pytest.mark.parametrize("d,D1,D2", [(2, 4, 6), (3, 9, 9), (2, 12, 10), (3, 33, 17)])
+def test_two_lowest_eigenpairs_match_dense(ctx, d, D1, D2):
+    # LapackEigenSolver.C:62-109 (zheevx, IL=1, IU=2): both eigenpairs, ascending
+    W = sym_W(5, 5, d, 70)
+    L, R = sym_env(5, D1, 71), sym_env(5, D2, 72)
+    H = O.heff_dense(W, L, R, d, D1, D2)
+    w, V = np.linalg.eigh(H)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    ev, X, nmv, rc = ctx.eig_lowest(np.zeros(d * D1 * D2), tol=1e-11, maxit=400, nev=2)
+    assert rc == 0 and X.shape == (d * D1 * D2, 2)
+    scale = max(1.0, abs(w[0]))
+    assert abs(ev[0] - w[0]) < 1e-10 * scale and abs(ev[1] - w[1]) < 1e-10 * scale
+    for k in range(2):
+        assert abs(np.linalg.norm(X[:, k]) - 1) < 1e-12
+        assert np.linalg.norm(H @ X[:, k] - ev[k] * X[:, k]) < 1e-8 * scale
+        if w[k + 1] - w[k] > 1e-6 and (k == 0 or w[k] - w[k - 1] > 1e-6):
+            assert abs(abs(X[:, k] @ V[:, k]) - 1) < 1e-9
+    assert abs(X[:, 0] @ X[:, 1]) < 1e-8
+    # nev = 1 returns one vector, the same lowest pair
+    ev1, x1, _, _ = ctx.eig_lowest(np.zeros(d * D1 * D2), tol=1e-11, maxit=400, nev=1)
+    assert x1.shape == (d * D1 * D2,) and abs(ev1[0] - w[0]) < 1e-10 * scale
+
+
+def test_positive_operator_lowest_eigenvalue(ctx):
+    # all eigenvalues positive: the random start covers only the logical
+    # entries, so the zero padding of the device layout (D not a multiple of
+    # 32) never offers the eigenvalue 0
+    d, D1, D2 = 2, 5, 7
+    W = sym_W(5, 5, d, 80)
+    # channel 0 of both environments the identity: W[0, 0] then acts as
+    # I x W00 x I, so adding c*I to it shifts the whole spectrum by c
+    Ls = sym_env(5, D1, 81).reshape(D1, D1, 5).copy()
+    Rs = sym_env(5, D2, 82).reshape(D2, D2, 5).copy()
+    Ls[:, :, 0] = np.eye(D1)
+    Rs[:, :, 0] = np.eye(D2)
+    w0 = np.linalg.eigvalsh(O.heff_dense(W, Ls.ravel(), Rs.ravel(), d, D1, D2))[0]
+    Wsh = W.copy()
+    Wsh[0, 0] += (abs(w0) + 1.0) * np.eye(d)
+    Hs = O.heff_dense(Wsh, Ls.ravel(), Rs.ravel(), d, D1, D2)
+    ws = np.linalg.eigvalsh(Hs)
+    assert ws[0] > 0.5
+    ctx.site_bind(Wsh, Ls.ravel(), Rs.ravel(), d, D1, D2, tn.TN_FP64)
+    ev, X, _, rc = ctx.eig_lowest(np.zeros(d * D1 * D2), tol=1e-11, maxit=300, nev=2)
+    assert rc == 0
+    assert abs(ev[0] - ws[0]) < 1e-10 * ws[-1] and abs(ev[1] - ws[1]) < 1e-10 * ws[-1]
+
+
+def _qr_positive(A):
+    q, r = np.linalg.qr(A)
+    sg = np.sign(np.diag(r))
+    sg[sg == 0] = 1.0
+    return q * sg, (r.T * sg).T
+
+
+@pytest.mark.parametrize("m,n", [(40, 24), (96, 96), (300, 70), (1030, 257)])
+def test_thin_qr_matches_householder(ctx, m, n):
+    # LapackQRSolver::SolveThinQR as MPSSite::NormalizeQR calls it (DLeft)
+    A = np.random.default_rng(m + n).standard_normal((m, n))
+    Q, R, rc = ctx.qr(A, 0)
+    assert rc == 0
+    q, r = _qr_positive(A)
+    assert np.abs(np.tril(R, -1)).max() == 0.0
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-12
+    assert np.abs(Q @ R - A).max() < 1e-12 * np.abs(A).max() * np.sqrt(m)
+    assert np.abs(R - r).max() < 1e-10 * np.abs(r).max()
+
+
+@pytest.mark.parametrize("m,n", [(24, 40), (70, 300), (257, 1030)])
+def test_thin_lq(ctx, m, n):
+    # SolveThinLQ (MPSSite::NormalizeQR, DRight): A = L Q, Q orthonormal rows
+    A = np.random.default_rng(m * n).standard_normal((m, n))
+    Q, Lm, rc = ctx.qr(A, 1)
+    assert rc == 0
+    assert np.abs(np.triu(Lm, 1)).max() == 0.0
+    assert np.abs(Q @ Q.T - np.eye(m)).max() < 1e-12
+    assert np.abs(Lm @ Q - A).max() < 1e-12 * np.abs(A).max() * np.sqrt(n)
+    q, r = _qr_positive(A.T)
+    assert np.abs(Lm - r.T).max() < 1e-10 * np.abs(r).max()
+
+
+def test_qr_rejects_wrong_shape(ctx):
+    with pytest.raises(tn.TnError):
+        ctx.qr(np.ones((4, 8)), 0)
+    with pytest.raises(tn.TnError):
+        ctx.qr(np.ones((8, 4)), 1)
+
+
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6)])
+def test_heff_apply_device_pointers(ctx, prec, tol):
+    torch = pytest.importorskip("torch")
+    d, D1, D2 = 3, 37, 45
+    W = sym_W(5, 5, d, 90)
+    L, R, x = rand(5 * D1 * D1, 91), rand(5 * D2 * D2, 92), rand(d * D1 * D2, 93)
+    ctx.site_bind(W, L, R, d, D1, D2, prec)
+    y_host = ctx.heff_apply(x)
+    xd = torch.tensor(x, dtype=torch.float64, device="cuda:0")
+    yd = torch.full_like(xd, float("nan"))
+    torch.cuda.synchronize()
+    ctx.heff_apply_device(xd.data_ptr(), yd.data_ptr(), x.size)
+    y_dev = yd.cpu().numpy()
+    assert np.array_equal(y_dev, y_host)  # same kernels, same order
+    y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    assert np.linalg.norm(y_dev - y_ref) / np.linalg.norm(y_ref) < tol
+
+
+def test_polled_readback_values(ctx):
+    # every one of up to 16384 published values (all four waves of the
+    # publishing workgroup write some) arrives at the host
+    for n in (1, 63, 64, 65, 256, 257, 4099, 16384):
+        x = np.random.default_rng(n).standard_normal(n)
+        assert np.array_equal(ctx.publish_test(x), x)
+
+
+# MPSNormTests.C:55-140: status strings after Normalize
+@pytest.mark.parametrize("S,D,state,direction,expect", [
+    (3.0, 2, 0, 0, "AIIIIIIIIA"),   # :55-60  product state, DLeft (S=3)
+    (3.0, 2, 0, 1, "BIIIIIIIIB"),   # :63-68  product state, DRight
+    (0.5, 2, 1, 0, "AAAAAAAAAA"),   # :71-76
+    (0.5, 2, 1, 1, "BBBBBBBBBB"),   # :78-83
+    (0.5, 8, 1, 0, "AAAAAAAAAA"),   # :93-101
+    (1.5, 3, 1, 0, "AAAAAAAAAA"),   # :103-110
+    (0.5, 1, 1, 0, "IIIIIIIIII"),   # :112-119
+    (2.5, 1, 1, 1, "IIIIIIIIII"),   # :128-135
+])
+def test_norm_status_strings(ctx, S, D, state, direction, expect):
+    ch = tn.Chain(ctx, xxz_mpo(10, S), D)
+    ch.initialize(state, 7)
+    ch.normalize(direction)
+    assert ch.norm_status() == expect
+    ch.close()
+
+
+def test_norm_status_qr_and_mixed(ctx):
+    L = 10
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), 8)
+    ch.initialize(1, 5)
+    assert "M" in ch.norm_status()
+    ch.normalize_qr(1)
+    assert ch.norm_status() == "B" * L
+    ch.normalize_qr(0)
+    assert ch.norm_status() == "A" * L
+    # one sweep leaves the chain right-normalised except its first site
+    o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1)
+    ch.load_caches(o)
+    ch.sweep(1, o)
+    s = ch.norm_status()
+    assert s[1:] == "B" * (L - 1), s
+    ch.close()
+
+
+@pytest.mark.parametrize("with_env", [False, True])
+def test_checkpoint_resume_reproduces_sweeps(ctx, tmp_path, with_env):
+    # an interrupted run resumed from a checkpoint continues exactly like the
+    # uninterrupted one
+    L, D = 12, 24
+    Ws = xxz_mpo(L, 1.0, Dz=0.1)
+    o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1)
+    a = tn.Chain(ctx, Ws, D)
+    a.initialize(1, 11)
+    a.normalize_qr(1)
+    a.load_caches(o)
+    for _ in range(2):
+        a.sweep(0, o)
+        a.sweep(1, o)
+    path = str(tmp_path / "chain.tnck")
+    meta = json.dumps({"line": 0, "D": D, "sweep": 2}).encode()
+    a.save(path, meta, environments=with_env)
+    for _ in range(2):
+        a.sweep(0, o)
+        a.sweep(1, o)
+    e_a = a.expectation()
+    b, m = tn.Chain.load(ctx, path)
+    assert json.loads(m) == {"line": 0, "D": D, "sweep": 2}
+    if not with_env:
+        b.load_caches(o)
+    for _ in range(2):
+        b.sweep(0, o)
+        b.sweep(1, o)
+    e_b = b.expectation()
+    assert abs(e_a - e_b) < 1e-12 * abs(e_a), (e_a, e_b)
+    assert b.dims() == a.dims()
+    for i in (0, L // 2, L - 1):
+        assert np.abs(b.site(i) - a.site(i)).max() < 1e-10
+    a.close()
+    b.close()
+
+
+def test_checkpoint_rejects_corruption(ctx, tmp_path):
+    ch = tn.Chain(ctx, tfim_mpo(6, 0.5, 1.0, 0.5), 4)
+    ch.initialize(1, 3)
+    path = str(tmp_path / "c.tnck")
+    ch.save(path)
+    raw = bytearray(open(path, "rb").read())
+    raw[len(raw) // 2] ^= 0xFF
+    bad = str(tmp_path / "bad.tnck")
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(tn.TnError):
+        tn.Chain.load(ctx, bad)
+    with pytest.raises(tn.TnError):
+        tn.Chain.load(ctx, str(tmp_path / "missing.tnck"))
+    ch.close()

@@ -65,8 +65,9 @@ def test_normalisation_makes_right_canonical(ctx):
     assert D1[L - 1] == 2 and D1[L - 2] == 4
 
 
-@pytest.mark.parametrize("case", [c for c in GOLD["dmrg_per_bond"] if c["L"] <= 10 and c["S"] == 0.5])
+@pytest.mark.parametrize("case", GOLD["dmrg_per_bond"], ids=lambda c: f"L{c['L']}-S{c['S']}-D{c['D']}")
 def test_variational_ground_state_schedule(ctx, case):
+    # every recorded one-site DMRG energy: S=1/2 up to L=19, S=1, S=5/2
     L, D, Dstart = case["L"], case["D"], case["Dstart"]
     Ws = xxz_mpo(L, case["S"])
     ch = tn.Chain(ctx, Ws, Dstart, Dcap=max(D, Dstart))
