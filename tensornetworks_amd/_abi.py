@@ -371,6 +371,8 @@ def _mpo_arrays(Ws):
 
 def read_checkpoint_mpo(path):
     """The MPO stored in a chain checkpoint (list of W[i], (Dw1, Dw2, d, d))."""
+    if not os.path.isfile(path):
+        raise TnError(-2, f"{path}: no such checkpoint file")
     with open(path, "rb") as f:
         head = f.read(8)
         if head != b"TNCHKPT1":

@@ -1040,6 +1040,21 @@ int tn_eig_lowest(tn_ctx* c, int64_t N, int nev, double tol, int maxit, double* 
             if (std::fabs(ov) < 1e-3) break;
             sigma *= 10.0;
         }
+        // the true residual |H x1 - e1 x1| of the undeflated operator
+        {
+            DBuf hx;
+            hx.ensure((size_t)n * 8);
+            heff_apply(c, op, x1.as<double>(), hx.as<double>(), false);
+            TN_HIP(axpby(-r2.e0, x1.as<double>(), 1.0, hx.as<double>(), n, c->stream));
+            TN_HIP(mdot(hx.as<double>(), n, 1, hx.as<double>(), n, c->defl.as<double>(), c->defl.as<double>() + 8,
+                        (int)pcap, c->stream));
+            double r2n = 0;
+            d2h(c, &r2n, c->defl.p, 8);
+            hx.release();
+            if (!(std::sqrt(std::max(r2n, 0.0)) <= 100.0 * std::max(tol, 1e-14 * std::fabs(sigma)) *
+                                                     std::max(1.0, std::fabs(r2.e0))))
+                r2.converged = false;
+        }
         conv = conv && r2.converged;
         sync(c);
         psi_from_dev(c, x1.as<double>(), evec + N);

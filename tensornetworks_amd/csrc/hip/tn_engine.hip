@@ -822,14 +822,18 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
                 read(hh.data(), h, j + 3);
                 al[j] = hh[0] + hh[j + 1];
                 be[j] = std::sqrt(std::max(hh[j + 2], 0.0));
+                tnorm = std::max(tnorm, std::fabs(al[j]) + be[j] + (j > 0 ? be[j - 1] : 0.0));
                 res.n_full++;
             }
             tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, nullptr, z.data(), false);
             k = j + 1;
             res.residual = std::fabs(be[j] * z[j]);
             if (!std::isfinite(res.e0)) throw TnError{TN_ERR_STATE, "Lanczos: non-finite Ritz value"};
+            // breakdown: the Krylov space is invariant up to rounding, judged
+            // against the operator's scale |T| (the remainder of a complete
+            // space is ~ eps |T| sqrt(N), not ~ eps |theta|)
             const bool done = res.residual < tol || res.residual < tol_rel * std::fabs(res.e0) ||
-                              be[j] < 1e-14 * std::max(1.0, std::fabs(res.e0));
+                              be[j] < 1e-14 * std::max(1.0, std::fabs(res.e0)) || be[j] <= 1e-13 * tnorm;
             if (done || j + 1 == kmax)  // the second Ritz value is only reported
                 tridiag_lowest(j + 1, al.data(), be.data(), &res.e0, &res.e1, z.data(), true);
             if (done) {

@@ -200,7 +200,7 @@ def test_lanczos_lowest_matches_dense(ctx, d, D1, D2):
     assert np.abs(H - H.T).max() < 1e-13  # the Hermiticity check of LapackEigenSolver.C:71-77
     w = np.linalg.eigvalsh(H)
     ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
-    ev, x, nmv, rc = ctx.eig_lowest(rand(d * D1 * D2, 43), tol=1e-11, maxit=400)
+    ev, x, nmv, rc = ctx.eig_lowest(rand(d * D1 * D2, 43), tol=1e-11, maxit=400, nev=1)
     assert rc == 0
     assert abs(ev[0] - w[0]) < 1e-10 * max(1, abs(w[0]))
     assert abs(np.linalg.norm(x) - 1) < 1e-12

@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 import tensornetworks_amd as tn
+from devmem import DeviceArray
 from tensornetworks_amd.models import xxz_mpo, tfim_mpo
 
 pytestmark = pytest.mark.gpu
@@ -121,17 +122,18 @@ def test_qr_rejects_wrong_shape(ctx):
 
 @pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6)])
 def test_heff_apply_device_pointers(ctx, prec, tol):
-    torch = pytest.importorskip("torch")
     d, D1, D2 = 3, 37, 45
     W = sym_W(5, 5, d, 90)
     L, R, x = rand(5 * D1 * D1, 91), rand(5 * D2 * D2, 92), rand(d * D1 * D2, 93)
     ctx.site_bind(W, L, R, d, D1, D2, prec)
     y_host = ctx.heff_apply(x)
-    xd = torch.tensor(x, dtype=torch.float64, device="cuda:0")
-    yd = torch.full_like(xd, float("nan"))
-    torch.cuda.synchronize()
-    ctx.heff_apply_device(xd.data_ptr(), yd.data_ptr(), x.size)
-    y_dev = yd.cpu().numpy()
+    xd, yd = DeviceArray(x.size), DeviceArray(x.size)
+    xd.upload(x)
+    yd.upload(np.full(x.size, np.nan))
+    ctx.heff_apply_device(xd.address, yd.address, x.size)
+    y_dev = yd.download()
+    xd.free()
+    yd.free()
     assert np.array_equal(y_dev, y_host)  # same kernels, same order
     y_ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
     assert np.linalg.norm(y_dev - y_ref) / np.linalg.norm(y_ref) < tol
