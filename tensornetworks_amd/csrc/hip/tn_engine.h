@@ -55,7 +55,9 @@ struct Scalars {
 struct DevMix {
     DBuf buf;
     MixTable t{};
-    // key
+    // key: the W values themselves (a caller may refill one host buffer with
+    // another operator of the same shape, so its address identifies nothing)
+    std::vector<double> wkey;
     const double* wsrc = nullptr;
     int op = -1, D1p = -1, D2p = -1, p = -1, dw1 = -1, dw2 = -1, wl = -1, wr = -1;
 };

@@ -168,9 +168,12 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
                hipStream_t s, int wl, int wr)
 {
     if (op != MIX_HEFF && op != MIX_HEFF_T) wl = wr = -1;
+    const size_t nw = (size_t)dw1 * dw2 * p * p;
     if (m.wsrc == W && m.op == op && m.D1p == D1p && m.D2p == D2p && m.p == p && m.dw1 == dw1 &&
-        m.dw2 == dw2 && m.wl == wl && m.wr == wr)
+        m.dw2 == dw2 && m.wl == wl && m.wr == wr && m.wkey.size() == nw &&
+        std::memcmp(m.wkey.data(), W, nw * sizeof(double)) == 0)
         return;
+    m.wkey.clear();  // set again once the new table is complete
     const bool ident = wl >= 0 || wr >= 0 || op == MIX_HEFF_T;
     const bool heff = op == MIX_HEFF || op == MIX_HEFF_T;
     // transposed stage-3 operand: row (m, b), column (w, a'), ldo = nL * D1p
@@ -262,6 +265,7 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     m.t.esel = ident ? (const int*)(b + o6) : nullptr;
     m.t.osel = ident ? (const int*)(b + o7) : nullptr;
     m.wsrc = W;
+    m.wkey.assign(W, W + nw);
     m.op = op;
     m.D1p = D1p;
     m.D2p = D2p;

@@ -235,3 +235,20 @@ def test_checkpoint_rejects_corruption(ctx, tmp_path):
     with pytest.raises(tn.TnError):
         tn.Chain.load(ctx, str(tmp_path / "missing.tnck"))
     ch.close()
+
+
+def test_rebind_same_shape_new_operator(ctx):
+    # a second tn_site_bind with the same shapes and another W (the caller's
+    # host array refilled in place) must apply the new operator, not a cached one
+    d, D1, D2 = 2, 4, 6
+    L, R = sym_env(5, D1, 101), sym_env(5, D2, 102)
+    x = rand(d * D1 * D2, 103)
+    W = sym_W(5, 5, d, 104)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    y1 = ctx.heff_apply(x)
+    W[...] = sym_W(5, 5, d, 105)
+    ctx.site_bind(W, L, R, d, D1, D2, tn.TN_FP64)
+    y2 = ctx.heff_apply(x)
+    ref = O.heff_apply_ref(W, L, R, x, d, D1, D2)
+    assert np.abs(y2 - ref).max() < 1e-13 * np.abs(ref).max()
+    assert np.abs(y1 - y2).max() > 1e-3
