@@ -83,6 +83,18 @@ int tn_env_update(tn_ctx* ctx, int dir, const double* M, double* Fout);
  * call (the library's stream does not wait on other streams); returns when y
  * is written.  No PCIe traffic: the layout change runs on the device. */
 int tn_heff_apply_device(tn_ctx* ctx, const double* x_dev, double* y_dev, int64_t N);
+/* tn_heff_apply_device that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc.  Every
+ * intermediate (x and y in the internal layout, the stage buffers) lives in a
+ * fixed 64 MiB work space per GPU, a device array the library holds from load
+ * until the process ends; its GEMMs run in one launch each with the tuning
+ * table's tile (no k-split, never a tuning run).  A site whose work space
+ * exceeds 64 MiB (for FP64 about d*D1*D2*(2 + 2*Dw) doubles, e.g. d=3 D=400
+ * Dw=5) is rejected with TN_ERR_ARG.  The site must be bound with
+ * tn_site_bind first (binding allocates).  Calls that overlap on one GPU
+ * (several contexts or threads at once) share the work space and race. */
+int tn_heff_apply_device_noalloc(tn_ctx* ctx, const double* x_dev, double* y_dev, int64_t N);
 /* Lowest eigenpair(s) of the bound Heff by Lanczos with full FP64
  * re-orthogonalisation; replaces LapackEigenSolver<dcmplx>::Solve(Heff, eps, 2)
  * (NumericalMethods/LapackEigenSolver.C:62-109: IL=1, IU=2).  evec holds nev

@@ -139,6 +139,7 @@ def _load():
         "tn_chain_expectation": (i, [v, _ip, _ip, _dp, _dp]),
         "tn_chain_sweep2": (i, [v, i, i, d, ctypes.POINTER(SweepOpts), ctypes.POINTER(SweepStats), _dp]),
         "tn_heff_apply_device": (i, [v, v, v, i64]),
+        "tn_heff_apply_device_noalloc": (i, [v, v, v, i64]),
         "tn_qr": (i, [v, i, i, i, _dp, _dp, _dp]),
         "tn_publish_test": (i, [v, i, _dp, _dp]),
         "tn_chain_norm_status": (i, [v, d, ctypes.c_char_p]),
@@ -311,6 +312,11 @@ class Context:
         """y = Heff x with x, y device pointers (reference layout, n doubles)."""
         self.check(lib.tn_heff_apply_device(self.h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), int(n)),
                    "tn_heff_apply_device")
+
+    def heff_apply_device_noalloc(self, x_ptr, y_ptr, n):
+        """tn_heff_apply_device through the fixed work space (allocates nothing)."""
+        self.check(lib.tn_heff_apply_device_noalloc(self.h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), int(n)),
+                   "tn_heff_apply_device_noalloc")
 
     def qr(self, A, mode=0):
         """Thin QR (mode 0, m >= n: A = Q R) or LQ (mode 1, m <= n: A = L Q) on

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "tn_chain.h"
 
@@ -612,6 +613,35 @@ void chain_reserve(tn_chain* ch, int Dcap)
 
 }  // namespace tn
 
+// ---------------------------------------------------------------------------
+// Work space of the allocation-free Heff apply (tn_heff_apply_device_noalloc):
+// one file-scope device array, which every GPU the library runs on holds until
+// the process ends.  Host code never uses the array's name as an address: its
+// device address is looked up per device (hipGetSymbolAddress, with that
+// device current) when a context is created, and only that address is passed
+// to kernels.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// device address of this device's work space array (the device must be current)
+static void* resolve_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    TN_REQUIRE(device >= 0 && device < kMaxDevices, TN_ERR_ARG, "device index beyond the work space table");
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        TN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TN_REQUIRE(p != nullptr, TN_ERR_HIP, "hipGetSymbolAddress returned no address for the work space");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
 // ===========================================================================
 // C ABI
 extern "C" {
@@ -644,6 +674,7 @@ int tn_ctx_create(int device, tn_ctx** out)
     c->sc.flag_h = (unsigned long long*)((char*)c->sc.h + pub);
     c->sc.flag_d = (unsigned long long*)((char*)c->sc.d + pub);
     *c->sc.flag_h = 0;
+    c->noalloc = resolve_scratch(device);
     *out = c;
     return TN_OK;
     }
@@ -1078,6 +1109,37 @@ int tn_heff_apply_device(tn_ctx* c, const double* x, double* y, int64_t N)
     HeffOp op = bound_op(c);
     heff_apply(c, op, c->b_x.as<double>(), c->b_y.as<double>(), false);
     TN_HIP(psi_int_to_ref(c->b_y.as<double>(), y, d, D1, D2, D2p, c->stream));
+    sync(c);
+    return TN_OK;
+    ABI_END(c)
+}
+
+int tn_heff_apply_device_noalloc(tn_ctx* c, const double* x, double* y, int64_t N)
+{
+    if (!c) return TN_ERR_ARG;
+    ABI_BEGIN
+    TN_REQUIRE(c->bound, TN_ERR_STATE, "tn_heff_apply_device_noalloc: no site bound");
+    TN_REQUIRE(x && y && N == (int64_t)c->b_d * c->b_D1 * c->b_D2, TN_ERR_ARG,
+               "tn_heff_apply_device_noalloc: N != d*D1*D2");
+    TN_REQUIRE(c->noalloc != nullptr, TN_ERR_STATE, "tn_heff_apply_device_noalloc: no work space on this device");
+    const int d = c->b_d, D1 = c->b_D1, D2 = c->b_D2, D1p = pad32(D1), D2p = pad32(D2);
+    HeffOp op = bound_op(c);
+    // work space layout: x and y in the internal layout, then the stage buffers;
+    // every byte a call reads is written earlier in the same call
+    size_t b1 = 0, b2 = 0, bx = 0;
+    heff_scratch_bytes(op, &b1, &b2, &bx);
+    const size_t bv = ((size_t)op.n() * 8 + 255) / 256 * 256;
+    const size_t total = 2 * bv + b1 + b2 + bx;
+    TN_REQUIRE(total <= kNoAllocBytes, TN_ERR_ARG,
+               "tn_heff_apply_device_noalloc: this site needs " + std::to_string(total) +
+                   " bytes of work space, above the fixed " + std::to_string(kNoAllocBytes) + " bytes");
+    char* base = (char*)c->noalloc;
+    double* xi = (double*)base;
+    double* yi = (double*)(base + bv);
+    HeffScratch scr{base + 2 * bv, b1, base + 2 * bv + b1, b2, bx ? base + 2 * bv + b1 + b2 : nullptr, bx};
+    TN_HIP(psi_ref_to_int(x, xi, d, D1, D2, D1p, D2p, c->stream));
+    heff_apply(c, op, xi, yi, false, &scr);
+    TN_HIP(psi_int_to_ref(yi, y, d, D1, D2, D2p, c->stream));
     sync(c);
     return TN_OK;
     ABI_END(c)

@@ -99,6 +99,8 @@ struct tn_ctx {
     tn::DBuf ident;
     // deflation of the lowest eigenvector for the second eigenpair (dot + partials)
     tn::DBuf defl;
+    // device address of this device's fixed work space (allocation-free apply)
+    void* noalloc = nullptr;
     // stats whose Heff event timings are recorded but not yet read
     tn_sweep_stats* heff_pending = nullptr;
     // seam state (tn_site_bind)
