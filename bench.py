@@ -25,7 +25,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MI355X TFLOP/s
+PEAKS = {"bf16": 2500.0, "bf16x3": 2500.0, "fp32": 157.3, "fp64": 78.6}  # dense MI355X TFLOP/s
 
 
 def parse():
@@ -37,9 +37,12 @@ def parse():
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--S", type=float, default=1.0)
     p.add_argument("--Dz", type=float, default=0.1)
-    p.add_argument("--precision", default="fp32", choices=["mixed", "bf16", "fp32", "fp64"],
-                   help="fp32: FP32 contractions (FP32 environments); mixed: BF16 contractions refined by "
-                        "FP32 ones; Krylov vectors, Ritz solve, gauge and energies are FP64 in every mode")
+    p.add_argument("--precision", default="fp32", choices=["mixed", "bf16", "bf16x3", "fp32", "fp64"],
+                   help="fp32: FP32 contractions (FP32 environments); bf16x3: Heff contractions with every "
+                        "operand split into three bfloat16 pieces, six cross products on the bfloat16 matrix "
+                        "cores with FP32 accumulation (FP32 environment updates); mixed: BF16 contractions "
+                        "refined by FP32 ones; Krylov vectors, Ritz solve, gauge and energies are FP64 in every "
+                        "mode")
     p.add_argument("--no-bf16-compare", action="store_true",
                    help="skip the extra timed sweep of the BF16-then-FP32 scheme")
     p.add_argument("--eig-tol", type=float, default=1e-8)
@@ -337,7 +340,7 @@ def main():
     import tensornetworks_amd as tn
     from tensornetworks_amd.models import xxz_mpo
 
-    P = {"bf16": tn.TN_BF16, "fp32": tn.TN_FP32, "fp64": tn.TN_FP64}
+    P = {"bf16": tn.TN_BF16, "bf16x3": tn.TN_BF16X3, "fp32": tn.TN_FP32, "fp64": tn.TN_FP64}
     prec = tn.TN_BF16 if a.precision == "mixed" else P[a.precision]
     if a.workload in ("batch", "precond", "twosite"):
         ctx = tn.Context(0)
@@ -371,6 +374,8 @@ def main():
     if a.precision == "mixed":
         o_mixed = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
                                        refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
+    elif a.precision == "bf16x3":
+        o_mixed = tn.SweepOpts.default(precision=prec, env_precision=tn.TN_FP32, refine=0, **common)
     else:
         o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, refine=0, **common)
     o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, refine=0, **common)
@@ -470,7 +475,7 @@ def main():
             ctx.tune_save(a.tune_save)
         # tile of the bulk-site stage-1 GEMM: x (D*d x D) times R^T (nR*D x D),
         # nR = right channels contracted (Dw, or Dw - 1 with the identity copied)
-        tname = {"fp32": "f32->f32", "fp64": "f64->f64"}.get(a.precision, "bf16->f32")
+        tname = {"fp32": "f32->f32", "fp64": "f64->f64", "bf16x3": "bf16x3->f32"}.get(a.precision, "bf16->f32")
         nR = int(round(f1 / (2.0 * d * a.D ** 3)))
         key = f"{tname} KK M={a.D * d} N={nR * a.D} K={a.D} "
         for line in report.splitlines():
@@ -489,7 +494,7 @@ def main():
     value = sti.heff_flops / max(sti.heff_ms, 1e-12) / 1e9
     achieved = f1 / (s1 * 1e-3) / 1e12
     hp = "bf16" if a.precision == "mixed" else a.precision
-    B1 = {"bf16": 2, "fp32": 4, "fp64": 8}[hp]
+    B1 = {"bf16": 2, "bf16x3": 6, "fp32": 4, "fp64": 8}[hp]
     out = {
         "metric": "DMRG sweep time + Heff TFLOP/s at D=1024; mixed vs FP64 at |dE/L|<=1e-8",
         "value": value,

@@ -34,8 +34,14 @@ enum tn_status {
     TN_ERR_NOMEM = -4
 };
 
-/* arithmetic of the contractions (Heff apply / environment updates) */
-enum tn_precision { TN_FP64 = 0, TN_FP32 = 1, TN_BF16 = 2 };
+/* arithmetic of the contractions (Heff apply / environment updates).
+ * TN_BF16X3: every FP64 operand is split into three bfloat16 pieces
+ * (x = x0 + x1 + x2, about 24 significant bits) and the contraction sums the
+ * six leading cross products x_i y_j (i + j <= 2) on the bfloat16 matrix
+ * cores with FP32 accumulation: FP32-class accuracy at bfloat16 throughput.
+ * Heff only (tn_sweep_opts.precision, tn_site_bind); environment updates
+ * take FP64, FP32 or BF16. */
+enum tn_precision { TN_FP64 = 0, TN_FP32 = 1, TN_BF16 = 2, TN_BF16X3 = 3 };
 
 /* ---- context ------------------------------------------------------------ */
 /* Replaces nothing in the reference (it has no device); owns the HIP stream
@@ -294,7 +300,8 @@ int tn_chain_heff_bench(tn_chain* chain, int site, int precision, int reps, int 
 
 /* One batched GEMM through the device kernel, for numerics tests of the
  * matrix-core path: C[b] = A[b] B[b] with operands rounded to tin
- * (enum tn_precision values 0 fp64, 1 fp32, 2 bf16) and the result stored as
+ * (enum tn_precision values 0 fp64, 1 fp32, 2 bf16, 3 split bf16 -- the
+ * last with a_kmajor = b_kmajor = 1 only) and the result stored as
  * tout before conversion back to double.  a_kmajor: A[b] is M x K row-major
  * (else K x M); b_kmajor: B[b] is N x K row-major (else K x N).  M, N multiples
  * of 32, K of 32.  tile: 0 automatic, else a forced tile code.  ksplit > 1

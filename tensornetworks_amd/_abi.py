@@ -8,7 +8,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libtn_hip.so")
 
-TN_FP64, TN_FP32, TN_BF16 = 0, 1, 2
+TN_FP64, TN_FP32, TN_BF16, TN_BF16X3 = 0, 1, 2, 3
 TN_OK, TN_WARN_NOT_CONVERGED = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -16,14 +16,15 @@ struct tn_chain {
     DBuf sites_buf, spare_buf, env_buf, bnd_buf;
     std::vector<double*> A, Lenv, Renv;
     // reduced-precision copies of the environments, one set per precision
-    // (index TN_FP32 / TN_BF16); each set is refreshed whenever an FP64
-    // environment changes
+    // (index TN_FP32 / TN_BF16 / TN_BF16X3, the last as three bfloat16 planes
+    // env_cap elements apart, the boundary ones 32*32*Dwmax apart); each set
+    // is refreshed whenever an FP64 environment changes
     struct LoSet {
         bool active = false;
         DBuf buf;
         std::vector<void*> L, R;
         void *bL = nullptr, *bR = nullptr;
-    } lo[3];
+    } lo[4];
     double *bL = nullptr, *bR = nullptr;  // boundary environments, fp64
     std::vector<DevMix> mixH, mixL, mixR;
     // bonds b = 0..L-2

@@ -80,7 +80,7 @@ void ensure_lo(tn_chain* ch, int prec)
     tn_chain::LoSet& s = ch->lo[prec];
     if (s.active) return;
     tn_ctx* c = ch->ctx;
-    const size_t es = prec == TN_FP32 ? 4 : 2;
+    const size_t es = lowprec_bytes(prec);
     s.buf.ensure((size_t)2 * ch->L * ch->env_cap * es + 2 * 32 * 32 * ch->Dwmax * es + 1024);
     char* base = (char*)s.buf.p;
     s.L.assign(ch->L, nullptr);
@@ -96,8 +96,8 @@ void ensure_lo(tn_chain* ch, int prec)
     make_lowprec(c, prec, ch->bL, s.bL, (long)32 * 32 * ch->Dwmax);
     make_lowprec(c, prec, ch->bR, s.bR, (long)32 * 32 * ch->Dwmax);
     for (int i = 0; i < ch->L; i++) {
-        make_lowprec(c, prec, ch->Lenv[i], s.L[i], (long)ch->env_cap);
-        make_lowprec(c, prec, ch->Renv[i], s.R[i], (long)ch->env_cap);
+        make_lowprec(c, prec, ch->Lenv[i], s.L[i], (long)ch->env_cap, (long)ch->env_cap);
+        make_lowprec(c, prec, ch->Renv[i], s.R[i], (long)ch->env_cap, (long)ch->env_cap);
     }
 }
 
@@ -130,21 +130,25 @@ void env_update(tn_chain* ch, int i, int dir, const tn_sweep_opts& o, tn_sweep_s
     const int d = ch->d, D1p = pad32(ch->D1[i]), D2p = pad32(ch->D2[i]);
     const int Dw1 = ch->dw1[i], Dw2 = ch->dw2[i];
     const int eprec = o.env_precision;
+    TN_REQUIRE(eprec >= TN_FP64 && eprec <= TN_BF16, TN_ERR_ARG,
+               "env_precision must be FP64, FP32 or BF16 (the split-bfloat16 mode is for Heff)");
     double t0 = now_ms();
     if (dir == 0) {
         mix_build(ch->mixL[i], MIX_ENVL, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
         const double* Lin = i > 0 ? ch->Lenv[i - 1] : ch->bL;
         env_left(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixL[i], Lin, lo_left(ch, eprec, i), ch->A[i], ch->Lenv[i],
                  o.gemm_tile);
-        for (int p = TN_FP32; p <= TN_BF16; p++)
-            if (ch->lo[p].active) make_lowprec(c, p, ch->Lenv[i], ch->lo[p].L[i], (long)D2p * Dw2 * D2p);
+        for (int p = TN_FP32; p <= TN_BF16X3; p++)
+            if (ch->lo[p].active)
+                make_lowprec(c, p, ch->Lenv[i], ch->lo[p].L[i], (long)D2p * Dw2 * D2p, (long)ch->env_cap);
     } else {
         mix_build(ch->mixR[i], MIX_ENVR, ch->W[i].data(), Dw1, Dw2, d, D1p, D2p, c->stream);
         const double* Rin = i < ch->L - 1 ? ch->Renv[i + 1] : ch->bR;
         env_right(c, eprec, d, D1p, D2p, Dw1, Dw2, ch->mixR[i], Rin, lo_right(ch, eprec, i), ch->A[i], ch->Renv[i],
                   o.gemm_tile);
-        for (int p = TN_FP32; p <= TN_BF16; p++)
-            if (ch->lo[p].active) make_lowprec(c, p, ch->Renv[i], ch->lo[p].R[i], (long)Dw1 * D1p * D1p);
+        for (int p = TN_FP32; p <= TN_BF16X3; p++)
+            if (ch->lo[p].active)
+                make_lowprec(c, p, ch->Renv[i], ch->lo[p].R[i], (long)Dw1 * D1p * D1p, (long)ch->env_cap);
     }
     if (st) {
         sync(c);
@@ -225,6 +229,10 @@ HeffOp make_heff(tn_chain* ch, int i, int prec, int tile, int wl, int wr, int mi
     op.precision = prec;
     op.Llo = lo_left(ch, prec, i);
     op.Rlo = lo_right(ch, prec, i);
+    if (prec == TN_BF16X3) {
+        op.Lps = i > 0 ? (long)ch->env_cap : (long)32 * 32 * ch->Dwmax;
+        op.Rps = i < ch->L - 1 ? (long)ch->env_cap : (long)32 * 32 * ch->Dwmax;
+    }
     op.wl_id = wl;
     op.wr_id = wr;
     mix_build(ch->mixH[i], mixop, ch->W[i].data(), op.Dw1, op.Dw2, ch->d, op.D1p, op.D2p, ch->ctx->stream,
@@ -252,7 +260,7 @@ const double* w_diag(tn_chain* ch, int i)
 // unit roundoff of the Heff operands: the relative rounding of an apply
 double unit_roundoff(int precision)
 {
-    return precision == TN_FP64 ? 0x1p-53 : precision == TN_FP32 ? 0x1p-24 : 0x1p-8;
+    return precision == TN_FP64 ? 0x1p-53 : (precision == TN_FP32 || precision == TN_BF16X3) ? 0x1p-24 : 0x1p-8;
 }
 
 // local eigensolve of site i; returns (e0, e1)
@@ -300,7 +308,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
         if (st) st->lanczos_ms += now_ms() - t0;
         return;
     }
-    const bool refining = o.refine && o.refine_precision < o.precision;
+    const bool refining = o.refine && unit_roundoff(o.refine_precision) < unit_roundoff(o.precision);
     const double rel = refining ? (o.precision == TN_BF16 ? 4e-3 : 4e-7) : 0.0;
     LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel,
                               o.reorth == 1, unit_roundoff(o.precision));
@@ -696,7 +704,7 @@ int tn_ctx_tune_report(tn_ctx* c, char* buf, int len)
 {
     if (!c || !buf || len <= 0) return TN_ERR_ARG;
     std::string out;
-    static const char* tn_[] = {"f64", "f32", "bf16"};
+    static const char* tn_[] = {"f64", "f32", "bf16", "bf16x3"};
     for (const auto& kv : c->tune) {
         const auto& k = kv.first;
         char line[192];
@@ -723,7 +731,7 @@ int tn_ctx_tune_load(tn_ctx* c, const char* path)
                   &v[7], &v[8], &v[9], &v[10]) == 11) {
         // a malformed line is skipped: element types in {f64, f32, bf16},
         // layout and accumulate flags 0/1, positive shape, a known tile and k-split
-        const bool types_ok = v[0] >= 0 && v[0] <= 2 && v[1] >= 0 && v[1] <= 2;
+        const bool types_ok = v[0] >= 0 && v[0] <= 3 && v[1] >= 0 && v[1] <= 2;
         const bool flags_ok = (v[2] == 0 || v[2] == 1) && (v[3] == 0 || v[3] == 1) && (v[8] == 0 || v[8] == 1);
         const bool shape_ok = v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] > 0;
         if (!types_ok || !flags_ok || !shape_ok) continue;
@@ -836,7 +844,7 @@ int tn_site_bind(tn_ctx* c, int d, int D1, int D2, int Dw1, int Dw2, const doubl
     if (!c) return TN_ERR_ARG;
     ABI_BEGIN
     TN_REQUIRE(d > 0 && D1 > 0 && D2 > 0 && Dw1 > 0 && Dw2 > 0 && W && L && R, TN_ERR_ARG, "tn_site_bind: bad arguments");
-    TN_REQUIRE(precision >= TN_FP64 && precision <= TN_BF16, TN_ERR_ARG, "tn_site_bind: bad precision");
+    TN_REQUIRE(precision >= TN_FP64 && precision <= TN_BF16X3, TN_ERR_ARG, "tn_site_bind: bad precision");
     const int D1p = pad32(D1), D2p = pad32(D2);
     c->b_d = d; c->b_D1 = D1; c->b_D2 = D2; c->b_Dw1 = Dw1; c->b_Dw2 = Dw2; c->b_prec = precision;
     c->b_W.assign(W, W + (size_t)Dw1 * Dw2 * d * d);
@@ -853,8 +861,8 @@ int tn_site_bind(tn_ctx* c, int d, int D1, int D2, int Dw1, int Dw2, const doubl
     h2d(c, c->b_L.p, Li.data(), Li.size() * 8);
     h2d(c, c->b_R.p, Ri.data(), Ri.size() * 8);
     if (precision != TN_FP64) {
-        c->b_Llo.ensure(Li.size() * 4);
-        c->b_Rlo.ensure(Ri.size() * 4);
+        c->b_Llo.ensure(Li.size() * lowprec_bytes(precision));
+        c->b_Rlo.ensure(Ri.size() * lowprec_bytes(precision));
         make_lowprec(c, precision, c->b_L.as<double>(), c->b_Llo.p, (long)Li.size());
         make_lowprec(c, precision, c->b_R.as<double>(), c->b_Rlo.p, (long)Ri.size());
     }
@@ -898,6 +906,10 @@ static HeffOp bound_op(tn_ctx* c)
     op.R = c->b_R.as<double>();
     op.Llo = c->b_prec == TN_FP64 ? nullptr : c->b_Llo.p;
     op.Rlo = c->b_prec == TN_FP64 ? nullptr : c->b_Rlo.p;
+    if (c->b_prec == TN_BF16X3) {
+        op.Lps = (long)pad32(c->b_D1) * c->b_Dw1 * pad32(c->b_D1);
+        op.Rps = (long)c->b_Dw2 * pad32(c->b_D2) * pad32(c->b_D2);
+    }
     op.precision = c->b_prec;
     op.mix = &c->b_mix_heff;
     op.tile = 0;
@@ -950,7 +962,8 @@ int tn_env_update(tn_ctx* c, int dir, const double* M, double* Fout)
     const int Dw1 = c->b_Dw1, Dw2 = c->b_Dw2;
     c->b_M.ensure((size_t)D1p * d * D2p * 8);
     psi_to_dev(c, M, c->b_M.as<double>());
-    const int prec = c->b_prec;
+    // the split-bfloat16 binding updates environments in FP64
+    const int prec = c->b_prec == TN_BF16X3 ? TN_FP64 : c->b_prec;
     if (dir == 0) {
         mix_build(c->b_mix_envl, MIX_ENVL, c->b_W.data(), Dw1, Dw2, d, D1p, D2p, c->stream);
         c->b_y.ensure((size_t)D2p * Dw2 * D2p * 8);
@@ -2028,7 +2041,15 @@ int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, i
     h2d(c, b64.p, B, nB * 8);
     const void* Ad = a64.p;
     const void* Bd = b64.p;
-    if (tin != T_F64) {
+    if (tin == T_BF16X3) {
+        TN_REQUIRE(akm && bkm, TN_ERR_ARG, "tn_gemm_test: the split-bfloat16 GEMM takes K-major operands");
+        alo.ensure(nA * 6);
+        blo.ensure(nB * 6);
+        TN_HIP(split3(T_F64, a64.p, alo.p, nA, nA, c->stream));
+        TN_HIP(split3(T_F64, b64.p, blo.p, nB, nB, c->stream));
+        Ad = alo.p;
+        Bd = blo.p;
+    } else if (tin != T_F64) {
         alo.ensure(nA * 4);
         blo.ensure(nB * 4);
         TN_HIP(convert(T_F64, a64.p, tin, alo.p, nA, c->stream));
@@ -2045,6 +2066,7 @@ int tn_gemm_test(tn_ctx* c, int tin, int tout, int akm, int bkm, int M, int N, i
     g.B = Bd; g.ldb = bkm ? K : N; g.sB = (long)K * N;
     g.C = cout.p; g.ldc = N; g.sC = (long)M * N;
     g.alpha = 1.0f; g.tile = tile;
+    g.psA = nA; g.psB = nB;
     gemm_split(c, g, ksplit);
     TN_HIP(convert(tout, cout.p, T_F64, c64.p, nC, c->stream));
     d2h(c, C, c64.p, nC * 8);

@@ -132,6 +132,7 @@ struct HeffOp {
     const double* R;
     const void* Llo;  // low precision copies (nullptr when precision == FP64)
     const void* Rlo;
+    long Lps = 0, Rps = 0;  // TN_BF16X3 copies: element stride between the three planes
     int precision;
     const DevMix* mix;  // MIX_HEFF table for (D1p, D2p, wl_id, wr_id)
     int tile;
@@ -178,8 +179,11 @@ void env_left(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, co
               const double* Lin, const void* Lin_lo, const double* A, double* Lout, int tile);
 void env_right(tn_ctx* c, int prec, int p, int D1p, int D2p, int Dw1, int Dw2, const DevMix& mix,
                const double* Rin, const void* Rin_lo, const double* A, double* Rout, int tile);
-// store a low precision copy of an fp64 buffer
-void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n);
+// store a low precision copy of an fp64 buffer (TN_BF16X3: three bfloat16
+// planes ps elements apart, ps = 0 meaning n)
+void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n, long ps = 0);
+// bytes per element of a low precision copy (all planes)
+inline size_t lowprec_bytes(int prec) { return prec == TN_FP32 ? 4 : (prec == TN_BF16X3 ? 6 : 2); }
 
 // --- Lanczos ---------------------------------------------------------------
 struct LanczosResult {

@@ -310,7 +310,7 @@ size_t heff_scratch_bytes(const HeffOp& op, size_t* t1, size_t* t2, size_t* xlo)
 {
     const int nR = op.Dw2 - (op.wr_id >= 0 ? 1 : 0), nL = op.Dw1 - (op.wl_id >= 0 ? 1 : 0);
     const size_t n1 = (size_t)op.D1p * op.p * nR * op.D2p, n2 = (size_t)nL * op.D1p * op.p * op.D2p;
-    const size_t es = op.precision == TN_FP64 ? 8 : (op.precision == TN_FP32 ? 4 : 2);
+    const size_t es = op.precision == TN_FP64 ? 8 : lowprec_bytes(op.precision);
     const size_t a = op.precision == TN_FP64 ? n1 * 8 : n1 * 4;
     const size_t b = n2 * es;
     const size_t x = op.precision == TN_FP64 ? 0 : (size_t)op.n() * es;
@@ -398,6 +398,30 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
         g3.tin = T_F64; g3.tout = T_F64; g3.A = op.L + loff; g3.B = T2; g3.C = y;
         run(g3);
+    } else if (op.precision == TN_BF16X3) {
+        // split-bfloat16 contractions: x, the environments and the MPO-stage
+        // output as three bfloat16 planes, stage 1 into FP32, stage 3 into y
+        if (!scr) {
+            c->xlo.ensure((size_t)n * 6);
+            c->T1.ensure(n1 * 4);
+            c->T2.ensure((size_t)n2 * 6);
+            XL = c->xlo.p;
+            T1 = c->T1.p;
+            T2 = c->T2.p;
+        }
+        TN_REQUIRE(op.Lps > 0 && op.Rps > 0, TN_ERR_STATE, "heff_apply: split-bfloat16 operator without plane strides");
+        TN_HIP(split3(T_F64, x, XL, n, n, s));
+        if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
+        g1.tin = T_BF16X3; g1.tout = T_F32; g1.A = XL; g1.psA = n;
+        g1.B = (const bf16*)op.Rlo + roff; g1.psB = op.Rps; g1.C = T1;
+        run(g1);
+        if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
+        TN_HIP(wmix_idt(T_F32, T_F64, T_BF16X3, T1, (long)p * nR * D2p, x, (long)p * D2p, T2, (long)nL * D1p, y,
+                        (long)p * D2p, D1p, D2p, op.mix->t, s, n2));
+        if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
+        g3.tin = T_BF16X3; g3.tout = T_F64; g3.A = (const bf16*)op.Llo + loff; g3.psA = op.Lps;
+        g3.B = T2; g3.psB = n2; g3.C = y;
+        run(g3);
     } else {
         const int tl = lo_type(op.precision);
         const size_t es = dtype_size(tl);
@@ -447,9 +471,13 @@ void heff_stage_times(tn_ctx* c, double* s1, double* mix, double* s3)
     *s3 = t34;
 }
 
-void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n)
+void make_lowprec(tn_ctx* c, int prec, const double* src, void* dst, long n, long ps)
 {
     if (prec == TN_FP64) return;
+    if (prec == TN_BF16X3) {
+        TN_HIP(split3(T_F64, src, dst, n, ps > 0 ? ps : n, c->stream));
+        return;
+    }
     TN_HIP(convert(T_F64, src, lo_type(prec), dst, n, c->stream));
 }
 

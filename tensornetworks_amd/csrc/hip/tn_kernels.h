@@ -10,7 +10,12 @@ namespace tn {
 typedef uint16_t bf16;  // storage type of bfloat16 operands
 
 enum Prec { P_FP64 = 0, P_FP32 = 1, P_BF16 = 2 };
-enum DType { T_F64 = 0, T_F32 = 1, T_BF16 = 2 };
+// T_BF16X3: a value held as three bfloat16 planes v = v0 + v1 + v2 (each the
+// rounding of what the previous ones leave, about 24 significant bits
+// together); products of such operands sum the six leading cross products
+// A_i B_j (i + j <= 2) on the bfloat16 matrix cores in FP32.  dtype_size is
+// the size of one plane's element.
+enum DType { T_F64 = 0, T_F32 = 1, T_BF16 = 2, T_BF16X3 = 3 };
 
 inline size_t dtype_size(int t) { return t == T_F64 ? 8 : (t == T_F32 ? 4 : 2); }
 
@@ -29,6 +34,9 @@ struct GemmDesc {
     float alpha;
     int accumulate;
     int tile;  // 0 = automatic, else forced tile code (autotuner)
+    // T_BF16X3 operands: plane p of A starts at A + p*psA elements (likewise
+    // B); a_kmajor && b_kmajor only
+    long psA = 0, psB = 0;
 };
 hipError_t gemm(const GemmDesc& g, hipStream_t s);
 
@@ -86,8 +94,11 @@ hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx
 // columns (w, a')),
 // through an LDS tile of 32 x-values by 64 b-values so both the reads (along
 // b) and the writes (along x) are coalesced.  X must be a multiple of 32.
+// tout = T_BF16X3: out is written as three bfloat16 planes ops elements apart
+// (with tin = T_F32 and the second input tx = T_F64)
 hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
-                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s);
+                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s,
+                    long ops = 0);
 
 // Small device -> host transfer without a stream synchronisation: copies n
 // (<= TN_PUBLISH_MAX) doubles to host-mapped coherent memory, then
@@ -98,6 +109,8 @@ hipError_t publish(const double* src, int n, double* host_dst, unsigned long lon
 
 // elementwise
 hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s);
+// three bfloat16 planes of n values (T_F64 or T_F32 input): out[p*ps + i], p = 0..2
+hipError_t split3(int tin, const void* in, void* out, long n, long ps, hipStream_t s);
 hipError_t fill_zero(void* p, long bytes, hipStream_t s);
 hipError_t fill_value(double* p, double v, long n, hipStream_t s);
 

@@ -51,6 +51,25 @@ template <typename A, typename T> __device__ __forceinline__ A to_acc(T v)
     else return to_f(v);
 }
 
+// v = h0 + h1 + h2 + O(2^-24 |v|): each bfloat16 piece is the rounding of what
+// the previous ones leave (the differences are exact in the wider type)
+__device__ __forceinline__ void split3_val(double v, bf16& h0, bf16& h1, bf16& h2)
+{
+    h0 = f2bf((float)v);
+    double r = v - (double)bf2f(h0);
+    h1 = f2bf((float)r);
+    r -= (double)bf2f(h1);
+    h2 = f2bf((float)r);
+}
+__device__ __forceinline__ void split3_val(float v, bf16& h0, bf16& h1, bf16& h2)
+{
+    h0 = f2bf(v);
+    float r = v - bf2f(h0);
+    h1 = f2bf(r);
+    r -= bf2f(h1);
+    h2 = f2bf(r);
+}
+
 // ---------------------------------------------------------------------------
 // MFMA traits: 16x16 output block per wave instruction
 //   A frag: lane l holds A[l%16][kbase + (l/16)*KPL + 0..KPL-1]
@@ -280,6 +299,150 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
                                             p.accumulate, tm * BM, tn * BN, As, Bs);
 }
 
+// One BM x BN output tile of the split-bfloat16 product: A and B each held as
+// three bfloat16 planes (A = A0 + A1 + A2), K-major, every plane staged
+// through LDS; each K step issues the six leading cross products A_i B_j
+// (i + j <= 2) of v_mfma_f32_16x16x32_bf16 into one FP32 accumulator.  Same
+// wave layout, register prefetch and wave-priority scheme as gemm_tile.
+template <typename TO, int BM, int BN, int BK>
+__device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long psA, const bf16* __restrict__ B,
+                                             long psB, TO* __restrict__ C, long lda, long ldb, long ldc, int K,
+                                             float alpha, int accumulate, int m0, int n0, bf16* __restrict__ As,
+                                             bf16* __restrict__ Bs)
+{
+    typedef MT<bf16> T;
+    typedef T::frag_t frag_t;
+    typedef T::acc_t acc_t;
+    constexpr int LDK = BK + T::PAD;
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int TA = BM * BK / T::VEC, TB = BN * BK / T::VEC;
+    constexpr int RA = (TA + 255) / 256, RB = (TB + 255) / 256;
+    constexpr int SA = BM * LDK, SB = BN * LDK;
+    static_assert(FM >= 1 && FN >= 1 && BK % T::MK == 0, "tile");
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
+
+    u32x4 ra[3][RA], rb[3][RB];
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+
+    const int nk = K / BK;
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        tile_gload<bf16, BM, BK, true>(ra[p], A + p * psA, lda, m0, 0, tid);
+        tile_gload<bf16, BN, BK, true>(rb[p], B + p * psB, ldb, n0, 0, tid);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        tile_sstore<bf16, BM, BK, true>(ra[p], As + p * SA, tid);
+        tile_sstore<bf16, BN, BK, true>(rb[p], Bs + p * SB, tid);
+    }
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        if (t + 1 < nk) {
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                tile_gload<bf16, BM, BK, true>(ra[p], A + p * psA, lda, m0, (t + 1) * BK, tid);
+                tile_gload<bf16, BN, BK, true>(rb[p], B + p * psB, ldb, n0, (t + 1) * BK, tid);
+            }
+        }
+        __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += T::MK) {
+            frag_t a[3][FM], b[3][FN];
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+                    a[p][i] = *(const frag_t*)(As + p * SA + (wm0 + i * 16 + (lane & 15)) * LDK + kk +
+                                               (lane >> 4) * T::KPL);
+#pragma unroll
+                for (int j = 0; j < FN; j++)
+                    b[p][j] = *(const frag_t*)(Bs + p * SB + (wn0 + j * 16 + (lane & 15)) * LDK + kk +
+                                               (lane >> 4) * T::KPL);
+            }
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++) {
+                    // smallest terms first
+                    acc[i][j] = mma(a[2][i], b[0][j], acc[i][j]);
+                    acc[i][j] = mma(a[0][i], b[2][j], acc[i][j]);
+                    acc[i][j] = mma(a[1][i], b[1][j], acc[i][j]);
+                    acc[i][j] = mma(a[1][i], b[0][j], acc[i][j]);
+                    acc[i][j] = mma(a[0][i], b[1][j], acc[i][j]);
+                    acc[i][j] = mma(a[0][i], b[0][j], acc[i][j]);
+                }
+        }
+        __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+        if (t + 1 < nk) {
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                tile_sstore<bf16, BM, BK, true>(ra[p], As + p * SA, tid);
+                tile_sstore<bf16, BN, BK, true>(rb[p], Bs + p * SB, tid);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long row = m0 + wm0 + i * 16 + T::crow(lane, r);
+                const long col = n0 + wn0 + j * 16 + (lane & 15);
+                TO* cp = C + row * ldc + col;
+                if constexpr (std::is_same<TO, double>::value) {
+                    double v = (double)acc[i][j][r] * (double)alpha;
+                    if (accumulate) v += *cp;
+                    *cp = v;
+                } else {
+                    float v = acc[i][j][r] * alpha;
+                    if (accumulate) v += to_f(*cp);
+                    *cp = cvt_from_f<TO>(v);
+                }
+            }
+}
+
+struct GemmX3Params {
+    const bf16* A;
+    const bf16* B;
+    void* C;
+    int M, N, K;
+    long lda, ldb, ldc, sA, sB, sC, psA, psB;
+    float alpha;
+    int accumulate;
+};
+
+template <typename TO, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) gemm_x3_k(GemmX3Params p)
+{
+    constexpr int LDK = BK + MT<bf16>::PAD;
+    __shared__ __attribute__((aligned(16))) bf16 As[3 * BM * LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[3 * BN * LDK];
+    const int ntn = p.N / BN, ntm = p.M / BM;
+    const int nblk = ntn * ntm;
+    int bid = blockIdx.x;
+    // XCD-aware tile order as in gemm_k
+    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    constexpr int GM = 8;
+    const int gsz = GM * ntn, g = bid / gsz, first = g * GM;
+    const int rows = ntm - first < GM ? ntm - first : GM;
+    const int local = bid - g * gsz;
+    const int tm = first + local % rows, tn = local / rows;
+    const long bz = blockIdx.y;
+    gemm_tile_x3<TO, BM, BN, BK>(p.A + bz * p.sA, p.psA, p.B + bz * p.sB, p.psB, (TO*)p.C + bz * p.sC, p.lda, p.ldb,
+                                 p.ldc, p.K, p.alpha, p.accumulate, tm * BM, tn * BN, As, Bs);
+}
+
 // Grouped GEMM: one persistent launch over the output tiles of many problems
 // (problem q owns tiles [off[q], off[q+1]), row-major over its N/BN columns).
 // Workgroups stride over the tile list; the owning problem is found by a
@@ -387,12 +550,53 @@ static hipError_t grouped_layout(int tin, int tout, int code, int bk, const Gemm
     return hipErrorInvalidValue;
 }
 
+template <typename TO>
+static hipError_t launch_x3(int code, const GemmX3Params& p, int batch, hipStream_t s)
+{
+    const TileCfg& t = kTiles[code];
+    dim3 grid((p.M / t.bm) * (p.N / t.bn), batch);
+    switch (code) {
+    case 0: gemm_x3_k<TO, 128, 128, 32><<<grid, 256, 0, s>>>(p); break;
+    case 1: gemm_x3_k<TO, 128, 64, 32><<<grid, 256, 0, s>>>(p); break;
+    case 2: gemm_x3_k<TO, 64, 128, 32><<<grid, 256, 0, s>>>(p); break;
+    case 3: gemm_x3_k<TO, 64, 64, 32><<<grid, 256, 0, s>>>(p); break;
+    case 4: gemm_x3_k<TO, 32, 64, 32><<<grid, 256, 0, s>>>(p); break;
+    case 5: gemm_x3_k<TO, 64, 32, 32><<<grid, 256, 0, s>>>(p); break;
+    case 6: gemm_x3_k<TO, 32, 32, 32><<<grid, 256, 0, s>>>(p); break;
+    // deep tiles: a K step of 64 per LDS round (half the barriers per MFMA)
+    case 7: gemm_x3_k<TO, 128, 128, 64><<<grid, 256, 0, s>>>(p); break;
+    case 8: gemm_x3_k<TO, 128, 64, 64><<<grid, 256, 0, s>>>(p); break;
+    default: gemm_x3_k<TO, 64, 128, 64><<<grid, 256, 0, s>>>(p); break;
+    }
+    return hipGetLastError();
+}
+
 static int choose_tile(int M, int N, int batch);
+// the split-bfloat16 product: K-major planes only, K a multiple of 32
+static hipError_t gemm_x3(const GemmDesc& g, hipStream_t s)
+{
+    if (!g.a_kmajor || !g.b_kmajor || g.K % 32 || g.psA <= 0 || g.psB <= 0) return hipErrorInvalidValue;
+    GemmX3Params p;
+    p.A = (const bf16*)g.A; p.B = (const bf16*)g.B; p.C = g.C;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.lda = g.lda; p.ldb = g.ldb; p.ldc = g.ldc;
+    p.sA = g.sA; p.sB = g.sB; p.sC = g.sC;
+    p.psA = g.psA; p.psB = g.psB;
+    p.alpha = g.alpha; p.accumulate = g.accumulate;
+    const int code = g.tile > 0 ? g.tile - 1 : choose_tile(g.M, g.N, g.batch);
+    if (code < 0 || code >= kNumTiles || g.M % kTiles[code].bm || g.N % kTiles[code].bn) return hipErrorInvalidValue;
+    if (kTiles[code].deep && g.K % 64) return hipErrorInvalidValue;
+    if (g.tout == T_F32) return launch_x3<float>(code, p, g.batch, s);
+    if (g.tout == T_F64) return launch_x3<double>(code, p, g.batch, s);
+    return hipErrorInvalidValue;
+}
+
 bool gemm_tile_ok(int code, int tin, int M, int N, int K)
 {
     if (code < 1 || code > kNumTiles) return false;
     const TileCfg& t = kTiles[code - 1];
     if (M % t.bm || N % t.bn) return false;
+    if (tin == T_BF16X3) return K % (t.deep ? 64 : 32) == 0;
     const int bk = gemm_k_step(tin, K);
     return K % (t.deep ? 2 * bk : bk) == 0;
 }
@@ -484,6 +688,7 @@ hipError_t gemm(const GemmDesc& g, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.batch <= 0) return hipErrorInvalidValue;
     if (g.M % 32 || g.N % 32 || g.K % 16) return hipErrorInvalidValue;
+    if (g.tin == T_BF16X3) return gemm_x3(g, s);
     if (g.a_kmajor && g.b_kmajor) return gemm_layout<true, true>(g, s);
     if (g.a_kmajor && !g.b_kmajor) return gemm_layout<true, false>(g, s);
     if (!g.a_kmajor && !g.b_kmajor) return gemm_layout<false, false>(g, s);
@@ -587,9 +792,9 @@ hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx
     return hipErrorInvalidValue;
 }
 
-template <typename TI, typename TX, typename TO>
+template <typename TI, typename TX, typename TO, bool X3 = false>
 __global__ void __launch_bounds__(256) wmix_idt_k(const TI* __restrict__ in, long isx, const TX* __restrict__ in2,
-                                                  long isx2, TO* __restrict__ out, long ldo,
+                                                  long isx2, TO* __restrict__ out, long ldo, long ops,
                                                   double* __restrict__ out2, long osx2, int Bn, int nout,
                                                   const int* __restrict__ optr, const long* __restrict__ ooff,
                                                   const long* __restrict__ eoff, const int* __restrict__ esel,
@@ -636,24 +841,54 @@ __global__ void __launch_bounds__(256) wmix_idt_k(const TI* __restrict__ in, lon
         __syncthreads();
         if (wok) {
             TO* dst = out + ooff[o] + (long)(blockIdx.x * 64 + wb) * ldo + (long)blockIdx.y * 32 + wc;
+            if constexpr (X3) {
+                // three bfloat16 planes, one 16-byte store per plane (8 values;
+                // every offset here is a multiple of 8 elements)
+                uint32_t h[3][8];
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                if constexpr (std::is_same<TO, double>::value) dst[c] = (double)tile[wb][wc + c];
-                else dst[c] = cvt_from_f<TO>((float)tile[wb][wc + c]);
+                for (int c = 0; c < 8; c++) {
+                    bf16 h0, h1, h2;
+                    split3_val((float)tile[wb][wc + c], h0, h1, h2);
+                    h[0][c] = h0;
+                    h[1][c] = h1;
+                    h[2][c] = h2;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    u32x4 w;
+                    w.x = h[q][0] | (h[q][1] << 16);
+                    w.y = h[q][2] | (h[q][3] << 16);
+                    w.z = h[q][4] | (h[q][5] << 16);
+                    w.w = h[q][6] | (h[q][7] << 16);
+                    *(u32x4*)(dst + q * ops) = w;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    if constexpr (std::is_same<TO, double>::value) dst[c] = (double)tile[wb][wc + c];
+                    else dst[c] = cvt_from_f<TO>((float)tile[wb][wc + c]);
+                }
             }
         }
     }
 }
 
 hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const void* in2, long isx2, void* out,
-                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s)
+                    long ldo, double* out2, long osx2, int X, int Bn, const MixTable& t, hipStream_t s, long ops)
 {
     if (!t.esel || !t.osel || X % 32) return hipErrorInvalidValue;
     dim3 grid((Bn + 63) / 64, X / 32);
+    if (tout == T_BF16X3) {
+        if (tin != T_F32 || tx != T_F64 || ops <= 0 || ops % 8 || ldo % 8) return hipErrorInvalidValue;
+        wmix_idt_k<float, double, bf16, true><<<grid, 256, 0, s>>>((const float*)in, isx, (const double*)in2, isx2,
+                                                                 (bf16*)out, ldo, ops, out2, osx2, Bn, t.nout, t.optr,
+                                                                 t.ooff, t.eoff, t.esel, t.osel, t.eval, t.evalf);
+        return hipGetLastError();
+    }
 #define WT_CASE(TIc, TXc, TOc, TIt, TXt, TOt)                                                              \
     if (tin == TIc && tx == TXc && tout == TOc) {                                                          \
         wmix_idt_k<TIt, TXt, TOt><<<grid, 256, 0, s>>>((const TIt*)in, isx, (const TXt*)in2, isx2,         \
-                                                       (TOt*)out, ldo, out2, osx2, Bn, t.nout, t.optr,     \
+                                                       (TOt*)out, ldo, 0, out2, osx2, Bn, t.nout, t.optr,  \
                                                        t.ooff, t.eoff, t.esel, t.osel, t.eval, t.evalf);   \
         return hipGetLastError();                                                                          \
     }
@@ -713,6 +948,30 @@ hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStre
     CV_CASE(T_F64, T_F64, double, double)
 #undef CV_CASE
     return hipErrorInvalidValue;
+}
+
+template <typename TI>
+__global__ void split3_k(const TI* __restrict__ in, bf16* __restrict__ out, long n, long ps)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        bf16 h0, h1, h2;
+        split3_val(in[i], h0, h1, h2);
+        out[i] = h0;
+        out[ps + i] = h1;
+        out[2 * ps + i] = h2;
+    }
+}
+
+hipError_t split3(int tin, const void* in, void* out, long n, long ps, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (ps < n) return hipErrorInvalidValue;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (tin == T_F64) split3_k<double><<<blocks, 256, 0, s>>>((const double*)in, (bf16*)out, n, ps);
+    else if (tin == T_F32) split3_k<float><<<blocks, 256, 0, s>>>((const float*)in, (bf16*)out, n, ps);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 __global__ void fill_zero_k(uint32_t* p, long n)

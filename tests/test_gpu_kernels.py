@@ -147,7 +147,8 @@ IDENT_CASES = [(2, 8, 8, 5, 5, 0, 4), (3, 27, 40, 5, 5, 4, 0), (2, 37, 19, 5, 5,
 
 
 @pytest.mark.parametrize("d,D1,D2,Dw1,Dw2,wl,wr", IDENT_CASES)
-@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2)])
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2),
+                                      (tn.TN_BF16X3, 5e-6)])
 def test_heff_apply_identity_channels_match_oracle(ctx, d, D1, D2, Dw1, Dw2, wl, wr, prec, tol):
     # an environment channel equal to the identity is applied as a copy of
     # the input (right side) or of the mixed slice into y (left side); the
@@ -288,3 +289,81 @@ def test_cholqr_polar_finish(ctx, cond):
     assert ok
     assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-11
     assert np.abs(Q @ R - A).max() < 1e-12 * np.abs(A).max() * np.sqrt(n)
+
+
+# ---- bulk shapes of the benched workload and the split-bfloat16 product ----
+
+@pytest.mark.parametrize("tin,tol", [(3, 3e-7), (1, 3e-7)])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_split_bf16_gemm_reaches_fp32_accuracy(ctx, tin, tol, tile):
+    # three bfloat16 pieces per operand and the six leading cross products:
+    # against the exact (FP64) product of the unrounded operands the error is
+    # FP32-class, far below a plain bfloat16 product (~1e-3)
+    M, N, K = 256, 384, 512
+    A = rand((1, M, K), 31)
+    B = rand((1, K, N), 32)
+    ref = A @ B
+    C = ctx.gemm_test(A, np.ascontiguousarray(B.transpose(0, 2, 1)), tin, 1, True, True, M, N, K, 1, tile)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < tol * np.sqrt(K), (tin, tile, err)
+    if tin == 3:
+        Cb = ctx.gemm_test(A, np.ascontiguousarray(B.transpose(0, 2, 1)), 2, 1, True, True, M, N, K, 1, tile)
+        assert np.abs(Cb - ref).max() / np.abs(ref).max() > 30 * err
+
+
+@pytest.mark.parametrize("tin", [1, 2, 3])
+@pytest.mark.parametrize("M,N,K,tile", [
+    (3072, 256, 256, 1),   # 24 row blocks of 128: three full groups of 8 (the bulk stage-1 row count)
+    (1664, 256, 128, 1),   # 13 row blocks: one group of 8 and a partial group of 5
+    (1664, 96, 64, 7),     # 52 blocks of 32 rows, block count not a multiple of 8 (no XCD remap)
+    (2176, 640, 96, 3),    # 34 row blocks of 64 x 10 column blocks
+])
+def test_gemm_tile_order_many_row_blocks(ctx, tin, M, N, K, tile):
+    # the XCD-aware, grouped tile order of gemm_k with more than 8 row blocks,
+    # partial last groups and block counts that are not a multiple of 8
+    A = rand((1, M, K), M + 1)
+    B = rand((1, K, N), N + 2)
+    ref = (ROUND[tin] if tin != 3 else (lambda x: x))(A) @ (ROUND[tin] if tin != 3 else (lambda x: x))(B)
+    C = ctx.gemm_test(A, np.ascontiguousarray(B.transpose(0, 2, 1)), tin, 1, True, True, M, N, K, 1, tile)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < (3e-7 * np.sqrt(K)), (tin, M, N, err)
+
+
+@pytest.mark.parametrize("tin", [1, 2, 3])
+@pytest.mark.parametrize("ks", [2, 4])
+def test_split_k_into_fp64_output(ctx, tin, ks):
+    # stage 3 of the benched apply: K split into ks chunks, partial products
+    # in FP32, reduced into an FP64 output
+    M, N, K = 1024, 384, 2048
+    A = rand((1, M, K), 41)
+    B = rand((1, K, N), 42)
+    Ar = A if tin == 3 else ROUND[tin](A)
+    Br = B if tin == 3 else ROUND[tin](B)
+    ref = Ar @ Br
+    C = ctx.gemm_test(A, np.ascontiguousarray(B.transpose(0, 2, 1)), tin, 0, True, True, M, N, K, 1, 0, ks)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < 3e-7 * np.sqrt(K), (tin, ks, err)
+
+
+@pytest.mark.parametrize("D", [256, 512])
+@pytest.mark.parametrize("ident", [False, True])
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP32, 5e-6), (tn.TN_BF16X3, 5e-6), (tn.TN_BF16, 2e-2)])
+def test_heff_apply_bulk_shapes_match_oracle(ctx, D, ident, prec, tol):
+    # the benched operator shapes (d=3, Dw=5, D1 = D2 = D): stage 1 is
+    # (3D x 4D or 5D) x D, stage 3 D x 3D x (4D or 5D), with and without the
+    # identity channels copied
+    d = 3
+    W = np.ascontiguousarray(O.xxz_bulk_W(d, 1.0, 1.0, 0.0, 0.1, 0))
+    L, R = rand(5 * D * D, 51 + D), rand(5 * D * D, 52 + D)
+    if ident:
+        L = with_identity_channel(L, 5, D, 0)
+        R = with_identity_channel(R, 5, D, 4)
+    x = rand(d * D * D, 53)
+    # the oracle's GEMM-ordered apply (OpenMP; pinned against the reference-
+    # layout apply in tests/test_oracle.py) keeps D=512 within seconds
+    y_ref = O.psi_int_to_ref(O.heff_apply_int(W, O.lenv_ref_to_int(L, 5, D), O.renv_ref_to_int(R, 5, D),
+                                              O.psi_ref_to_int(x, d, D, D), d, D, D), d, D, D)
+    ctx.site_bind(W, L, R, d, D, D, prec)
+    y = ctx.heff_apply(x)
+    err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    assert err < tol, err

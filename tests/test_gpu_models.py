@@ -99,8 +99,8 @@ def _s1_ed_chain(ctx, prec, refine):
     ch = tn.Chain(ctx, Ws, D)
     ch.initialize(1, 77)
     ch.normalize_qr(1)
-    o = tn.SweepOpts.default(eig_tol=1e-11, gauge=1, precision=prec, env_precision=(tn.TN_FP32 if prec == tn.TN_BF16
-                                                                                   else prec),
+    o = tn.SweepOpts.default(eig_tol=1e-11, gauge=1, precision=prec,
+                             env_precision=(tn.TN_FP32 if prec in (tn.TN_BF16, tn.TN_BF16X3) else prec),
                              refine=refine, refine_precision=tn.TN_FP32 if prec == tn.TN_BF16 else tn.TN_FP64,
                              refine_tol=1e-10, max_krylov=40, reorth=1)
     ch.load_caches(o)
@@ -111,7 +111,8 @@ def _s1_ed_chain(ctx, prec, refine):
 
 
 @pytest.mark.parametrize("prec,refine,tol", [(tn.TN_FP64, 0, 1e-10), (tn.TN_FP32, 0, 1e-8), (tn.TN_FP32, 1, 1e-9),
-                                             (tn.TN_BF16, 1, 1e-8)])
+                                             (tn.TN_BF16, 1, 1e-8), (tn.TN_BF16X3, 0, 1e-8),
+                                             (tn.TN_BF16X3, 1, 1e-9)])
 def test_s1_dz_sweep_matches_exact_diagonalisation(ctx, prec, refine, tol):
     L = 8
     ed = O.ed_ground(L, 3, Dz=0.1, tol=1e-13, maxit=400)
@@ -162,3 +163,6 @@ def test_mixed_precision_matches_fp64_at_bulk_shapes(ctx, tmp_path):
     ob = dict(base, refine=1, refine_precision=tn.TN_FP32)
     ebf = run(tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, **ob))
     assert abs(ebf - e64) / L <= 1e-8, (ebf, e64)
+    # three-piece bfloat16 contractions (FP32 environment updates)
+    ex3 = run(tn.SweepOpts.default(precision=tn.TN_BF16X3, env_precision=tn.TN_FP32, **base))
+    assert abs(ex3 - e64) / L <= 1e-8, (ex3, e64)

@@ -46,7 +46,8 @@ def apply_both(ctx, W, L, R, d, D1, D2, prec, seed):
     return out
 
 
-@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2)])
+@pytest.mark.parametrize("prec,tol", [(tn.TN_FP64, 1e-13), (tn.TN_FP32, 5e-6), (tn.TN_BF16, 2e-2),
+                                      (tn.TN_BF16X3, 5e-6)])
 def test_noalloc_apply_matches_allocating_apply(ctx, prec, tol):
     # call 1: d=3, D1=150, D2=131, identity channels on both sides (the
     # copied-channel path); call 2: a smaller site (d=2, 37 x 45, no identity)
