@@ -8,13 +8,16 @@ update), everything resident in HBM.
 
 Warmup: W sweeps from the random state in the headline precision bring the
 MPS close to its fixed point; the state is saved.  Timed region: K sweeps in
-the headline precision (default "fp32": Heff and environment contractions in
-FP32 on the matrix cores; Krylov vectors, re-orthogonalisation, tridiagonal
-solve, energies and QR/SVD gauge moves in FP64).  From the identical saved
-state the same K sweeps are then timed with BF16 contractions refined by FP32
-ones, and in FP64; |dE|/L compares the FP64 energies <H> of the final MPS of
-the headline and FP64 runs.  value = algorithmic Heff flops / device time of
-the Heff applications (HIP events) during the timed sweeps.
+the headline precision (default "bf16x3": every Heff operand split into three
+bfloat16 pieces, the six leading cross products on the bfloat16 matrix cores
+with FP32 accumulation, FP32 environment updates; Krylov vectors,
+re-orthogonalisation, tridiagonal solve, energies and QR/SVD gauge moves in
+FP64).  From the identical saved state the same K sweeps are then timed with
+every local solve refined by FP64 Lanczos iterations, with BF16 contractions
+refined by FP32 ones, and in FP64; |dE|/L compares the FP64 energies <H> of
+the final MPS of the headline and FP64 runs.  value = algorithmic Heff flops
+(identity channels counted, as if contracted) / device time of the Heff
+applications (HIP events) during the timed sweeps.
 """
 import argparse
 import json
@@ -37,7 +40,7 @@ def parse():
     p.add_argument("--D", type=int, default=1024)
     p.add_argument("--S", type=float, default=1.0)
     p.add_argument("--Dz", type=float, default=0.1)
-    p.add_argument("--precision", default="fp32", choices=["mixed", "bf16", "bf16x3", "fp32", "fp64"],
+    p.add_argument("--precision", default="bf16x3", choices=["mixed", "bf16", "bf16x3", "fp32", "fp64"],
                    help="fp32: FP32 contractions (FP32 environments); bf16x3: Heff contractions with every "
                         "operand split into three bfloat16 pieces, six cross products on the bfloat16 matrix "
                         "cores with FP32 accumulation (FP32 environment updates); mixed: BF16 contractions "
@@ -56,6 +59,9 @@ def parse():
     p.add_argument("--no-skip-identity", action="store_true",
                    help="contract identity environment channels like the others instead of copying them")
     p.add_argument("--no-fp64-compare", action="store_true")
+    p.add_argument("--no-refine-compare", action="store_true",
+                   help="skip the extra timed sweep in the headline precision with every local solve refined "
+                        "by FP64 Lanczos iterations")
     p.add_argument("--no-autotune", action="store_true", help="built-in GEMM tile choice instead of per-shape tuning")
     p.add_argument("--tune-db", default=os.path.join(ROOT, "tensornetworks_amd", "tuning", "mi355x_gemm.txt"),
                    help="saved GEMM tuning table to start from (shapes not in it are tuned on first use)")
@@ -78,29 +84,80 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(D, d, reps=4):
-    """The oracle's GEMM-ordered FP64 Heff apply (OpenMP) on the host cores:
-    a bounded sample of `reps` applies at the workload's bulk shape."""
+def _median_s(fn, reps=3):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(D, d, L, matvecs_per_sweep, Dz):
+    """Host-core baselines on bounded samples (medians of 3 runs each).
+
+    MF-CPU (the value): the oracle's GEMM-ordered FP64 Heff apply (OpenMP) at
+    the workload's bulk shape, in the unit of the GPU value; with the oracle's
+    environment update at the same shape it gives a sweep-time estimate,
+    matvecs-per-sweep x apply + 2L x environment update (gauge moves left
+    out, so a lower bound).  RF-CPU: the reference's own local-solve scheme
+    (GetHeff dense, then the lowest two eigenpairs with LAPACK, as
+    LapackEigenSolver does with zheevx; here dsyevr on the real matrix) at
+    the S=1/2 bulk shape of the reference's test chains, D in {8,16,32,48}."""
     import numpy as np
+    import scipy.linalg as sla
     import oracle as O
     rng = np.random.default_rng(1)
-    W = O.xxz_bulk_W(d, 1.0, 1.0, 0.0, 0.1, 0)
     Dw = 5
-    L = rng.uniform(-1, 1, D * Dw * D)
+    W = np.ascontiguousarray(O.xxz_bulk_W(d, 1.0, 1.0, 0.0, Dz, 0))
+    Lv = rng.uniform(-1, 1, D * Dw * D)
     R = rng.uniform(-1, 1, Dw * D * D)
     x = rng.uniform(-1, 1, D * d * D)
     work = np.zeros(O.lib.orc_heff_work_size(d, D, D, Dw, Dw))
     y = np.zeros_like(x)
-    Wc = np.ascontiguousarray(W)
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        O.lib.orc_heff_apply_int(d, D, D, Dw, Dw, O.P(Wc), O.P(L), O.P(R), O.P(x), O.P(y), O.P(work))
-    dt = (time.perf_counter() - t0) / reps
+    envo = np.zeros(D * Dw * D)
+    t_apply = _median_s(lambda: O.lib.orc_heff_apply_int(d, D, D, Dw, Dw, O.P(W), O.P(Lv), O.P(R), O.P(x), O.P(y),
+                                                         O.P(work)))
+    t_env = _median_s(lambda: O.lib.orc_env_left_int(d, D, D, Dw, Dw, O.P(W), O.P(Lv), O.P(x), O.P(envo),
+                                                     O.P(work)))
     flops = 4.0 * d * Dw * D ** 3 + 2.0 * d * d * Dw * Dw * D * D
+    rf = {}
+    W2 = np.ascontiguousarray(O.xxz_bulk_W(2, 1.0, 1.0, 0.0, 0.0, 0))
+    for Dr in (8, 16, 32, 48):
+        Le = rng.uniform(-1, 1, Dr * Dw * Dr)
+        Re = rng.uniform(-1, 1, Dw * Dr * Dr)
+
+        def site():
+            H = O.heff_dense(W2, Le, Re, 2, Dr, Dr)
+            H = 0.5 * (H + H.T)  # random environments: symmetrised for the symmetric solver
+            sla.eigh(H, subset_by_index=[0, 1], driver="evr", overwrite_a=True, check_finite=False)
+        rf[str(Dr)] = _median_s(site)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} FP64 Heff applies at D={D}, d={d}, Dw=5 (oracle GEMM-ordered, OpenMP), "
-                      f"{dt:.2f} s each"}
+    return {"value": flops / t_apply / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+            "sample": f"median of 3 FP64 Heff applies at D={D}, d={d}, Dw=5 (oracle GEMM-ordered, OpenMP), "
+                      f"{t_apply:.3f} s each; environment update {t_env:.3f} s",
+            "mf_cpu": {"heff_apply_s": t_apply, "env_update_s": t_env, "matvecs_per_sweep": matvecs_per_sweep,
+                       "sweep_estimate_s": matvecs_per_sweep * t_apply + 2 * L * t_env,
+                       "note": "matvecs of the FP64 GPU sweep x CPU apply + 2L x CPU environment update; "
+                               "gauge moves not counted (lower bound)"},
+            "rf_cpu": {"site_solve_s": rf,
+                       "config1_sweep_estimate_s": 2 * 10 * rf["8"],
+                       "note": "per bulk-site update of the S=1/2 chain: dense GetHeff (oracle) + lowest two "
+                               "eigenpairs (LAPACK dsyevr, the real counterpart of zheevx IL=1 IU=2); the config-1 "
+                               "estimate is 2L x the D=8 bulk site at L=10"}}
 
 
 def run_batch(a, tn, ctx):
@@ -379,6 +436,12 @@ def main():
     else:
         o_mixed = tn.SweepOpts.default(precision=prec, env_precision=prec, refine=0, **common)
     o_fp64 = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, refine=0, **common)
+    # the headline scheme with the Ritz vector of every local solve refined by
+    # FP64 Lanczos iterations (FP64 contractions) to eig_tol
+    o_ref64 = tn.SweepOpts.from_buffer_copy(o_mixed)
+    o_ref64.refine = 1
+    o_ref64.refine_precision = tn.TN_FP64
+    o_ref64.refine_tol = a.eig_tol
     o_bf16 = tn.SweepOpts.default(precision=tn.TN_BF16, env_precision=tn.TN_FP32, refine=1,
                                   refine_precision=tn.TN_FP32, refine_tol=a.eig_tol, **common)
     ch.load_caches(o_mixed)
@@ -418,7 +481,8 @@ def main():
     E_mixed = ch.expectation()
     print(f"# timed {a.steps} sweep(s) {wall:.2f}s E/L={E_mixed / a.L:.12f} matvecs={st.n_matvec} "
           f"(fp64 {st.n_matvec_fp64}) full-basis passes={st.n_full_reorth} qr={st.n_qr} "
-          f"shifted={st.n_qr_shifted} fallback={st.n_qr_fallback}", file=sys.stderr, flush=True)
+          f"shifted={st.n_qr_shifted} fallback={st.n_qr_fallback} local solves converged "
+          f"{st.n_solve_converged} capped {st.n_solve_capped}", file=sys.stderr, flush=True)
 
     def timed_from_saved(opts, tune=True):
         """K sweeps from the saved state (after one untimed sweep that tunes
@@ -449,6 +513,15 @@ def main():
           file=sys.stderr, flush=True)
     res_fp64 = None
     res_bf16 = None
+    res_ref64 = None
+    if a.precision != "fp64" and not a.no_refine_compare:
+        str_, wallr, Er = timed_from_saved(o_ref64)
+        res_ref64 = {"scheme": f"{a.precision} contractions, every local solve refined by FP64 Lanczos to eig_tol",
+                     "sweep_s": wallr / a.steps, "E_per_site": Er / a.L, "n_matvec": str_.n_matvec,
+                     "n_matvec_fp64": str_.n_matvec_fp64, "n_solve_converged": str_.n_solve_converged,
+                     "n_solve_capped": str_.n_solve_capped}
+        print(f"# {a.precision} + fp64 refine {wallr:.2f}s E/L={Er / a.L:.12f} matvecs={str_.n_matvec} "
+              f"(fp64 {str_.n_matvec_fp64})", file=sys.stderr, flush=True)
     if a.precision != "mixed" and not a.no_bf16_compare:
         stb, wallb, Eb = timed_from_saved(o_bf16)
         res_bf16 = {"scheme": "BF16 contractions, FP32 environments, FP32-contraction refinement",
@@ -464,6 +537,7 @@ def main():
                     "dE_per_site": abs(E_mixed - E64) / a.L,
                     "gauge_ms": st64.svd_ms, "env_ms": st64.env_ms, "lanczos_ms": st64.lanczos_ms,
                     "heff_ms": st64.heff_ms, "n_qr_fallback": st64.n_qr_fallback,
+                    "n_solve_converged": st64.n_solve_converged, "n_solve_capped": st64.n_solve_capped,
                     "ms_per_matvec": st64.heff_ms / max(st64.n_matvec, 1)}
         print(f"# fp64 {wall64:.2f}s E/L={E64 / a.L:.12f} |dE|/L={abs(E_mixed - E64) / a.L:.3e}",
               file=sys.stderr, flush=True)
@@ -492,7 +566,10 @@ def main():
                 and tr.get("tile") == stage1_tile):
             traffic = {"bytes": tr["bytes_per_launch"], "source": "profiles/heff_stage1_traffic.json"}
     value = sti.heff_flops / max(sti.heff_ms, 1e-12) / 1e9
-    achieved = f1 / (s1 * 1e-3) / 1e12
+    # the split-bfloat16 scheme's arithmetic is its six bfloat16 products per
+    # FP32-equivalent product: those are what the matrix cores must execute
+    nprod = 6 if a.precision == "bf16x3" else 1
+    achieved = nprod * f1 / (s1 * 1e-3) / 1e12
     hp = "bf16" if a.precision == "mixed" else a.precision
     B1 = {"bf16": 2, "bf16x3": 6, "fp32": 4, "fp64": 8}[hp]
     out = {
@@ -527,21 +604,27 @@ def main():
         "n_matvec": st.n_matvec, "n_matvec_fp64": st.n_matvec_fp64, "n_full_reorth": st.n_full_reorth,
         "heff_gemm_tflops_launched": sti.heff_gemm_flops / max(sti.heff_gemm_ms, 1e-12) / 1e9,
         "n_ident_left": st.n_ident_left, "n_ident_right": st.n_ident_right,
+        "n_solve_converged": st.n_solve_converged, "n_solve_capped": st.n_solve_capped,
         "ms_per_matvec": sti.heff_ms / max(sti.n_matvec, 1),
         "fp64": res_fp64,
         "bf16_then_fp32": res_bf16,
+        "fp64_refined": res_ref64,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[hp], "unit": "TFLOP/s",
                      "frac": achieved / PEAKS[hp], "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None, "stage1_tile": stage1_tile,
                      "kernel": "Heff stage-1 GEMM (psi x R, NT) at the bulk site",
-                     "per_launch_flops": f1, "avg_launch_ms": s1,
+                     "per_launch_flops": nprod * f1, "per_launch_flops_fp32_equivalent": f1,
+                     "products_per_fp32_product": nprod, "avg_launch_ms": s1,
                      "stage3_ms": s3, "stage3_flops": f3, "mpo_stage_ms": mix,
                      "algorithmic_bytes_B1": B1 * (2 * 5 * a.D ** 2 + 2 * d * a.D ** 2) + 8 * d * d * 25},
         "setup_s": setup_s,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.D, d)
+        mv = (res_fp64["n_matvec"] if res_fp64 else st.n_matvec) / a.steps
+        out["cpu_baseline"] = cpu_baseline(a.D, d, a.L, mv, a.Dz)
+        out["cpu_baseline"]["gpu_sweep_speedup_vs_mf_cpu_estimate"] = (
+            out["cpu_baseline"]["mf_cpu"]["sweep_estimate_s"] / (wall / a.steps))
     line = json.dumps(out)
     print(line, flush=True)
     if a.out:

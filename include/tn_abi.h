@@ -237,6 +237,9 @@ typedef struct tn_sweep_stats {
     int64_t n_precond_fallback;  /* learned-correction Davidson solves finished by Lanczos */
     int64_t n_full_reorth;     /* Lanczos iterations with a Gram-Schmidt pass against the whole
                                   Krylov basis (every iteration with reorth = 0) */
+    int64_t n_solve_converged; /* local eigensolves that met eig_tol (refine_tol when refined) */
+    int64_t n_solve_capped;    /* local eigensolves stopped by the iteration budget
+                                  max_krylov * (1 + max_restarts) above the tolerance */
 } tn_sweep_stats;
 
 void tn_sweep_opts_default(tn_sweep_opts* o);

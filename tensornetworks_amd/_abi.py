@@ -79,6 +79,8 @@ class SweepStats(ctypes.Structure):
         ("n_ident_right", ctypes.c_int64),
         ("n_precond_fallback", ctypes.c_int64),
         ("n_full_reorth", ctypes.c_int64),
+        ("n_solve_converged", ctypes.c_int64),
+        ("n_solve_capped", ctypes.c_int64),
     ]
 
     def as_dict(self):

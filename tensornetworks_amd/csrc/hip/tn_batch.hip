@@ -645,6 +645,10 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
         // chains still above the tolerance restart from their Ritz vector
         todo = act;
     }
+    if (st) {
+        st->n_solve_capped += (int64_t)todo.size();
+        st->n_solve_converged += (int64_t)(ns - (int)todo.size());
+    }
 }
 
 // One CholeskyQR pass X_s = Q_s L_s^T on every slot (X: m x n row-major, ld

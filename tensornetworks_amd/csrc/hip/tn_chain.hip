@@ -296,6 +296,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
                                      o.precond_delta);
         *e0 = dr.e0;
         *e1 = dr.e1;
+        bool conv = dr.converged;
         if (learned && !dr.converged) {
             if (st) st->n_precond_fallback++;
             LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, 0.0,
@@ -303,7 +304,9 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
             if (st) st->n_full_reorth += r.n_full;
             *e0 = r.e0;
             *e1 = r.e1;
+            conv = r.converged;
         }
+        if (st) (conv ? st->n_solve_converged : st->n_solve_capped)++;
         heff_timing_flush(c);
         if (st) st->lanczos_ms += now_ms() - t0;
         return;
@@ -322,7 +325,9 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
         if (st) st->n_full_reorth += r2.n_full;
         r.e0 = r2.e0;
         r.e1 = r2.e1;
+        r.converged = r2.converged;
     }
+    if (st) (r.converged ? st->n_solve_converged : st->n_solve_capped)++;
     heff_timing_flush(c);
     if (st) st->lanczos_ms += now_ms() - t0;
     *e0 = r.e0;
@@ -736,6 +741,7 @@ int tn_ctx_tune_load(tn_ctx* c, const char* path)
         const bool shape_ok = v[4] > 0 && v[5] > 0 && v[6] > 0 && v[7] > 0;
         if (!types_ok || !flags_ok || !shape_ok) continue;
         if (v[9] < 1 || v[9] > gemm_num_tiles() || (v[10] != 1 && v[10] != 2 && v[10] != 4)) continue;
+        if (!gemm_tile_ok((int)v[9], (int)v[0], (int)v[4], (int)v[5], (int)v[6])) continue;
         const std::array<long, 9> key{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
         c->tune[key] = {(int)v[9], (int)v[10]};
         n++;
@@ -1928,9 +1934,11 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
             if (st) st->n_full_reorth += r2.n_full;
             r.e0 = r2.e0;
             r.e1 = r2.e1;
+            r.converged = r2.converged;
         }
         heff_timing_flush(c);
         if (st) {
+            (r.converged ? st->n_solve_converged : st->n_solve_capped)++;
             st->lanczos_ms += now_ms() - tl;
             st->energy = r.e0;
         }

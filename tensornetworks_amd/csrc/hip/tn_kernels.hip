@@ -472,13 +472,17 @@ __global__ void __launch_bounds__(256) gemm_grouped_k(const GemmProb* __restrict
 }
 
 // tile menu (code -> BM x BN; "deep" tiles stage twice the K slice per LDS
-// round, halving the barriers per flop at twice the LDS footprint)
-struct TileCfg { int bm, bn; bool deep; const char* name; };
+// round, halving the barriers per flop at twice the LDS footprint).  The
+// 96-wide tiles exist for the split-bfloat16 product only: at two
+// workgroups per CU, 3072 x 4096 in 96 x 128 tiles (and 1024 x 3072 in
+// 128 x 96 tiles, split-K 2) fills the 256 CUs in whole rounds, where
+// 128 x 128 tiles leave the last round half empty.
+struct TileCfg { int bm, bn; bool deep; const char* name; bool x3only; };
 static const TileCfg kTiles[] = {
-    {128, 128, false, "128x128"}, {128, 64, false, "128x64"}, {64, 128, false, "64x128"},
-    {64, 64, false, "64x64"},     {32, 64, false, "32x64"},   {64, 32, false, "64x32"},
-    {32, 32, false, "32x32"},     {128, 128, true, "128x128d"}, {128, 64, true, "128x64d"},
-    {64, 128, true, "64x128d"},
+    {128, 128, false, "128x128", false}, {128, 64, false, "128x64", false}, {64, 128, false, "64x128", false},
+    {64, 64, false, "64x64", false},     {32, 64, false, "32x64", false},   {64, 32, false, "64x32", false},
+    {32, 32, false, "32x32", false},     {128, 128, true, "128x128d", false}, {128, 64, true, "128x64d", false},
+    {64, 128, true, "64x128d", false},   {96, 128, false, "96x128", true},  {128, 96, false, "128x96", true},
 };
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int gemm_num_tiles() { return kNumTiles; }
@@ -566,7 +570,9 @@ static hipError_t launch_x3(int code, const GemmX3Params& p, int batch, hipStrea
     // deep tiles: a K step of 64 per LDS round (half the barriers per MFMA)
     case 7: gemm_x3_k<TO, 128, 128, 64><<<grid, 256, 0, s>>>(p); break;
     case 8: gemm_x3_k<TO, 128, 64, 64><<<grid, 256, 0, s>>>(p); break;
-    default: gemm_x3_k<TO, 64, 128, 64><<<grid, 256, 0, s>>>(p); break;
+    case 9: gemm_x3_k<TO, 64, 128, 64><<<grid, 256, 0, s>>>(p); break;
+    case 10: gemm_x3_k<TO, 96, 128, 32><<<grid, 256, 0, s>>>(p); break;
+    default: gemm_x3_k<TO, 128, 96, 32><<<grid, 256, 0, s>>>(p); break;
     }
     return hipGetLastError();
 }
@@ -595,7 +601,7 @@ bool gemm_tile_ok(int code, int tin, int M, int N, int K)
 {
     if (code < 1 || code > kNumTiles) return false;
     const TileCfg& t = kTiles[code - 1];
-    if (M % t.bm || N % t.bn) return false;
+    if (M % t.bm || N % t.bn || (t.x3only && tin != T_BF16X3)) return false;
     if (tin == T_BF16X3) return K % (t.deep ? 64 : 32) == 0;
     const int bk = gemm_k_step(tin, K);
     return K % (t.deep ? 2 * bk : bk) == 0;
@@ -614,7 +620,7 @@ hipError_t gemm_grouped(int tin, int tout, bool akm, bool bkm, int tile, int bk,
                         const int* off, int nprob, int total_tiles, hipStream_t s)
 {
     if (nprob <= 0 || total_tiles <= 0) return hipSuccess;
-    if (tile < 1 || tile > kNumTiles) return hipErrorInvalidValue;
+    if (tile < 1 || tile > kNumTiles || kTiles[tile - 1].x3only) return hipErrorInvalidValue;
     const int grid = total_tiles < 2048 ? total_tiles : 2048;
     if (akm && bkm) return grouped_layout<true, true>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
     if (akm && !bkm) return grouped_layout<true, false>(tin, tout, tile - 1, bk, probs, off, nprob, grid, s);
@@ -629,7 +635,7 @@ static int choose_tile(int M, int N, int batch)
     int best = -1;
     long best_blocks = -1;
     for (int c = 0; c < kNumTiles; c++) {
-        if (kTiles[c].deep || M % kTiles[c].bm || N % kTiles[c].bn) continue;
+        if (kTiles[c].deep || kTiles[c].x3only || M % kTiles[c].bm || N % kTiles[c].bn) continue;
         long blocks = (long)(M / kTiles[c].bm) * (N / kTiles[c].bn) * batch;
         if (blocks >= 384) return c;
         if (blocks > best_blocks) { best_blocks = blocks; best = c; }
@@ -647,7 +653,7 @@ static hipError_t gemm_typed(const GemmDesc& g, hipStream_t s)
     p.sA = g.sA; p.sB = g.sB; p.sC = g.sC;
     p.alpha = g.alpha; p.accumulate = g.accumulate;
     int code = g.tile > 0 ? g.tile - 1 : choose_tile(g.M, g.N, g.batch);
-    if (code < 0 || code >= kNumTiles || g.M % kTiles[code].bm || g.N % kTiles[code].bn)
+    if (code < 0 || code >= kNumTiles || kTiles[code].x3only || g.M % kTiles[code].bm || g.N % kTiles[code].bn)
         return hipErrorInvalidValue;
     if (kTiles[code].deep && g.K % (2 * gemm_k_step(g.tin, g.K))) return hipErrorInvalidValue;
     if constexpr (std::is_same<TI, double>::value) {

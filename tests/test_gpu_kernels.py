@@ -311,6 +311,27 @@ def test_split_bf16_gemm_reaches_fp32_accuracy(ctx, tin, tol, tile):
         assert np.abs(Cb - ref).max() / np.abs(ref).max() > 30 * err
 
 
+@pytest.mark.parametrize("M,N,K,tile,ks", [
+    (384, 384, 256, 11, 1),    # 96 x 128 tiles, 4 x 3 blocks
+    (3072, 512, 128, 11, 1),   # 32 row blocks of 96: four full groups of 8
+    (256, 288, 512, 12, 1),    # 128 x 96 tiles
+    (1024, 384, 1024, 12, 2),  # 128 x 96 tiles with split-K into FP64
+])
+def test_split_bf16_96_wide_tiles(ctx, M, N, K, tile, ks):
+    # the 96-row / 96-column tiles of the split-bfloat16 product (three row
+    # or column fragments per wave); the other element types refuse them
+    A = rand((1, M, K), M + 7)
+    B = rand((1, K, N), N + 8)
+    ref = A @ B
+    Bt = np.ascontiguousarray(B.transpose(0, 2, 1))
+    C = ctx.gemm_test(A, Bt, 3, 0 if ks > 1 else 1, True, True, M, N, K, 1, tile, ks)
+    err = np.abs(C - ref).max() / np.abs(ref).max()
+    assert err < 3e-7 * np.sqrt(K), (M, N, K, tile, err)
+    for tin in (1, 2):
+        with pytest.raises(tn.TnError):
+            ctx.gemm_test(A, Bt, tin, 1, True, True, M, N, K, 1, tile)
+
+
 @pytest.mark.parametrize("tin", [1, 2, 3])
 @pytest.mark.parametrize("M,N,K,tile", [
     (3072, 256, 256, 1),   # 24 row blocks of 128: three full groups of 8 (the bulk stage-1 row count)
