@@ -1728,9 +1728,10 @@ hipError_t scale_rows(double* A, long lda, int r, int c, const double* sv, hipSt
 
 // ---------------------------------------------------------------------------
 // CholeskyQR pieces.  chol_diag_k factors one CB x CB diagonal block of a
-// symmetric positive definite G (one wavefront) and writes L_kk (lower) and
-// its inverse; a non-positive or non-finite pivot sets flag[0] = 1.
+// symmetric positive definite G and writes L_kk (lower) and its inverse; a
+// non-positive or non-finite pivot sets flag[0] = 1.
 #define CB 64
+#define CW 16  // columns of the trailing matrix (rows of the inverse) per wave
 // v from lane `lane` (wave-uniform index) of the wavefront, as a register
 // broadcast (two 32-bit readlanes)
 __device__ __forceinline__ double wave_bcast(double v, int lane)
@@ -1741,98 +1742,113 @@ __device__ __forceinline__ double wave_bcast(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// One wavefront, the block in registers.  Lane r keeps the not yet factored
-// part of row r, a[kk] = A[r][j + kk], shifted down by one column per step so
-// that every register index is a compile-time constant while the step loop
-// stays rolled.  Right-looking step j: the pivot A[j][j] comes from lane j,
-// lane r > j forms L[r][j] = A[r][j] / L[j][j], and the trailing update
-// A[r][k] -= L[r][j] L[k][j] takes L[k][j] from lane k by readlane (register
-// broadcast: no LDS round trip, no barrier).  The same broadcast drives the
-// forward substitution for L^-1, fused into the step: lane c keeps the
-// pending right-hand side of column c, sv[kk] = row j + kk, emits X[j][c]
-// and subtracts L[j+kk][j] X[j][c].  Lane indices wrap to lanes < j, whose
-// column entry is zero, so the full-width updates need no guard.
+// Four wavefronts, lane r = row r.  Wave w keeps columns 16w .. 16w+15 of the
+// not yet factored matrix, a[kk] = A[r][16w + kk], and rows 16w .. 16w+15 of
+// the right-hand side of L X = I, b[kk] = B[16w + kk][r] (lane r = column r
+// of X).  Right-looking step j, owned by wave j / 16 (register index j % 16 a
+// compile-time constant: the 16 steps of a column block are unrolled):
+//   owner: pivot A[j][j] from lane j, L[r][j] = A[r][j] / L[j][j] (the whole
+//          column, zero above the diagonal) and X[j][r] = B[j][r] / L[j][j]
+//          into LDS; X[j][:] is row j of L^-1;
+//   all:   A[r][k] -= L[r][j] L[k][j] on their columns k and
+//          B[i][r] -= L[i][j] X[j][r] on their rows i, L[k][j] read from LDS
+//          as a broadcast.
+// Entries above the diagonal of column j of L are zero, so the full-width
+// updates need no guard; columns and rows already consumed take values that
+// are never read again.  The column and row buffers alternate between two
+// LDS slots by step parity, so one barrier per step orders them: a wave can
+// only overwrite a slot after every wave has passed the next barrier, that
+// is, after every wave has read it.
 __device__ __forceinline__ void chol_block(const double* __restrict__ G, long ldg, int k0, int nbk,
                                            double* __restrict__ L, long ldl, double* __restrict__ Linv, long ldi,
-                                           int* __restrict__ flag, double (*Ls)[CB + 1])
+                                           int* __restrict__ flag, double (*Ls)[CB + 1], double (*col)[CB],
+                                           double (*xrow)[CB])
 {
-    const int r = threadIdx.x;
-    // coalesced load (lane = column), 16 loads in flight per lane; a block
+    const int tid = threadIdx.x, r = tid & 63, wid = tid >> 6;
+    // coalesced load (lane = column, one row per wave per round); a block
     // narrower than CB (the last one) is completed with the identity
 #pragma unroll
-    for (int i0 = 0; i0 < CB; i0 += 16) {
-        double tmp[16];
+    for (int i0 = 0; i0 < CB; i0 += 4 * 4) {
+        double tmp[4];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int i = i0 + u;
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * 4 + wid;
             tmp[u] = (i < nbk && r < nbk) ? G[(long)(k0 + i) * ldg + k0 + r] : (i == r ? 1.0 : 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 16; u++) Ls[i0 + u][r] = tmp[u];
+        for (int u = 0; u < 4; u++) Ls[i0 + u * 4 + wid][r] = tmp[u];
     }
     __syncthreads();
-    double a[CB], sv[CB];
+    double a[CW], bv[CW];
 #pragma unroll
-    for (int k = 0; k < CB; k++) {
-        a[k] = Ls[r][k];
-        sv[k] = k == r ? 1.0 : 0.0;
+    for (int kk = 0; kk < CW; kk++) {
+        a[kk] = Ls[r][wid * CW + kk];
+        bv[kk] = (wid * CW + kk == r) ? 1.0 : 0.0;
     }
     __syncthreads();  // every row is in registers before Ls takes L
     int bad = 0;
-    for (int j = 0; j < CB; j++) {
-        const double piv = wave_bcast(a[0], j);
-        const bool ok = piv > 0.0 && isfinite(piv);
-        bad |= ok ? 0 : 1;
-        const double d = ok ? sqrt(piv) : 1.0;
-        const double inv = 1.0 / d;
-        const double cj = r > j ? a[0] * inv : (r == j ? d : 0.0);
-        const double xj = sv[0] * inv;
-        Ls[r][j] = cj;
-        if (j < nbk && r < nbk) Linv[(long)(k0 + j) * ldi + k0 + r] = xj;
-        // chunks wholly past the last row (kk >= CB - j) are skipped: their
-        // entries are never read again, so they may keep stale values
+    for (int jb = 0; jb < CB / CW; jb++) {
 #pragma unroll
-        for (int k0c = 1; k0c < CB; k0c += 9) {
-            if (k0c < CB - j) {
-#pragma unroll
-                for (int u = 0; u < 9; u++) {
-                    const int kk = k0c + u;
-                    const double lk = wave_bcast(cj, (j + kk) & (CB - 1));  // L[j+kk][j]
-                    a[kk - 1] = fma(-cj, lk, a[kk]);
-                    sv[kk - 1] = fma(-lk, xj, sv[kk]);
-                }
+        for (int jj = 0; jj < CW; jj++) {
+            const int j = jb * CW + jj, slot = jj & 1;
+            if (wid == jb) {
+                const double piv = wave_bcast(a[jj], j);
+                const bool ok = piv > 0.0 && isfinite(piv);
+                bad |= ok ? 0 : 1;
+                const double d = ok ? sqrt(piv) : 1.0;
+                const double inv = 1.0 / d;
+                const double cj = r > j ? a[jj] * inv : (r == j ? d : 0.0);
+                const double xj = bv[jj] * inv;
+                col[slot][r] = cj;
+                xrow[slot][r] = xj;
+                Ls[r][j] = cj;
+                if (j < nbk && r < nbk) Linv[(long)(k0 + j) * ldi + k0 + r] = xj;
             }
-            __builtin_amdgcn_sched_barrier(0);  // bounds the broadcasts in flight (SGPRs)
+            __syncthreads();
+            const double lr = col[slot][r];   // L[r][j]
+            const double xr = xrow[slot][r];  // X[j][r]
+#pragma unroll
+            for (int kk = 0; kk < CW; kk += 2) {
+                const double2 lk = *(const double2*)&col[slot][wid * CW + kk];  // L[k][j], L[k+1][j]
+                a[kk] = fma(-lr, lk.x, a[kk]);
+                a[kk + 1] = fma(-lr, lk.y, a[kk + 1]);
+                bv[kk] = fma(-lk.x, xr, bv[kk]);
+                bv[kk + 1] = fma(-lk.y, xr, bv[kk + 1]);
+            }
         }
     }
     __syncthreads();
     // L (lower; Ls holds zeros above the diagonal), coalesced rows
-    for (int i = 0; i < nbk; i++)
+    for (int i = wid; i < nbk; i += 4)
         if (r < nbk) L[(long)(k0 + i) * ldl + k0 + r] = Ls[i][r];
     if (r == 0 && bad) flag[0] = 1;
 }
 
-__global__ void __launch_bounds__(64) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
-                                                  double* __restrict__ L, long ldl,
-                                                  double* __restrict__ Linv, long ldi,
-                                                  int* __restrict__ flag)
+__global__ void __launch_bounds__(256) chol_diag_k(const double* __restrict__ G, long ldg, int k0, int nbk,
+                                                   double* __restrict__ L, long ldl,
+                                                   double* __restrict__ Linv, long ldi,
+                                                   int* __restrict__ flag)
 {
     __shared__ double Ls[CB][CB + 1];
-    chol_block(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag, Ls);
+    __shared__ __attribute__((aligned(16))) double col[2][CB];
+    __shared__ double xrow[2][CB];
+    chol_block(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag, Ls, col, xrow);
 }
 
 // one diagonal block per workgroup, one job per chain
-__global__ void __launch_bounds__(64) chol_diag_batch_k(const CholJob* __restrict__ jobs)
+__global__ void __launch_bounds__(256) chol_diag_batch_k(const CholJob* __restrict__ jobs)
 {
     __shared__ double Ls[CB][CB + 1];
+    __shared__ __attribute__((aligned(16))) double col[2][CB];
+    __shared__ double xrow[2][CB];
     const CholJob& j = jobs[blockIdx.x];
-    chol_block(j.G, j.ldg, j.k0, j.nbk, j.L, j.ldl, j.Linv, j.ldi, j.flag, Ls);
+    chol_block(j.G, j.ldg, j.k0, j.nbk, j.L, j.ldl, j.Linv, j.ldi, j.flag, Ls, col, xrow);
 }
 
 hipError_t chol_diag_batch(const CholJob* jobs, int n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    chol_diag_batch_k<<<n, 64, 0, s>>>(jobs);
+    chol_diag_batch_k<<<n, 256, 0, s>>>(jobs);
     return hipGetLastError();
 }
 
@@ -1840,7 +1856,7 @@ hipError_t chol_diag(const double* G, long ldg, int k0, int nbk, double* L, long
                      int* flag, hipStream_t s)
 {
     if (nbk <= 0 || nbk > CB) return hipErrorInvalidValue;
-    chol_diag_k<<<1, 64, 0, s>>>(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag);
+    chol_diag_k<<<1, 256, 0, s>>>(G, ldg, k0, nbk, L, ldl, Linv, ldi, flag);
     return hipGetLastError();
 }
 
