@@ -299,6 +299,32 @@ __global__ void __launch_bounds__(256) gemm_k(GemmParams p)
                                             p.accumulate, tm * BM, tn * BN, As, Bs);
 }
 
+// Split-product LDS layout for BK = 32: rows of 64 bytes without padding,
+// the four 16-byte k-units of row r stored in the order u ^ g(r), g(r) =
+// ((r >> 1) ^ (r >> 2)) & 3.  A tile store (lanes 4i .. 4i+3 write the four
+// units of one row) and a fragment read (lanes 0..15 read one unit of rows
+// 0..15) then fall on distinct banks in every group of 8 or 16 lanes, with
+// 32 or with 64 banks of 4 bytes; the padded layout (80-byte rows) leaves the
+// stores 2-way conflicted.
+__device__ __forceinline__ int x3_swz(int row, int unit) { return row * 32 + ((unit ^ ((row >> 1) ^ (row >> 2))) & 3) * 8; }
+
+template <int ROWS, int BK, int R>
+__device__ __forceinline__ void x3_sstore(const u32x4 (&rg)[R], bf16* __restrict__ S, int tid)
+{
+    constexpr int UPR = BK / 8;  // 16-byte units per row
+    constexpr int TOT = ROWS * UPR;
+    constexpr int LDK = BK == 32 ? BK : BK + MT<bf16>::PAD;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int v = tid + r * 256;
+        if (TOT % 256 == 0 || v < TOT) {
+            const int row = v / UPR, kv = v % UPR;
+            if constexpr (BK == 32) *(u32x4*)(S + x3_swz(row, kv)) = rg[r];
+            else *(u32x4*)(S + row * LDK + kv * 8) = rg[r];
+        }
+    }
+}
+
 // One BM x BN output tile of the split-bfloat16 product: A and B each held as
 // three bfloat16 planes (A = A0 + A1 + A2), K-major, every plane staged
 // through LDS; each K step issues the six leading cross products A_i B_j
@@ -313,7 +339,8 @@ __device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long ps
     typedef MT<bf16> T;
     typedef T::frag_t frag_t;
     typedef T::acc_t acc_t;
-    constexpr int LDK = BK + T::PAD;
+    constexpr bool SWZ = BK == 32;
+    constexpr int LDK = SWZ ? BK : BK + T::PAD;
     constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int TA = BM * BK / T::VEC, TB = BN * BK / T::VEC;
     constexpr int RA = (TA + 255) / 256, RB = (TB + 255) / 256;
@@ -322,6 +349,11 @@ __device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long ps
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
+    // LDS element offset of the fragment of row `row` at k-step kk
+    auto foff = [&](int row, int kk) {
+        if constexpr (SWZ) return x3_swz(row, (lane >> 4) + kk / 8);
+        else return row * LDK + kk + (lane >> 4) * T::KPL;
+    };
 
     u32x4 ra[3][RA], rb[3][RB];
     acc_t acc[FM][FN];
@@ -340,8 +372,8 @@ __device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long ps
     }
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-        tile_sstore<bf16, BM, BK, true>(ra[p], As + p * SA, tid);
-        tile_sstore<bf16, BN, BK, true>(rb[p], Bs + p * SB, tid);
+        x3_sstore<BM, BK>(ra[p], As + p * SA, tid);
+        x3_sstore<BN, BK>(rb[p], Bs + p * SB, tid);
     }
     __syncthreads();
     for (int t = 0; t < nk; t++) {
@@ -360,12 +392,10 @@ __device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long ps
             for (int p = 0; p < 3; p++) {
 #pragma unroll
                 for (int i = 0; i < FM; i++)
-                    a[p][i] = *(const frag_t*)(As + p * SA + (wm0 + i * 16 + (lane & 15)) * LDK + kk +
-                                               (lane >> 4) * T::KPL);
+                    a[p][i] = *(const frag_t*)(As + p * SA + foff(wm0 + i * 16 + (lane & 15), kk));
 #pragma unroll
                 for (int j = 0; j < FN; j++)
-                    b[p][j] = *(const frag_t*)(Bs + p * SB + (wn0 + j * 16 + (lane & 15)) * LDK + kk +
-                                               (lane >> 4) * T::KPL);
+                    b[p][j] = *(const frag_t*)(Bs + p * SB + foff(wn0 + j * 16 + (lane & 15), kk));
             }
 #pragma unroll
             for (int i = 0; i < FM; i++)
@@ -385,8 +415,8 @@ __device__ __forceinline__ void gemm_tile_x3(const bf16* __restrict__ A, long ps
         if (t + 1 < nk) {
 #pragma unroll
             for (int p = 0; p < 3; p++) {
-                tile_sstore<bf16, BM, BK, true>(ra[p], As + p * SA, tid);
-                tile_sstore<bf16, BN, BK, true>(rb[p], Bs + p * SB, tid);
+                x3_sstore<BM, BK>(ra[p], As + p * SA, tid);
+                x3_sstore<BN, BK>(rb[p], Bs + p * SB, tid);
             }
             __syncthreads();
         }
@@ -425,7 +455,7 @@ struct GemmX3Params {
 template <typename TO, int BM, int BN, int BK>
 __global__ void __launch_bounds__(256) gemm_x3_k(GemmX3Params p)
 {
-    constexpr int LDK = BK + MT<bf16>::PAD;
+    constexpr int LDK = BK == 32 ? BK : BK + MT<bf16>::PAD;
     __shared__ __attribute__((aligned(16))) bf16 As[3 * BM * LDK];
     __shared__ __attribute__((aligned(16))) bf16 Bs[3 * BN * LDK];
     const int ntn = p.N / BN, ntm = p.M / BM;
