@@ -28,3 +28,14 @@ def ctx():
     c = Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def restore_mlp(ctx):
+    """For tests that load their own preconditioner network into the session
+    context: the default network (init_params(0), unit normalisation) is put
+    back afterwards, also when the test fails."""
+    yield ctx
+    import numpy as np
+    from tensornetworks_amd import precond
+    ctx.set_mlp(precond.init_params(0), np.array([0, 0, 0, 0, 1, 1, 1, 1], dtype=np.float32))

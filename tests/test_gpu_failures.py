@@ -78,7 +78,7 @@ def test_unconverged_local_solves_stay_variational(ctx):
 
 
 @pytest.mark.parametrize("kind", [2, 3])
-def test_non_finite_network_gives_the_diagonal_correction(ctx, kind):
+def test_non_finite_network_gives_the_diagonal_correction(ctx, restore_mlp, kind):
     # a network whose output is NaN: the factor clamp turns it into a finite
     # constant, i.e. the diagonal correction up to scale, and the solves
     # converge to the reference energy
@@ -90,4 +90,3 @@ def test_non_finite_network_gives_the_diagonal_correction(ctx, kind):
         ch.sweep(0, o, st)
         ch.sweep(1, o, st)
     assert abs(ch.expectation() / 9 - E_REF) < 1e-11
-    ctx.set_mlp(precond.init_params(0), np.array([0, 0, 0, 0, 1, 1, 1, 1], dtype=np.float32))

@@ -73,7 +73,7 @@ def test_davidson_reference_energy(ctx, kind):
     assert abs(ch.expectation() / 9 - (-0.473113399629092)) < 1e-11
 
 
-def test_identity_network_equals_diagonal_davidson(ctx):
+def test_identity_network_equals_diagonal_davidson(ctx, restore_mlp):
     # w3 = 0, b3 = 1 makes the learned correction the diagonal one exactly;
     # one basis large enough that no solve restarts or falls back
     p = precond.init_params(5)
@@ -97,21 +97,53 @@ def test_identity_network_equals_diagonal_davidson(ctx):
 
 
 def test_precond_samples_are_exact_corrections(ctx):
-    # with t the exact correction, u + t is the exact local ground state up to
-    # normalisation: the samples' a/t relation is consistent with g ~ 1 early on
-    ch = tn.Chain(ctx, xxz_mpo(10, 0.5, Jz=1.3), 16)
+    # Davidson's first iterate is u = x/|x| (x the site tensor), theta = u.Hu,
+    # r = Hu - theta u.  Every sample of that iterate must be one component of
+    # a = -r/(diag - theta)/|r| (|diag - theta| clamped to delta) and of the
+    # exact correction t = (u0/(u.u0) - u)/|r|, u0 the local ground state, so
+    # that u + |r| t is u0 up to normalisation.  Checked against the same
+    # site solved on the host: environments zipped by the oracle from the
+    # chain's own tensors, dense Heff (MPSSite_Contractions.C:155-193), numpy
+    # eigh for u0
+    L, d, s, delta = 8, 2, 3, 1e-2
+    Ws = xxz_mpo(L, 0.5, Jz=1.3)
+    ch = tn.Chain(ctx, Ws, 6)
     ch.initialize(1, 5)
     ch.normalize_qr(1)
     o = tn.SweepOpts.default(eig_tol=1e-10, gauge=1)
     ch.load_caches(o)
-    X, a, t = ch.precond_samples(0, iters=3, nsamp=500, seed=1)
-    assert X.shape == (1500, 4) and np.all(np.isfinite(X)) and np.all(np.isfinite(a)) and np.all(np.isfinite(t))
-    # the diagonal correction points the right way on average
-    assert np.sum(a * t) > 0
+    X, a, t = ch.precond_samples(s, iters=1, nsamp=400, seed=1, delta=delta)
+    assert X.shape == (400, 4) and np.all(np.isfinite(X)) and np.all(np.isfinite(a)) and np.all(np.isfinite(t))
+    D1, D2 = ch.dims()
+    F = np.ones(1)
+    for i in range(s):
+        F = O.env_left_ref(Ws[i], F, ch.site(i), d, D1[i], D2[i])
+    G = np.ones(1)
+    for i in reversed(range(s + 1, L)):
+        G = O.env_right_ref(Ws[i], G, ch.site(i), d, D1[i], D2[i])
+    H = O.heff_dense(Ws[s], F, G, d, D1[s], D2[s])
+    assert H.shape == (72, 72) and np.abs(H - H.T).max() < 1e-12 * np.abs(H).max()
+    w, Z = np.linalg.eigh(H)
+    u0 = Z[:, 0]
+    u = ch.site(s) / np.linalg.norm(ch.site(s))
+    theta = u @ H @ u
+    r = H @ u - theta * u
+    nr = np.linalg.norm(r)
+    den = np.diag(H) - theta
+    pa = -r / np.where(np.abs(den) < delta, delta, den) / nr
+    pt = (u0 / (u @ u0) - u) / nr
+    px = np.arcsinh(den / delta)
+    sa, st = np.abs(pa).max(), np.abs(pt).max()
+    for q in range(len(a)):
+        e = np.argmin(np.abs(pa - a[q]) / sa + np.abs(pt - t[q]) / st)
+        assert abs(pa[e] - a[q]) < 1e-9 * sa and abs(pt[e] - t[q]) < 1e-8 * st, (q, e)
+        assert abs(px[e] - X[q, 0]) < 1e-8 * max(1.0, abs(px[e]))
+    # the samples reach many distinct components
+    assert len(np.unique(np.round(t / st, 9))) > 40
 
 
 @pytest.mark.parametrize("kind", [2, 3])
-def test_learned_davidson_falls_back_to_lanczos(ctx, kind):
+def test_learned_davidson_falls_back_to_lanczos(ctx, restore_mlp, kind):
     # one Davidson basis of 6 vectors cannot reach 1e-12: every unconverged
     # learned-correction solve is finished by plain Lanczos, counted in the
     # stats, and the sweeps still reach the reference energy
