@@ -1124,8 +1124,11 @@ static void qr_accumulate(tn_ctx* c, double* Lacc, const double* Lk, int n)
 
 // Second pass without a factorisation: with G = Q1^T Q1 = I + E and
 // |E|_F <= 1e-3, Q = Q1 G^{-1/2} and A = Q1 L1^T = Q (L1 G^{1/2})^T, the
-// inverse square root and the square root from their binomial series to E^3
-// (remainder below 35/128 |E|^4 <= 3e-13, so no Gram check follows).  The
+// inverse square root and the square root from their binomial series, taken
+// to the lowest order k whose remainder c_{k+1} |E|^{k+1} (c = 3/8, 5/16,
+// 35/128 for the inverse square root, the larger of the two) stays below
+// 3e-13, so no Gram check follows: order 1 for |E| <= 8.9e-7, 2 for
+// |E| <= 9.9e-5, 3 up to 1e-3.  With |E|_F <= 1e-13 Q1 is kept as it is.  The
 // factor is no longer triangular; a gauge move does not need it to be.
 // Returns false (nothing written) when |E| is too large for the series.
 static bool polar_finish(tn_ctx* c, const double* Q1, int m, int n, int nl, double* Q, double* Lacc)
@@ -1147,16 +1150,28 @@ static bool polar_finish(tn_ctx* c, const double* Q1, int m, int n, int nl, doub
     TN_HIP(mdot(E, nn, 1, E, nn, h, c->part.as<double>(), (int)pcap, s));
     double e2 = 0;
     d2h_small(c, &e2, h, 1);
-    if (!(std::sqrt(e2) <= 1e-3)) return false;
+    const double e = std::sqrt(e2);
+    if (!(e <= 1e-3)) return false;
+    if (e <= 1e-13) {
+        if (Q != Q1) TN_HIP(hipMemcpyAsync(Q, Q1, (size_t)m * n * 8, hipMemcpyDeviceToDevice, s));
+        return true;
+    }
+    const int order = e <= 8.9e-7 ? 1 : (e <= 9.9e-5 ? 2 : 3);
     GemmDesc g{};
     g.tin = T_F64; g.tout = T_F64; g.a_kmajor = true; g.b_kmajor = false;
     g.M = n; g.N = n; g.K = n; g.batch = 1; g.lda = n; g.ldb = n; g.ldc = n; g.alpha = 1.0f;
-    g.A = E; g.B = E; g.C = E2;
-    gemm_run(c, g);
-    g.A = E2; g.B = E; g.C = E3;
-    gemm_run(c, g);
-    const double ci[3] = {-0.5, 0.375, -0.3125}, cs[3] = {0.5, -0.125, 0.0625};
-    TN_HIP(series3(E, E2, E3, n, ci, cs, Xm, Ym, s));
+    if (order >= 2) {
+        g.A = E; g.B = E; g.C = E2;
+        gemm_run(c, g);
+    }
+    if (order >= 3) {
+        g.A = E2; g.B = E; g.C = E3;
+        gemm_run(c, g);
+    }
+    // unused powers are read with a zero coefficient
+    const double ci[3] = {-0.5, order >= 2 ? 0.375 : 0.0, order >= 3 ? -0.3125 : 0.0};
+    const double cs[3] = {0.5, order >= 2 ? -0.125 : 0.0, order >= 3 ? 0.0625 : 0.0};
+    TN_HIP(series3(E, order >= 2 ? E2 : E, order >= 3 ? E3 : E, n, ci, cs, Xm, Ym, s));
     GemmDesc q{};
     q.tin = T_F64; q.tout = T_F64; q.a_kmajor = true; q.b_kmajor = false;
     q.M = m; q.N = n; q.K = n; q.batch = 1;

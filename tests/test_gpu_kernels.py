@@ -275,11 +275,13 @@ def test_cholqr_rank_deficient(ctx):
     assert np.abs(Q @ R - A).max() < 1e-12
 
 
-@pytest.mark.parametrize("cond", [1.0, 1e3, 1e6, 1e12])
+@pytest.mark.parametrize("cond", [1.0, 1e2, 1e3, 1e4, 1e5, 1e6, 1e12])
 def test_cholqr_polar_finish(ctx, cond):
     # the gauge move's mode: any A = Q R with orthonormal Q (R need not be
-    # triangular); well-conditioned inputs take the polar finish, the others
-    # the further CholeskyQR passes
+    # triangular); well-conditioned inputs take the polar finish (its series
+    # order grows with the first pass's loss of orthogonality, ~u cond^2:
+    # none, first, second or third order), the others the further
+    # CholeskyQR passes
     rng = np.random.default_rng(int(np.log10(cond)) + 11)
     m, n = 1100, 700
     U, _ = np.linalg.qr(rng.standard_normal((m, n)))
