@@ -117,6 +117,8 @@ static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
     TN_HIP(hipEventCreate(&e1));
     std::pair<int, int> best{0, 1};
     float best_ms = 1e30f;
+    // TN_TUNE_VERBOSE: every candidate's time on stderr
+    static const bool verbose = getenv("TN_TUNE_VERBOSE") != nullptr;
     for (int code = 1; code <= nt; code++) {
         if (!gemm_tile_ok(code, g.tin, g.M, g.N, g.K)) continue;
         for (int ks : {1, 2, 4}) {
@@ -129,6 +131,9 @@ static std::pair<int, int> tune_gemm(tn_ctx* c, const GemmDesc& g)
             TN_HIP(hipEventSynchronize(e1));
             float ms = 0;
             TN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (verbose)
+                fprintf(stderr, "tune tin %d tout %d M %d N %d K %d batch %d: tile %s ksplit %d %.4f ms\n", g.tin,
+                        g.tout, g.M, g.N, g.K, g.batch, gemm_tile_name(code), ks, ms / 3);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = {code, ks};

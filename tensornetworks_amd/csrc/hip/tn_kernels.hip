@@ -473,6 +473,152 @@ __global__ void __launch_bounds__(256) gemm_x3_k(GemmX3Params p)
                                  p.ldc, p.K, p.alpha, p.accumulate, tm * BM, tn * BN, As, Bs);
 }
 
+// The split product with eight waves per workgroup and one workgroup per CU
+// (the whole chip in one round at the bulk shapes: 3072 x 4096 in 192 x 256
+// tiles, 1024 x 3072 in 256 x 192 tiles with a 4-way K split).  Wave grid
+// WGM x WGN, wave tile (BM / WGM) x (BN / WGN) = 96 x 64 or 64 x 96: 0.28 KB
+// of LDS traffic per MFMA against 0.44 KB for the 96 x 128 four-wave tile.
+// Same swizzled LDS layout, register prefetch and priority scheme; the
+// fragments of B are loaded column block by column block to bound registers.
+template <int ROWS, int R>
+__device__ __forceinline__ void x3w_gload(u32x4 (&rg)[R], const bf16* __restrict__ G, long ld, int r0, int k0, int tid)
+{
+    constexpr int TOT = ROWS * 4;  // 16-byte units of a 32-wide K slice
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int v = tid + r * 512;
+        if (TOT % 512 == 0 || v < TOT) rg[r] = *(const u32x4*)(G + (long)(r0 + v / 4) * ld + k0 + (v % 4) * 8);
+    }
+}
+template <int ROWS, int R>
+__device__ __forceinline__ void x3w_sstore(const u32x4 (&rg)[R], bf16* __restrict__ S, int tid)
+{
+    constexpr int TOT = ROWS * 4;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int v = tid + r * 512;
+        if (TOT % 512 == 0 || v < TOT) *(u32x4*)(S + x3_swz(v / 4, v % 4)) = rg[r];
+    }
+}
+
+template <typename TO, int BM, int BN, int WGM, int WGN>
+__global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
+{
+    typedef MT<bf16> T;
+    typedef T::frag_t frag_t;
+    typedef T::acc_t acc_t;
+    static_assert(WGM * WGN == 8, "eight waves");
+    constexpr int BK = 32;
+    constexpr int WM = BM / WGM, WN = BN / WGN, FM = WM / 16, FN = WN / 16;
+    static_assert(FM * 16 == WM && FN * 16 == WN, "wave tile");
+    constexpr int RA = (BM * 4 + 511) / 512, RB = (BN * 4 + 511) / 512;
+    constexpr int SA = BM * BK, SB = BN * BK;
+    __shared__ __attribute__((aligned(16))) bf16 As[3 * SA];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[3 * SB];
+
+    const int ntn = p.N / BN, ntm = p.M / BM;
+    const int nblk = ntn * ntm;
+    int bid = blockIdx.x;
+    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    constexpr int GM = 8;
+    const int gsz = GM * ntn, g = bid / gsz, first = g * GM;
+    const int rows = ntm - first < GM ? ntm - first : GM;
+    const int local = bid - g * gsz;
+    const int m0 = (first + local % rows) * BM, n0 = (local / rows) * BN;
+    const long bz = blockIdx.y;
+    const bf16* __restrict__ A = p.A + bz * p.sA;
+    const bf16* __restrict__ B = p.B + bz * p.sB;
+    TO* __restrict__ C = (TO*)p.C + bz * p.sC;
+    const long lda = p.lda, ldb = p.ldb, psA = p.psA, psB = p.psB;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm0 = (wid / WGN) * WM, wn0 = (wid % WGN) * WN;
+
+    u32x4 ra[3][RA], rb[3][RB];
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+
+    const int nk = p.K / BK;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        x3w_gload<BM>(ra[q], A + q * psA, lda, m0, 0, tid);
+        x3w_gload<BN>(rb[q], B + q * psB, ldb, n0, 0, tid);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        x3w_sstore<BM>(ra[q], As + q * SA, tid);
+        x3w_sstore<BN>(rb[q], Bs + q * SB, tid);
+    }
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        if (t + 1 < nk) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                x3w_gload<BM>(ra[q], A + q * psA, lda, m0, (t + 1) * BK, tid);
+                x3w_gload<BN>(rb[q], B + q * psB, ldb, n0, (t + 1) * BK, tid);
+            }
+        }
+        __builtin_amdgcn_s_setprio(1);
+        frag_t a[3][FM];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+                a[q][i] = *(const frag_t*)(As + q * SA + x3_swz(wm0 + i * 16 + (lane & 15), lane >> 4));
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+            frag_t b[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                b[q] = *(const frag_t*)(Bs + q * SB + x3_swz(wn0 + j * 16 + (lane & 15), lane >> 4));
+#pragma unroll
+            for (int i = 0; i < FM; i++) {
+                // smallest terms first
+                acc[i][j] = mma(a[2][i], b[0], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[2], acc[i][j]);
+                acc[i][j] = mma(a[1][i], b[1], acc[i][j]);
+                acc[i][j] = mma(a[1][i], b[0], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[1], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[0], acc[i][j]);
+            }
+        }
+        __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+        if (t + 1 < nk) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                x3w_sstore<BM>(ra[q], As + q * SA, tid);
+                x3w_sstore<BN>(rb[q], Bs + q * SB, tid);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long row = m0 + wm0 + i * 16 + T::crow(lane, r);
+                const long col = n0 + wn0 + j * 16 + (lane & 15);
+                TO* cp = C + row * p.ldc + col;
+                if constexpr (std::is_same<TO, double>::value) {
+                    double v = (double)acc[i][j][r] * (double)p.alpha;
+                    if (p.accumulate) v += *cp;
+                    *cp = v;
+                } else {
+                    float v = acc[i][j][r] * p.alpha;
+                    if (p.accumulate) v += to_f(*cp);
+                    *cp = cvt_from_f<TO>(v);
+                }
+            }
+}
+
 // Grouped GEMM: one persistent launch over the output tiles of many problems
 // (problem q owns tiles [off[q], off[q+1]), row-major over its N/BN columns).
 // Workgroups stride over the tile list; the owning problem is found by a
@@ -513,6 +659,7 @@ static const TileCfg kTiles[] = {
     {64, 64, false, "64x64", false},     {32, 64, false, "32x64", false},   {64, 32, false, "64x32", false},
     {32, 32, false, "32x32", false},     {128, 128, true, "128x128d", false}, {128, 64, true, "128x64d", false},
     {64, 128, true, "64x128d", false},   {96, 128, false, "96x128", true},  {128, 96, false, "128x96", true},
+    {192, 256, false, "192x256w8", true}, {256, 192, false, "256x192w8", true},
 };
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int gemm_num_tiles() { return kNumTiles; }
@@ -602,7 +749,10 @@ static hipError_t launch_x3(int code, const GemmX3Params& p, int batch, hipStrea
     case 8: gemm_x3_k<TO, 128, 64, 64><<<grid, 256, 0, s>>>(p); break;
     case 9: gemm_x3_k<TO, 64, 128, 64><<<grid, 256, 0, s>>>(p); break;
     case 10: gemm_x3_k<TO, 96, 128, 32><<<grid, 256, 0, s>>>(p); break;
-    default: gemm_x3_k<TO, 128, 96, 32><<<grid, 256, 0, s>>>(p); break;
+    case 11: gemm_x3_k<TO, 128, 96, 32><<<grid, 256, 0, s>>>(p); break;
+    // eight waves, one workgroup per CU
+    case 12: gemm_x3w_k<TO, 192, 256, 2, 4><<<grid, 512, 0, s>>>(p); break;
+    default: gemm_x3w_k<TO, 256, 192, 4, 2><<<grid, 512, 0, s>>>(p); break;
     }
     return hipGetLastError();
 }

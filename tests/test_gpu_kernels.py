@@ -318,10 +318,15 @@ def test_split_bf16_gemm_reaches_fp32_accuracy(ctx, tin, tol, tile):
     (3072, 512, 128, 11, 1),   # 32 row blocks of 96: four full groups of 8
     (256, 288, 512, 12, 1),    # 128 x 96 tiles
     (1024, 384, 1024, 12, 2),  # 128 x 96 tiles with split-K into FP64
+    (384, 512, 256, 13, 1),    # eight-wave 192 x 256 tiles (96 x 64 per wave)
+    (3072, 1024, 64, 13, 1),   # 16 row blocks of 192: two full groups of 8
+    (512, 384, 512, 14, 1),    # eight-wave 256 x 192 tiles (64 x 96 per wave)
+    (1024, 384, 1024, 14, 4),  # 256 x 192 tiles with a 4-way split-K into FP64
 ])
 def test_split_bf16_96_wide_tiles(ctx, M, N, K, tile, ks):
     # the 96-row / 96-column tiles of the split-bfloat16 product (three row
-    # or column fragments per wave); the other element types refuse them
+    # or column fragments per wave) and its eight-wave tiles; the other
+    # element types refuse them
     A = rand((1, M, K), M + 7)
     B = rand((1, K, N), N + 8)
     ref = A @ B
