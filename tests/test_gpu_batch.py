@@ -130,3 +130,34 @@ def test_batch_rejects_fp64_options(ctx):
     with pytest.raises(tn.TnError):
         b.sweep(0, tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, max_krylov=16))
     b.close()
+
+
+def test_batch_of_128_chains_matches_single_chain_sweeps(ctx):
+    # the batched workload's chain count: 128 chains of the batched workload's
+    # couplings (lengths 8-24, bond dimensions 8-48 to keep the test short),
+    # one batched sweep pair against the same sweeps chain by chain from the
+    # same states; converged local solves in FP32 contractions (whose
+    # summation order differs between the grouped and the single launches)
+    # agree to |dE|/L <= 1e-8
+    o = fp32_opts(reorth=1)
+    spec = batched_workload(128, 4242, L_range=(8, 24), D_range=(8, 48))
+    batch_chains = [make_chain(ctx, *s, seed=500 + k, opts=o) for k, s in enumerate(spec)]
+    b = tn.Batch(ctx, batch_chains, max_krylov=40)
+    st = tn.SweepStats()
+    b.sweep(0, o, st)
+    b.sweep(1, o, st)
+    eb = [c.expectation() for c in batch_chains]
+    b.close()
+    for c in batch_chains:
+        c.close()
+    worst = 0.0
+    for k, s in enumerate(spec):
+        c = make_chain(ctx, *s, seed=500 + k, opts=o)
+        c.sweep(0, o)
+        c.sweep(1, o)
+        es = c.expectation()
+        worst = max(worst, abs(eb[k] - es) / s[0])
+        c.close()
+    assert worst < 1e-8, worst
+    # nearly every local solve converges (the counts cover all 128 chains)
+    assert st.n_solve_converged > 0 and st.n_solve_capped <= 0.01 * st.n_solve_converged

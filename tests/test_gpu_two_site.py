@@ -61,3 +61,33 @@ def test_two_site_eps_truncation(ctx):
         st, tr = ch.sweep2(1, 32, 1e-8, o)
     assert tr < 1e-8
     assert abs(ch.expectation() - O.ed_ground(L, 2)) < 1e-6
+
+
+def test_two_site_L64_ramp_to_256(ctx):
+    # the two-site workload at its full size: S=1/2 Heisenberg L=64, two-site
+    # sweeps at Dmax 32, 64, 128, 256 (FP64, tolerance 1e-12, no cutoff).
+    # What the mathematics fixes: the energy falls with Dmax, the truncation at
+    # 256 is negligible, and one-site sweeps at D=256 from the final state
+    # (which cannot leave the kept space) agree with it to the truncation level
+    L = 64
+    o = tn.SweepOpts.default(precision=tn.TN_FP64, env_precision=tn.TN_FP64, eig_tol=1e-12, max_krylov=24,
+                             max_restarts=8, gauge=0, reorth=1)
+    ch = tn.Chain(ctx, xxz_mpo(L, 0.5), 32, Dcap=256)
+    ch.initialize(1, 20261202)
+    ch.normalize(1)
+    ch.load_caches(o)
+    es = []
+    for Dmax in (32, 64, 128, 256):
+        ch.sweep2(0, Dmax, 0.0, o)
+        _, tr = ch.sweep2(1, Dmax, 0.0, o)
+        es.append(ch.expectation() / (L - 1))
+    assert max(ch.dims()[1]) == 256
+    assert all(b < a + 1e-12 for a, b in zip(es, es[1:])), es
+    assert tr < 1e-12, tr
+    e2 = es[-1]
+    ch.load_caches(o)
+    ch.sweep(0, o)
+    ch.sweep(1, o)
+    e1 = ch.expectation() / (L - 1)
+    assert abs(e1 - e2) < 1e-9, (e1, e2)
+    ch.close()
