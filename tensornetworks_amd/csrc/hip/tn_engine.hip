@@ -188,7 +188,7 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     const int nR = dw2 - (wr >= 0 ? 1 : 0);
     auto cl = [&](int w) { return w - (wl >= 0 && w > wl ? 1 : 0); };
     auto cr = [&](int w) { return w - (wr >= 0 && w > wr ? 1 : 0); };
-    std::vector<int> optr, esel, osel;
+    std::vector<int> optr, esel, osel, ocode, ecode;
     std::vector<long> ooff, eoff;
     std::vector<double> ev;
     auto Wv = [&](int w, int wp, int mm, int n) { return W[(((long)w * dw2 + wp) * p + mm) * p + n]; };
@@ -199,11 +199,13 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
                 if (heff && w == wl) {
                     ooff.push_back((long)mm * D2p);  // y[a][mm][b]
                     osel.push_back(1);
+                    ocode.push_back(mm | (1 << 16));
                 } else {
                     if (op == MIX_HEFF_T) ooff.push_back((long)mm * D2p * ldo + (long)cl(w) * D1p);
                     else if (op == MIX_HEFF) ooff.push_back((long)cl(w) * D1p * p * D2p + (long)mm * D2p);
                     else ooff.push_back((long)w * p * D2p + (long)mm * D2p);
                     osel.push_back(0);
+                    ocode.push_back(mm | (cl(w) << 8));
                 }
                 for (int wp = 0; wp < dw2; wp++)
                     for (int n = 0; n < p; n++) {
@@ -212,9 +214,11 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
                             if (wp == wr) {
                                 eoff.push_back((long)n * D2p);  // x[a'][n][b]
                                 esel.push_back(1);
+                                ecode.push_back(n | (1 << 16));
                             } else {
                                 eoff.push_back((long)n * nR * D2p + (long)cr(wp) * D2p);
                                 esel.push_back(0);
+                                ecode.push_back(n | (cr(wp) << 8));
                             }
                             ev.push_back(v);
                         }
@@ -243,7 +247,9 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     size_t o1 = 0, o2 = o1 + al((nout + 1) * sizeof(int)), o3 = o2 + al(nout * sizeof(long) + 8),
            o4 = o3 + al(nnz * sizeof(long) + 8), o5 = o4 + al(nnz * sizeof(double) + 8),
            o6 = o5 + al(nnz * sizeof(float) + 8), o7 = o6 + al(nnz * sizeof(int) + 8),
-           tot = o7 + al(nout * sizeof(int) + 8);
+           o8 = o7 + al(nout * sizeof(int) + 8), o9 = o8 + al(nout * sizeof(int) + 8),
+           tot = o9 + al(nnz * sizeof(int) + 8);
+    const bool codes = op == MIX_HEFF_T;
     std::vector<char> host(tot, 0);
     std::memcpy(host.data() + o1, optr.data(), (nout + 1) * sizeof(int));
     if (nout) std::memcpy(host.data() + o2, ooff.data(), nout * sizeof(long));
@@ -255,6 +261,10 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     if (ident) {
         if (nnz) std::memcpy(host.data() + o6, esel.data(), nnz * sizeof(int));
         if (nout) std::memcpy(host.data() + o7, osel.data(), nout * sizeof(int));
+    }
+    if (codes) {
+        if (nout) std::memcpy(host.data() + o8, ocode.data(), nout * sizeof(int));
+        if (nnz) std::memcpy(host.data() + o9, ecode.data(), nnz * sizeof(int));
     }
     m.buf.ensure(tot);
     TN_HIP(hipMemcpyAsync(m.buf.p, host.data(), tot, hipMemcpyHostToDevice, s));
@@ -269,6 +279,8 @@ void mix_build(DevMix& m, int op, const double* W, int dw1, int dw2, int p, int 
     m.t.evalf = (const float*)(b + o5);
     m.t.esel = ident ? (const int*)(b + o6) : nullptr;
     m.t.osel = ident ? (const int*)(b + o7) : nullptr;
+    m.t.ocode = codes ? (const int*)(b + o8) : nullptr;
+    m.t.ecode = codes ? (const int*)(b + o9) : nullptr;
     m.wsrc = W;
     m.wkey.assign(W, W + nw);
     m.op = op;
@@ -406,9 +418,14 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
     } else if (op.precision == TN_BF16X3) {
         // split-bfloat16 contractions: x, the environments and the MPO-stage
         // output as three bfloat16 planes, stage 1 into FP32, stage 3 into y
+        // stage 1 with the MPO stage in its epilogue (no T1 in HBM) where the
+        // fused kernel covers the shape; the allocation-free apply keeps the
+        // two kernels (its work space is sized for them)
+        static const bool no_fuse = getenv("TN_NO_FUSED_MIX") != nullptr;
+        const bool fuse = !scr && !no_fuse && x3_stage1_mix_ok(p, nR, op.mix->t, D1p, D2p);
         if (!scr) {
             c->xlo.ensure((size_t)n * 6);
-            c->T1.ensure(n1 * 4);
+            if (!fuse) c->T1.ensure(n1 * 4);
             c->T2.ensure((size_t)n2 * 6);
             XL = c->xlo.p;
             T1 = c->T1.p;
@@ -417,13 +434,22 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
         TN_REQUIRE(op.Lps > 0 && op.Rps > 0, TN_ERR_STATE, "heff_apply: split-bfloat16 operator without plane strides");
         TN_HIP(split3(T_F64, x, XL, n, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
-        g1.tin = T_BF16X3; g1.tout = T_F32; g1.A = XL; g1.psA = n;
-        g1.B = (const bf16*)op.Rlo + roff; g1.psB = op.Rps; g1.C = T1;
-        run(g1);
-        if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
-        TN_HIP(wmix_idt(T_F32, T_F64, T_BF16X3, T1, (long)p * nR * D2p, x, (long)p * D2p, T2, (long)nL * D1p, y,
-                        (long)p * D2p, D1p, D2p, op.mix->t, s, n2));
-        if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
+        if (fuse) {
+            TN_HIP(x3_stage1_mix(p, nR, (const bf16*)XL, n, (const bf16*)op.Rlo + roff, op.Rps, x, (bf16*)T2, n2,
+                                 (long)nL * D1p, y, D1p, D2p, op.mix->t, s));
+            if (timed) {
+                TN_HIP(hipEventRecord(c->ev[2], s));
+                TN_HIP(hipEventRecord(c->ev[3], s));
+            }
+        } else {
+            g1.tin = T_BF16X3; g1.tout = T_F32; g1.A = XL; g1.psA = n;
+            g1.B = (const bf16*)op.Rlo + roff; g1.psB = op.Rps; g1.C = T1;
+            run(g1);
+            if (timed) TN_HIP(hipEventRecord(c->ev[2], s));
+            TN_HIP(wmix_idt(T_F32, T_F64, T_BF16X3, T1, (long)p * nR * D2p, x, (long)p * D2p, T2, (long)nL * D1p,
+                            y, (long)p * D2p, D1p, D2p, op.mix->t, s, n2));
+            if (timed) TN_HIP(hipEventRecord(c->ev[3], s));
+        }
         g3.tin = T_BF16X3; g3.tout = T_F64; g3.A = (const bf16*)op.Llo + loff; g3.psA = op.Lps;
         g3.B = T2; g3.psB = n2; g3.C = y;
         run(g3);

@@ -81,9 +81,27 @@ struct MixTable {
     const float* evalf;  // device, nnz (same values, fp32)
     const int* esel;     // device, nnz (identity-channel tables only): 1 = entry reads in2
     const int* osel;     // device, nout (identity-channel tables only): 1 = output goes to out2
+    // MIX_HEFF_T only, for the fused stage 1 + MPO stage: per output
+    // m | (kept left channel << 8) | (osel << 16), per entry
+    // n | (kept right channel << 8) | (esel << 16)
+    const int* ocode;
+    const int* ecode;
 };
 hipError_t wmix(int tin, int tout, const void* in, long isx, void* out, long osx, int X,
                 int Bn, const MixTable& t, hipStream_t s);
+// Stage 1 of the split-bfloat16 Heff apply with the MPO stage in its
+// epilogue: T1 = X R^T tile by tile (X: planes of x, rows (a', n), K-major,
+// ld D2p; R: planes of the nR kept right channels, rows (w', b), ld D2p), then
+// in LDS T2t[(m, b)][(w, a')] = sum W T1 (+ the right identity channel from
+// the FP64 x), written as three bfloat16 planes (plane stride ps2, row
+// stride ldo), and the left identity channel's slice straight into y[a'][m][b]
+// (FP64).  One workgroup per 32 a' x 32 b.  Returns hipErrorInvalidValue for
+// shapes it does not cover (p not 2 or 3, nR not 2..5, tables larger than
+// 32 outputs / 256 entries): the caller then runs the two-kernel path.
+bool x3_stage1_mix_ok(int p, int nR, const MixTable& t, int D1p, int D2p);
+hipError_t x3_stage1_mix(int p, int nR, const bf16* X, long psX, const bf16* R, long psR, const double* x,
+                         bf16* T2, long ps2, long ldo, double* y, int D1p, int D2p, const MixTable& t,
+                         hipStream_t s);
 // The Heff mixing with a second input and a second output, its stage-3
 // operand written transposed: entries e with esel[e] read
 // in2[x*isx2 + eoff[e] + b] (type tx), outputs o with osel[o] write

@@ -1085,6 +1085,267 @@ hipError_t wmix_idt(int tin, int tx, int tout, const void* in, long isx, const v
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Stage 1 of the split-bfloat16 Heff apply with the MPO stage fused into its
+// epilogue.  A workgroup owns 32 a' x 32 b: its stage-1 tile is rows
+// (a', n) for the 32 a' (32 P rows of x) and columns (w', b) for every kept
+// right channel w' and the 32 b (the R rows w' D2p + b0 + 0..31), so every
+// (n, w') term of the MPO stage for its outputs is in the tile.  The K loop
+// is the four-wave loop of gemm_tile_x3 (swizzled LDS, register prefetch);
+// then the FP32 tile and the FP64 x slice of the right identity channel go
+// to LDS and the mixing sums are formed there: outputs of the kept left
+// channels with lanes along a' (32 consecutive a' of one (m, b, w) are one
+// 64-byte run of each plane), the left identity channel's outputs with
+// lanes along b (coalesced FP64 rows of y).
+struct X3MixParams {
+    const bf16* X;
+    long psX;
+    const bf16* R;
+    long psR;
+    const double* x;
+    bf16* T2;
+    long ps2, ldo;
+    double* y;
+    int D1p, D2p;
+    const int* optr;
+    const int* ocode;
+    const int* ecode;
+    const float* val;
+    int nout, nnz;
+};
+
+template <int P, int NR>
+__global__ void __launch_bounds__(256) x3_stage1_mix_k(X3MixParams q)
+{
+    typedef MT<bf16> T;
+    typedef T::frag_t frag_t;
+    typedef T::acc_t acc_t;
+    constexpr int BM = 32 * P, BN = 32 * NR, BK = 32;
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int SA = BM * BK, SB = BN * BK;
+    constexpr int RA = (BM * 4 + 255) / 256, RB = (BN * 4 + 255) / 256;
+    constexpr int TS = BN + 1;  // row stride of the FP32 tile (floats)
+    constexpr int XS = 33;      // row stride of the x slice
+    constexpr int OPB = 3 * (SA + SB) * 2;
+    constexpr int EPB = (BM * TS + BM * XS) * 4;
+    constexpr int LB = OPB > EPB ? OPB : EPB;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[LB];
+    __shared__ int s_optr[33], s_ocode[32], s_ecode[256];
+    __shared__ float s_val[256];
+    bf16* As = (bf16*)smem;
+    bf16* Bs = As + 3 * SA;
+    float* Ts = (float*)smem;
+    float* Xs = Ts + BM * TS;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int ntm = q.D1p / 32, ntn = q.D2p / 32, nblk = ntm * ntn;
+    int bid = blockIdx.x;
+    if ((nblk & 7) == 0) bid = (bid & 7) * (nblk >> 3) + (bid >> 3);
+    constexpr int GM = 8;
+    const int gsz = GM * ntn, g = bid / gsz, first = g * GM;
+    const int rows = ntm - first < GM ? ntm - first : GM;
+    const int local = bid - g * gsz;
+    const int a0 = (first + local % rows) * 32, b0 = (local / rows) * 32;
+    const int m0 = a0 * P;
+    const long D2p = q.D2p;
+
+    for (int i = tid; i <= q.nout; i += 256) s_optr[i] = q.optr[i];
+    for (int i = tid; i < q.nout; i += 256) s_ocode[i] = q.ocode[i];
+    for (int i = tid; i < q.nnz; i += 256) {
+        s_ecode[i] = q.ecode[i];
+        s_val[i] = q.val[i];
+    }
+    // the x slice of the right identity channel (rows (a', n), 32 b), read
+    // now so that its latency hides behind the K loop
+    constexpr int XR = BM * 32 / 256;
+    float xr[XR];
+#pragma unroll
+    for (int k = 0; k < XR; k++) {
+        const int v = tid + k * 256;
+        xr[k] = (float)q.x[(long)(m0 + (v >> 5)) * D2p + b0 + (v & 31)];
+    }
+
+    const int wm0 = (wid >> 1) * WM, wn0 = (wid & 1) * WN;
+    u32x4 ra[3][RA], rb[3][RB];
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+    // operand slices: A rows m0 + t, B rows (t / 32) D2p + b0 + t % 32
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int pl = 0; pl < 3; pl++) {
+#pragma unroll
+            for (int r = 0; r < RA; r++) {
+                const int v = tid + r * 256;
+                if ((BM * 4) % 256 == 0 || v < BM * 4)
+                    ra[pl][r] = *(const u32x4*)(q.X + pl * q.psX + (long)(m0 + v / 4) * D2p + k0 + (v % 4) * 8);
+            }
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const int v = tid + r * 256;
+                if ((BN * 4) % 256 == 0 || v < BN * 4) {
+                    const int t = v / 4;
+                    const long row = (long)(t >> 5) * D2p + b0 + (t & 31);
+                    rb[pl][r] = *(const u32x4*)(q.R + pl * q.psR + row * D2p + k0 + (v % 4) * 8);
+                }
+            }
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int pl = 0; pl < 3; pl++) {
+            x3_sstore<BM, BK>(ra[pl], As + pl * SA, tid);
+            x3_sstore<BN, BK>(rb[pl], Bs + pl * SB, tid);
+        }
+    };
+    const int nk = (int)(D2p / BK);
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int t = 0; t < nk; t++) {
+        if (t + 1 < nk) gload((t + 1) * BK);
+        __builtin_amdgcn_s_setprio(1);
+        frag_t a[3][FM], b[3][FN];
+#pragma unroll
+        for (int pl = 0; pl < 3; pl++) {
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+                a[pl][i] = *(const frag_t*)(As + pl * SA + x3_swz(wm0 + i * 16 + (lane & 15), lane >> 4));
+#pragma unroll
+            for (int j = 0; j < FN; j++)
+                b[pl][j] = *(const frag_t*)(Bs + pl * SB + x3_swz(wn0 + j * 16 + (lane & 15), lane >> 4));
+        }
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++) {
+                acc[i][j] = mma(a[2][i], b[0][j], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[2][j], acc[i][j]);
+                acc[i][j] = mma(a[1][i], b[1][j], acc[i][j]);
+                acc[i][j] = mma(a[1][i], b[0][j], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[1][j], acc[i][j]);
+                acc[i][j] = mma(a[0][i], b[0][j], acc[i][j]);
+            }
+        __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+        if (t + 1 < nk) {
+            sstore();
+            __syncthreads();
+        }
+    }
+    // the operand tiles are consumed (barrier above): the LDS takes the FP32
+    // stage-1 tile and the x slice (rows (a', n), 32 b)
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                Ts[(wm0 + i * 16 + T::crow(lane, r)) * TS + wn0 + j * 16 + (lane & 15)] = acc[i][j][r];
+#pragma unroll
+    for (int k = 0; k < XR; k++) {
+        const int v = tid + k * 256;
+        Xs[(v >> 5) * XS + (v & 31)] = xr[k];
+    }
+    __syncthreads();
+    // sums of the table entries of output o for NV values at rows a_i = al0 +
+    // i * astep (a' within the tile) and column bl: an entry's code and value
+    // are read once, then its NV source reads are independent
+    auto mixsums = [&](int o, int al0, int astep, int bl, float* sum, auto nv) {
+        constexpr int NV = decltype(nv)::value;
+#pragma unroll
+        for (int i = 0; i < NV; i++) sum[i] = 0.f;
+        const int e1 = s_optr[o + 1];
+        for (int e = s_optr[o]; e < e1; e++) {
+            const int ec = s_ecode[e], n = ec & 255;
+            const float v = s_val[e];
+            const bool fromx = ec >> 16;
+            const float* src = fromx ? Xs + n * XS + bl : Ts + n * TS + ((ec >> 8) & 255) * 32 + bl;
+            const int rs = (fromx ? XS : TS) * P;  // row step per a'
+#pragma unroll
+            for (int i = 0; i < NV; i++) sum[i] += v * src[(al0 + i * astep) * rs];
+        }
+    };
+    // kept left channels: T2t[(m, b)][(w, a')] as three planes; a lane forms
+    // 8 consecutive a' of one (m, b, w) and stores them as one 16-byte write
+    // per plane (4 lanes cover the tile's 32 a'), the two halves of the
+    // workgroup take alternate outputs
+    {
+        const int a8 = (tid & 3) * 8, bl = (tid >> 2) & 31, oh = tid >> 7;
+        for (int o = oh; o < q.nout; o += 2) {
+            const int oc = s_ocode[o];
+            if (oc >> 16) continue;
+            float v8[8];
+            mixsums(o, a8, 1, bl, v8, std::integral_constant<int, 8>());
+            uint32_t h[3][4];
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                bf16 p0, p1, p2, r0, r1, r2;
+                split3_val(v8[i], p0, p1, p2);
+                split3_val(v8[i + 1], r0, r1, r2);
+                h[0][i / 2] = (uint32_t)p0 | ((uint32_t)r0 << 16);
+                h[1][i / 2] = (uint32_t)p1 | ((uint32_t)r1 << 16);
+                h[2][i / 2] = (uint32_t)p2 | ((uint32_t)r2 << 16);
+            }
+            bf16* dst = q.T2 + (long)(oc & 255) * D2p * q.ldo + (long)((oc >> 8) & 255) * q.D1p +
+                        (long)(b0 + bl) * q.ldo + a0 + a8;
+#pragma unroll
+            for (int pl = 0; pl < 3; pl++) {
+                u32x4 w;
+                w.x = h[pl][0];
+                w.y = h[pl][1];
+                w.z = h[pl][2];
+                w.w = h[pl][3];
+                *(u32x4*)(dst + pl * q.ps2) = w;
+            }
+        }
+    }
+    // the left identity channel: y[a'][m][b], lanes along b
+    {
+        const int bl = tid & 31, ag = tid >> 5;
+        for (int o = 0; o < q.nout; o++) {
+            const int oc = s_ocode[o];
+            if (!(oc >> 16)) continue;
+            float v4[4];
+            mixsums(o, ag, 8, bl, v4, std::integral_constant<int, 4>());
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                q.y[((long)(a0 + ag + 8 * k) * P + (oc & 255)) * D2p + b0 + bl] = (double)v4[k];
+        }
+    }
+}
+
+bool x3_stage1_mix_ok(int p, int nR, const MixTable& t, int D1p, int D2p)
+{
+    return t.ocode && t.ecode && t.nout <= 32 && t.nnz <= 256 && D1p > 0 && D2p > 0 && D1p % 32 == 0 &&
+           D2p % 32 == 0 && (p == 2 || p == 3) && nR >= 2 && nR <= 5;
+}
+
+hipError_t x3_stage1_mix(int p, int nR, const bf16* X, long psX, const bf16* R, long psR, const double* x,
+                         bf16* T2, long ps2, long ldo, double* y, int D1p, int D2p, const MixTable& t,
+                         hipStream_t s)
+{
+    if (!x3_stage1_mix_ok(p, nR, t, D1p, D2p)) return hipErrorInvalidValue;
+    X3MixParams q;
+    q.X = X; q.psX = psX; q.R = R; q.psR = psR; q.x = x; q.T2 = T2; q.ps2 = ps2; q.ldo = ldo; q.y = y;
+    q.D1p = D1p; q.D2p = D2p;
+    q.optr = t.optr; q.ocode = t.ocode; q.ecode = t.ecode; q.val = t.evalf; q.nout = t.nout; q.nnz = t.nnz;
+    const dim3 grid((D1p / 32) * (D2p / 32));
+#define X3M_CASE(PP, NN)                                                  \
+    if (p == PP && nR == NN) {                                            \
+        x3_stage1_mix_k<PP, NN><<<grid, 256, 0, s>>>(q);                  \
+        return hipGetLastError();                                         \
+    }
+    X3M_CASE(2, 2) X3M_CASE(2, 3) X3M_CASE(2, 4) X3M_CASE(2, 5)
+    X3M_CASE(3, 2) X3M_CASE(3, 3) X3M_CASE(3, 4) X3M_CASE(3, 5)
+#undef X3M_CASE
+    return hipErrorInvalidValue;
+}
+
 __global__ void __launch_bounds__(256) publish_k(const double* __restrict__ src, int n, double* host_dst,
                                                  unsigned long long* host_flag, unsigned long long seq)
 {
