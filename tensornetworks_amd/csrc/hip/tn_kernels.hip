@@ -501,7 +501,7 @@ __device__ __forceinline__ void x3w_sstore(const u32x4 (&rg)[R], bf16* __restric
     }
 }
 
-template <typename TO, int BM, int BN, int WGM, int WGN>
+template <typename TO, int BM, int BN, int WGM, int WGN, bool DB>
 __global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
 {
     typedef MT<bf16> T;
@@ -513,8 +513,11 @@ __global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
     static_assert(FM * 16 == WM && FN * 16 == WN, "wave tile");
     constexpr int RA = (BM * 4 + 511) / 512, RB = (BN * 4 + 511) / 512;
     constexpr int SA = BM * BK, SB = BN * BK;
-    __shared__ __attribute__((aligned(16))) bf16 As[3 * SA];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[3 * SB];
+    // DB: two LDS stages, the next K slice stored into the other stage while
+    // the current one is in use, one barrier per step
+    constexpr int NS = DB ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) bf16 As[NS * 3 * SA];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[NS * 3 * SB];
 
     const int ntn = p.N / BN, ntm = p.M / BM;
     const int nblk = ntn * ntm;
@@ -556,6 +559,9 @@ __global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
     }
     __syncthreads();
     for (int t = 0; t < nk; t++) {
+        const int st = DB ? (t & 1) : 0;
+        const bf16* Ac = As + st * 3 * SA;
+        const bf16* Bc = Bs + st * 3 * SB;
         if (t + 1 < nk) {
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -569,13 +575,13 @@ __global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
         for (int q = 0; q < 3; q++)
 #pragma unroll
             for (int i = 0; i < FM; i++)
-                a[q][i] = *(const frag_t*)(As + q * SA + x3_swz(wm0 + i * 16 + (lane & 15), lane >> 4));
+                a[q][i] = *(const frag_t*)(Ac + q * SA + x3_swz(wm0 + i * 16 + (lane & 15), lane >> 4));
 #pragma unroll
         for (int j = 0; j < FN; j++) {
             frag_t b[3];
 #pragma unroll
             for (int q = 0; q < 3; q++)
-                b[q] = *(const frag_t*)(Bs + q * SB + x3_swz(wn0 + j * 16 + (lane & 15), lane >> 4));
+                b[q] = *(const frag_t*)(Bc + q * SB + x3_swz(wn0 + j * 16 + (lane & 15), lane >> 4));
 #pragma unroll
             for (int i = 0; i < FM; i++) {
                 // smallest terms first
@@ -588,14 +594,29 @@ __global__ void __launch_bounds__(512) gemm_x3w_k(GemmX3Params p)
             }
         }
         __builtin_amdgcn_s_setprio(0);
-        __syncthreads();
-        if (t + 1 < nk) {
+        if constexpr (DB) {
+            // the other stage was last read in step t - 1, before the barrier
+            // that ended it
+            if (t + 1 < nk) {
+                bf16* An = As + (st ^ 1) * 3 * SA;
+                bf16* Bn = Bs + (st ^ 1) * 3 * SB;
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                x3w_sstore<BM>(ra[q], As + q * SA, tid);
-                x3w_sstore<BN>(rb[q], Bs + q * SB, tid);
+                for (int q = 0; q < 3; q++) {
+                    x3w_sstore<BM>(ra[q], An + q * SA, tid);
+                    x3w_sstore<BN>(rb[q], Bn + q * SB, tid);
+                }
             }
             __syncthreads();
+        } else {
+            __syncthreads();
+            if (t + 1 < nk) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    x3w_sstore<BM>(ra[q], As + q * SA, tid);
+                    x3w_sstore<BN>(rb[q], Bs + q * SB, tid);
+                }
+                __syncthreads();
+            }
         }
     }
 #pragma unroll
@@ -660,6 +681,7 @@ static const TileCfg kTiles[] = {
     {32, 32, false, "32x32", false},     {128, 128, true, "128x128d", false}, {128, 64, true, "128x64d", false},
     {64, 128, true, "64x128d", false},   {96, 128, false, "96x128", true},  {128, 96, false, "128x96", true},
     {192, 256, false, "192x256w8", true}, {256, 192, false, "256x192w8", true},
+    {192, 128, false, "192x128w8db", true}, {128, 192, false, "128x192w8db", true},
 };
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 int gemm_num_tiles() { return kNumTiles; }
@@ -751,8 +773,11 @@ static hipError_t launch_x3(int code, const GemmX3Params& p, int batch, hipStrea
     case 10: gemm_x3_k<TO, 96, 128, 32><<<grid, 256, 0, s>>>(p); break;
     case 11: gemm_x3_k<TO, 128, 96, 32><<<grid, 256, 0, s>>>(p); break;
     // eight waves, one workgroup per CU
-    case 12: gemm_x3w_k<TO, 192, 256, 2, 4><<<grid, 512, 0, s>>>(p); break;
-    default: gemm_x3w_k<TO, 256, 192, 4, 2><<<grid, 512, 0, s>>>(p); break;
+    case 12: gemm_x3w_k<TO, 192, 256, 2, 4, false><<<grid, 512, 0, s>>>(p); break;
+    case 13: gemm_x3w_k<TO, 256, 192, 4, 2, false><<<grid, 512, 0, s>>>(p); break;
+    // eight waves, two LDS stages
+    case 14: gemm_x3w_k<TO, 192, 128, 4, 2, true><<<grid, 512, 0, s>>>(p); break;
+    default: gemm_x3w_k<TO, 128, 192, 2, 4, true><<<grid, 512, 0, s>>>(p); break;
     }
     return hipGetLastError();
 }

@@ -322,6 +322,11 @@ def test_split_bf16_gemm_reaches_fp32_accuracy(ctx, tin, tol, tile):
     (3072, 1024, 64, 13, 1),   # 16 row blocks of 192: two full groups of 8
     (512, 384, 512, 14, 1),    # eight-wave 256 x 192 tiles (64 x 96 per wave)
     (1024, 384, 1024, 14, 4),  # 256 x 192 tiles with a 4-way split-K into FP64
+    (384, 256, 256, 15, 1),    # eight waves, two LDS stages: 192 x 128
+    (3072, 512, 128, 15, 1),   # 16 row blocks of 192: two full groups of 8
+    (256, 384, 512, 16, 1),    # eight waves, two LDS stages: 128 x 192
+    (1024, 384, 96, 16, 1),    # three K steps (an odd count: the stages alternate)
+    (1024, 384, 1024, 16, 2),  # 128 x 192 with a 2-way split-K into FP64
 ])
 def test_split_bf16_96_wide_tiles(ctx, M, N, K, tile, ks):
     # the 96-row / 96-column tiles of the split-bfloat16 product (three row
