@@ -13,8 +13,8 @@ bfloat16 pieces, the six leading cross products on the bfloat16 matrix cores
 with FP32 accumulation, FP32 environment updates; Krylov vectors,
 re-orthogonalisation, tridiagonal solve, energies and QR/SVD gauge moves in
 FP64).  From the identical saved state the same K sweeps are then timed with
-every local solve refined by FP64 Lanczos iterations, with BF16 contractions
-refined by FP32 ones, and in FP64; |dE|/L compares the FP64 energies <H> of
+every local solve refined by FP64 Lanczos iterations, with FP32 contractions,
+with BF16 contractions refined by FP32 ones, and in FP64; |dE|/L compares the FP64 energies <H> of
 the final MPS of the headline and FP64 runs.  value = algorithmic Heff flops
 (identity channels counted, as if contracted) / device time of the Heff
 applications (HIP events) during the timed sweeps.
@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--no-skip-identity", action="store_true",
                    help="contract identity environment channels like the others instead of copying them")
     p.add_argument("--no-fp64-compare", action="store_true")
+    p.add_argument("--no-fp32-compare", action="store_true",
+                   help="skip the extra timed sweep with FP32 contractions (when the headline is another scheme)")
     p.add_argument("--no-refine-compare", action="store_true",
                    help="skip the extra timed sweep in the headline precision with every local solve refined "
                         "by FP64 Lanczos iterations")
@@ -514,6 +516,14 @@ def main():
     res_fp64 = None
     res_bf16 = None
     res_ref64 = None
+    res_fp32 = None
+    if a.precision not in ("fp32", "fp64") and not a.no_fp32_compare:
+        o_fp32 = tn.SweepOpts.default(precision=tn.TN_FP32, env_precision=tn.TN_FP32, refine=0, **common)
+        st32, wall32, E32 = timed_from_saved(o_fp32)
+        res_fp32 = {"scheme": "FP32 contractions, FP32 environments", "sweep_s": wall32 / a.steps,
+                    "E_per_site": E32 / a.L, "n_matvec": st32.n_matvec,
+                    "heff_tflops": st32.heff_flops / max(st32.heff_ms, 1e-12) / 1e9}
+        print(f"# fp32 {wall32:.2f}s E/L={E32 / a.L:.12f} matvecs={st32.n_matvec}", file=sys.stderr, flush=True)
     if a.precision != "fp64" and not a.no_refine_compare:
         str_, wallr, Er = timed_from_saved(o_ref64)
         res_ref64 = {"scheme": f"{a.precision} contractions, every local solve refined by FP64 Lanczos to eig_tol",
@@ -615,6 +625,7 @@ def main():
         "fp64": res_fp64,
         "bf16_then_fp32": res_bf16,
         "fp64_refined": res_ref64,
+        "fp32": res_fp32,
         "speedup_vs_fp64": (res_fp64["sweep_s"] / (wall / a.steps)) if res_fp64 else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAKS[hp], "unit": "TFLOP/s",
                      "frac": achieved / PEAKS[hp], "traffic": traffic["bytes"] if traffic else None,
