@@ -919,8 +919,73 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
 }
 
 // ---------------------------------------------------------------------------
-// Thin SVD by one-sided block Jacobi on the q <= p columns of X (p x q):
-// X = A (r >= c) or A^T (r < c).  X J = Q S, so A = Q S J^T or J S Q^T.
+// Householder QR of the column-major pp x qp matrix A (ld pp; pp >= qp, both
+// multiples of 32), blocked by 32 columns: each panel is factored by one
+// workgroup (hqr_panel), then the block reflector I - V T V^T is applied to
+// the trailing columns by three FP64 GEMMs,
+//   A_t <- A_t - V (T^T (V^T A_t)),
+// over rows c0.. only (V is zero above).  R overwrites A (zeros below the
+// diagonal).  Q (pp x qp, ld pp) is formed backwards from [I; 0]:
+//   Q[c0:, c0:] <- Q[c0:, c0:] - V (T (V^T Q[c0:, c0:])).
+// H holds the reflectors (pp x qp), T the panels' 32 x 32 factors, W1 and W2
+// qp x 32 each.
+static void house_qr(tn_ctx* c, double* A, int pp, int qp, double* Q, double* H, double* T, double* W1,
+                     double* W2)
+{
+    hipStream_t s = c->stream;
+    TN_REQUIRE(pp % 32 == 0 && qp % 32 == 0 && pp >= qp && qp > 0, TN_ERR_ARG, "house_qr: bad shape");
+    // Y (nt columns, ld pp, rows c0..pp-1) <- (I - V F V^T) Y, F = T^T for the
+    // factorisation (H_31 ... H_0 applied) or T for the Q formation
+    // (H_0 ... H_31), computed transposed: Y^T -= ((Y^T V) F^T) V^T
+    auto apply = [&](double* Y, int nt, int c0, const double* Tk, bool qform) {
+        const double* Vk = H + (long)c0 * pp + c0;
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.batch = 1; g.alpha = 1.0f;
+        // W1 = Y^T V  (nt x 32, row-major)
+        g.a_kmajor = true; g.b_kmajor = true;
+        g.M = nt; g.N = 32; g.K = pp - c0;
+        g.A = Y; g.lda = pp; g.B = Vk; g.ldb = pp; g.C = W1; g.ldc = 32;
+        TN_HIP(gemm(g, s));
+        // W2 = W1 T (the reflector's transpose on A) or W1 T^T (Q formation)
+        g.a_kmajor = true; g.b_kmajor = qform;
+        g.M = nt; g.N = 32; g.K = 32;
+        g.A = W1; g.lda = 32; g.B = Tk; g.ldb = 32; g.C = W2; g.ldc = 32;
+        TN_HIP(gemm(g, s));
+        // Y^T -= W2 V^T
+        g.a_kmajor = true; g.b_kmajor = false;
+        g.M = nt; g.N = pp - c0; g.K = 32;
+        g.A = W2; g.lda = 32; g.B = Vk; g.ldb = pp; g.C = Y; g.ldc = pp;
+        g.alpha = -1.0f; g.accumulate = 1;
+        TN_HIP(gemm(g, s));
+    };
+    const int np = qp / 32;
+    for (int k = 0; k < np; k++) {
+        const int c0 = 32 * k;
+        TN_HIP(hqr_panel(A, pp, pp, c0, H + (long)c0 * pp, pp, T + (long)k * 1024, s));
+        if (qp - c0 - 32 > 0) apply(A + (long)(c0 + 32) * pp + c0, qp - c0 - 32, c0, T + (long)k * 1024, false);
+    }
+    TN_HIP(fill_zero(Q, (long)pp * qp * 8, s));
+    TN_HIP(set_identity(Q, pp, qp, s));
+    for (int k = np - 1; k >= 0; k--) {
+        const int c0 = 32 * k;
+        apply(Q + (long)c0 * pp + c0, qp - c0, c0, T + (long)k * 1024, true);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thin SVD by one-sided block Jacobi on the q <= p columns of X0 (p x q):
+// X0 = A (r >= c) or A^T (r < c).
+//
+// Small q: X = X0, X J = Q S, so X0 = Q S J^T.
+//
+// q >= 128: QR-preconditioned (Z. Drmac, K. Veselic, SIAM J. Matrix Anal.
+// Appl. 29 (2008) 1322-1342): X0 P = Q_h R by Householder QR of the columns
+// sorted by norm (P), then the Jacobi runs on X = P R^T, X J = W S, so
+// X0 = (Q_h J) S W^T.  With a fast-decaying spectrum (every two-site split)
+// the plain iteration needs 20-50 sweeps; on P R^T it needs about 10.
+// largest cosine between two columns below which the plain iteration runs
+static const double kSvdPlainCos = 0.1;
+
 int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, double* U, long ldu,
                int zr, double* Vt, long ldvt, int zk, std::vector<double>& s_out)
 {
@@ -929,24 +994,93 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     const bool tall = r >= cc;
     const int p = tall ? r : cc, q = tall ? cc : r;
     const int qp = pad32(q);
-    const long ldx = p;
-    c->svd_X.ensure((size_t)ldx * qp * 8);
+    static const bool plain_only = getenv("TN_SVD_PLAIN") != nullptr;
+    const bool big = q >= 128 && !plain_only;
+    const int pp = pad32(p);
     c->svd_V.ensure((size_t)qp * qp * 8);
-    double* X = c->svd_X.as<double>();
     double* V = c->svd_V.as<double>();
-    // X[j*ldx + i] = tall ? A(i,j) : A(j,i)
-    if (tall) TN_HIP(copy2(A, cs, rs, q, p, X, ldx, 1, qp, p, s));
-    else TN_HIP(copy2(A, rs, cs, q, p, X, ldx, 1, qp, p, s));
-    TN_HIP(set_identity(V, qp, qp, s));
-    // scale invariance: X /= max|X| (Gram entries of tiny matrices would underflow)
     c->svd_s.ensure((size_t)(qp + 64) * 8 * 2 + 1024 * 8);
     double* amax_d = c->svd_s.as<double>() + 2 * qp + 128;
+    double* nd = c->svd_s.as<double>();
     c->part.ensure(1024 * 8);
-    TN_HIP(absmax(X, (long)ldx * qp, amax_d, c->part.as<double>(), 1024, s));
     double amax = 0.0;
-    d2h(c, &amax, amax_d, 8);
-    if (!std::isfinite(amax)) throw TnError{TN_ERR_STATE, "svd: non-finite matrix entries"};
-    if (amax > 0.0) TN_HIP(scale_by_inv(X, amax_d, (long)ldx * qp, s));
+    // X0 (column j of X0 at B + j*ldb), scaled to max |entry| = 1 (Gram
+    // entries of tiny matrices would underflow)
+    auto load_scaled = [&](double* B, long ldb, int rows_out) {
+        if (tall) TN_HIP(copy2(A, cs, rs, q, p, B, ldb, 1, qp, rows_out, s));
+        else TN_HIP(copy2(A, rs, cs, q, p, B, ldb, 1, qp, rows_out, s));
+        TN_HIP(absmax(B, ldb * qp, amax_d, c->part.as<double>(), 1024, s));
+        d2h(c, &amax, amax_d, 8);
+        if (!std::isfinite(amax)) throw TnError{TN_ERR_STATE, "svd: non-finite matrix entries"};
+        if (amax > 0.0) TN_HIP(scale_by_inv(B, amax_d, ldb * qp, s));
+    };
+    // the Jacobi iterate X (pj rows, column j at X + j*ldx)
+    double* X = nullptr;
+    long ldx = p;
+    int pj = p;
+    bool pre = false;
+    double* Qh = nullptr;  // pre: Q_h (pp x qp, ld pp)
+    if (big) {
+        c->svd_R.ensure((size_t)pp * qp * 8);
+        c->svd_Q.ensure((size_t)pp * qp * 8);
+        c->svd_H.ensure((size_t)pp * qp * 8);
+        double* R = c->svd_R.as<double>();
+        double* Hb = c->svd_H.as<double>();
+        load_scaled(Hb, pp, pp);
+        // precondition only columns far from orthogonal: the largest cosine
+        // between two columns, from the Gram matrix (one GEMM).  Nearly
+        // orthogonal columns (a one-site gauge move's site tensor) converge
+        // in a few plain sweeps, where the QR would cost more than it saves.
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.batch = 1; g.alpha = 1.0f;
+        g.a_kmajor = true; g.b_kmajor = true;
+        g.M = qp; g.N = qp; g.K = pp;
+        g.A = Hb; g.lda = pp; g.B = Hb; g.ldb = pp; g.C = R; g.ldc = qp;
+        TN_HIP(gemm(g, s));
+        TN_HIP(max_cos(R, qp, q, nd, s));
+        std::vector<double> mc(q);
+        d2h(c, mc.data(), nd, (size_t)q * 8);
+        const double cmax = *std::max_element(mc.begin(), mc.end());
+        pre = !(cmax < kSvdPlainCos);
+        if (getenv("TN_SVD_TRACE")) fprintf(stderr, "# svdcos q=%d cmax=%.3e pre=%d\n", q, cmax, (int)pre);
+        if (pre) {
+            c->svd_X.ensure((size_t)qp * qp * 8);
+            X = c->svd_X.as<double>();
+            ldx = qp;
+            pj = qp;
+            const size_t wt = (size_t)2 * qp * 32 + (size_t)(qp / 32) * 1024;
+            c->svd_W.ensure((wt + (size_t)qp / 2 + 8) * 8);
+            Qh = c->svd_Q.as<double>();
+            double* W = c->svd_W.as<double>();
+            int* dpiv = (int*)(W + wt);
+            // static column pivoting: the QR sees the columns by decreasing
+            // norm (the grading the preconditioning relies on; a two-site
+            // split's columns come in blocks by physical index, and its
+            // unsorted QR gave the Jacobi no fewer sweeps than the plain
+            // iteration)
+            TN_HIP(col_norms(Hb, pp, p, q, nd, s));
+            std::vector<double> cn(q);
+            d2h(c, cn.data(), nd, (size_t)q * 8);
+            std::vector<int> piv(qp);
+            std::iota(piv.begin(), piv.end(), 0);
+            std::stable_sort(piv.begin(), piv.begin() + q, [&](int a, int b) { return cn[a] > cn[b]; });
+            h2d(c, dpiv, piv.data(), (size_t)qp * sizeof(int));
+            TN_HIP(gather_cols(Hb, pp, R, pp, dpiv, pp, qp, s));
+            house_qr(c, R, pp, qp, Qh, Hb, W + (size_t)2 * qp * 32, W, W + (size_t)qp * 32);
+            // X0 P = Q_h R, so X0 = Q_h (P R^T)^T: X = P R^T,
+            // X[j*qp + piv[i]] = R(j, i) = R[i*pp + j]
+            TN_HIP(tr_scatter(R, pp, X, qp, dpiv, qp, s));
+        } else {
+            // the plain iteration on the loaded copy (rows p..pp-1 are zero)
+            X = Hb;
+            ldx = pp;
+        }
+    } else {
+        c->svd_X.ensure((size_t)p * qp * 8);
+        X = c->svd_X.as<double>();
+        load_scaled(X, ldx, p);
+    }
+    TN_HIP(set_identity(V, qp, qp, s));
     // tournament pairs for nb = qp/16 column blocks
     const int nb = qp / 16;
     std::vector<int> pairs((size_t)(nb - 1) * nb);
@@ -960,7 +1094,7 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     h2d(c, c->svd_pairs.p, pairs.data(), pairs.size() * sizeof(int));
     c->svd_flag.ensure(256);
     int* flag = c->svd_flag.as<int>();
-    const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)p));
+    const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)pj));
     int hflag = 1;
     // one sweep of rotations on each pair's Gram matrix per visit: the pairs
     // come back every outer sweep, and a launch has only nb/2 workgroups, so
@@ -972,14 +1106,14 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     for (int sweep = 0; sweep < 80 && hflag; sweep++) {
         TN_HIP(set_int(flag, 0, s));
         for (int rd = 0; rd < nb - 1; rd++)
-            TN_HIP(jacobi_step(X, ldx, p, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb,
+            TN_HIP(jacobi_step(X, ldx, pj, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb,
                                nb / 2, tol, flag, s, inner));
         TN_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         sync(c);
     }
     // singular values = column norms; sort descending
     double* sd = c->svd_s.as<double>();
-    TN_HIP(col_norms(X, ldx, p, q, sd, s));
+    TN_HIP(col_norms(X, ldx, pj, q, sd, s));
     std::vector<double> sv(q);
     TN_HIP(hipMemcpyAsync(sv.data(), sd, q * 8, hipMemcpyDeviceToHost, s));
     sync(c);
@@ -998,39 +1132,42 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     // normalise the good columns of X
     double* dinv = sd + qp + 32;
     h2d(c, dinv, inv.data(), qp * 8);
-    TN_HIP(scale_rows(X, ldx, q, p, dinv, s));
-    // orthonormal completion of (exactly) zero columns, in the sorted order
+    TN_HIP(scale_rows(X, ldx, q, pj, dinv, s));
+    // orthonormal completion of (exactly) zero columns, in the sorted order,
+    // within the first pc rows (pre: the right factor lives in q dimensions;
+    // its left factor Q_h J is orthonormal in every column already)
+    const int pc = pre ? q : p;
     for (int jj = 0; jj < q; jj++) {
         const int j = perm[jj];
         if (good[j]) continue;
-        if (p <= 0) break;
+        if (pc <= 0) break;
         double* xj = X + (long)j * ldx;
-        long pcap = mdot_part_size(p, q);
+        long pcap = mdot_part_size(pc, q);
         c->part.ensure(pcap * 8);
         c->pack.ensure((size_t)2 * q * 8 + 256);
         double* h = c->pack.as<double>();
         double* hc = h + q + 16;
-        for (int tries = 0; tries < p && !good[j]; tries++) {
-            TN_HIP(set_unit_col(xj, p, (jj + tries) % p, s));
+        for (int tries = 0; tries < pc && !good[j]; tries++) {
+            TN_HIP(set_unit_col(xj, pc, (jj + tries) % pc, s));
             for (int pass = 0; pass < 2; pass++) {
                 std::vector<double> coef(q);
-                TN_HIP(mdot(X, ldx, q, xj, p, h, c->part.as<double>(), (int)pcap, s));
+                TN_HIP(mdot(X, ldx, q, xj, pc, h, c->part.as<double>(), (int)pcap, s));
                 TN_HIP(hipMemcpyAsync(coef.data(), h, q * 8, hipMemcpyDeviceToHost, s));
                 sync(c);
                 for (int t = 0; t < q; t++)
                     if (!good[t] || t == j) coef[t] = 0.0;
                 h2d(c, hc, coef.data(), q * 8);
-                TN_HIP(gemv_acc(X, ldx, q, hc, -1.0, 1.0, xj, p, s));
+                TN_HIP(gemv_acc(X, ldx, q, hc, -1.0, 1.0, xj, pc, s));
             }
             double nr2 = 0;
-            TN_HIP(mdot(xj, p, 1, xj, p, h, c->part.as<double>(), (int)pcap, s));
+            TN_HIP(mdot(xj, pc, 1, xj, pc, h, c->part.as<double>(), (int)pcap, s));
             TN_HIP(hipMemcpyAsync(&nr2, h, 8, hipMemcpyDeviceToHost, s));
             sync(c);
             if (nr2 > 0.25) {
                 double f = 1.0 / std::sqrt(nr2);
                 std::vector<double> one(1, f);
                 h2d(c, hc, one.data(), 8);
-                TN_HIP(scale_rows(xj, p, 1, p, hc, s));
+                TN_HIP(scale_rows(xj, pc, 1, pc, hc, s));
                 sync(c);
                 good[j] = 1;
             }
@@ -1046,12 +1183,50 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     std::vector<double> onev(k, 1.0);
     h2d(c, dperm, perm.data(), k * sizeof(int));
     h2d(c, ones, onev.data(), k * 8);
+    // the factor with p-long columns (L, ld ldl) and the one with q-long
+    // columns (Rt, ld qp): plain X and J, preconditioned Q_h J and X
+    const double* Lf = X;
+    long ldl = ldx;
+    const double* Rf = V;
+    if (pre) {
+        double* QJ = c->svd_H.as<double>();
+        GemmDesc g{};
+        g.tin = T_F64; g.tout = T_F64; g.batch = 1; g.alpha = 1.0f;
+        // J has drifted from orthogonality by the rounding of its rotations
+        // (~1e-13 after ten sweeps at q = 512), which Q_h J would inherit: one
+        // Newton-Schulz step J <- J + J (I - J^T J) / 2 squares the drift away.
+        // E = I - J^T J in the R buffer, the polished J through QJ's buffer.
+        double* E = c->svd_R.as<double>();
+        TN_HIP(fill_zero(E, (long)qp * qp * 8, s));
+        TN_HIP(set_identity(E, qp, qp, s));
+        g.a_kmajor = true; g.b_kmajor = true;
+        g.M = qp; g.N = qp; g.K = qp;
+        g.A = V; g.lda = qp; g.B = V; g.ldb = qp; g.C = E; g.ldc = qp;
+        g.alpha = -1.0f; g.accumulate = 1;
+        TN_HIP(gemm(g, s));
+        // (J + J E / 2)^T = J^T + E J^T / 2 (E symmetric), row-major
+        TN_HIP(hipMemcpyAsync(QJ, V, (size_t)qp * qp * 8, hipMemcpyDeviceToDevice, s));
+        g.a_kmajor = true; g.b_kmajor = false;
+        g.A = E; g.lda = qp; g.B = V; g.ldb = qp; g.C = QJ; g.ldc = qp;
+        g.alpha = 0.5f; g.accumulate = 1;
+        TN_HIP(gemm(g, s));
+        TN_HIP(hipMemcpyAsync(V, QJ, (size_t)qp * qp * 8, hipMemcpyDeviceToDevice, s));
+        g.alpha = 1.0f; g.accumulate = 0;
+        // Q_h J into the reflector buffer: (Q_h J)^T = J^T Q_h^T, row-major
+        g.a_kmajor = true; g.b_kmajor = false;
+        g.M = qp; g.N = pp; g.K = qp;
+        g.A = V; g.lda = qp; g.B = Qh; g.ldb = pp; g.C = QJ; g.ldc = pp;
+        TN_HIP(gemm(g, s));
+        Lf = QJ;
+        ldl = pp;
+        Rf = X;
+    }
     if (tall) {
-        // U (r x k) from X, Vt (k x c) from V
-        TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, U, ldu, 1, Vt, ldvt, 1, s));
+        // U (r x k) from the p-side factor, Vt (k x c) from the q-side one
+        TN_HIP(svd_gather(Lf, ldl, p, Rf, qp, q, dperm, ones, k, U, ldu, 1, Vt, ldvt, 1, s));
     } else {
-        // Vt (k x c) from X (columns of length c), U (r x k) from V
-        TN_HIP(svd_gather(X, ldx, p, V, qp, q, dperm, ones, k, Vt, 1, ldvt, U, 1, ldu, s));
+        // Vt (k x c) from the p-side factor (columns of length c), U (r x k) from the q-side one
+        TN_HIP(svd_gather(Lf, ldl, p, Rf, qp, q, dperm, ones, k, Vt, 1, ldvt, U, 1, ldu, s));
     }
     s_out.resize(k);
     for (int j = 0; j < k; j++) s_out[j] = sv[perm[j]] * (amax > 0.0 ? amax : 1.0);

@@ -183,6 +183,19 @@ hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s);
 // each pair's 32 x 32 Gram matrix gets at most `inner` sweeps of rotations
 hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv,
                        const int* pairs, int npairs, double tol, int* flag, hipStream_t s, int inner);
+// Householder QR of the 32-column panel c0..c0+31 of the column-major A (pp
+// rows, ld lda), in place: R on and above the diagonal, zeros below; the
+// vectors in columns 0..31 of V (rows c0..pp-1, ld ldv) and the 32 x 32
+// upper-triangular T (row-major) of the panel's block reflector I - V T V^T.
+hipError_t hqr_panel(double* A, long lda, int pp, int c0, double* V, long ldv, double* T, hipStream_t s);
+// m[i] = max_{j != i, j < n} |G_ij| / sqrt(G_ii G_jj) (G row-major, ld ldg; pairs
+// with G_ii G_jj <= 1e-300 skipped): the largest cosine between columns
+hipError_t max_cos(const double* G, long ldg, int n, double* m, hipStream_t s);
+// out[j*ldo + i] = in[perm[j]*ldi + i], i < rows, j < ncols (perm: device, < the input's columns)
+hipError_t gather_cols(const double* in, long ldi, double* out, long ldo, const int* perm, int rows, int ncols,
+                       hipStream_t s);
+// X[j*ldx + perm[i]] = R[i*ldr + j], i, j < n (perm: device, a permutation of 0..n-1)
+hipError_t tr_scatter(const double* R, long ldr, double* X, long ldx, const int* perm, int n, hipStream_t s);
 hipError_t col_norms(const double* X, long ldx, int p, int q, double* s, hipStream_t st);
 hipError_t set_int(int* p, int v, hipStream_t s);
 hipError_t set_identity(double* V, long ldv, int n, hipStream_t s);

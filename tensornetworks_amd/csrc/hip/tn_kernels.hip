@@ -2082,6 +2082,213 @@ hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv, 
     return hipGetLastError();
 }
 
+// Householder QR of one 32-column panel (the column loop of LAPACK dgeqr2
+// with the forward, columnwise T of dlarft).  A is column-major (column j at
+// A + j*lda) with pp rows; the panel is columns c0..c0+31, rows c0..pp-1.
+// Column c = c0 + j has its diagonal in row r = c0 + j: H_j = I - tau v v^T,
+// v_r = 1, maps rows r.. of the column to beta e_r, so R(r, c) = beta and the
+// rows below become zero (a column whose part below the diagonal is zero, or
+// underflows to zero when squared, gets tau = 0).  v goes to column j of V
+// (rows c0..pp-1, zero above r) and the upper-triangular T (32 x 32,
+// row-major) satisfies H_0 H_1 ... H_31 = I - V T V^T.
+//
+// One workgroup.  Per column three phases, a barrier after each: (a) all 256
+// threads form v and the reflected column; (b) the dot products of v with the
+// 32 panel columns (the earlier v's for k < j, for T): 8 groups of 32 lanes,
+// group g owns columns 4g..4g+3 and each lane every 32nd row, so a thread
+// keeps 4 sums and its loads are all in flight together, and the 32 lanes
+// reduce by shuffles; (c) H_j on the columns right of j (same layout; the
+// others are stored back unchanged so the loads stay unconditional), the next
+// column's squared norm below its diagonal, and column j of T.
+__global__ void __launch_bounds__(256) hqr_panel_k(double* __restrict__ A, long lda, int pp, int c0,
+                                                    double* __restrict__ V, long ldv, double* __restrict__ T)
+{
+    __shared__ double fin[34];  // [0, 32): dot products; [32]: norm^2 below the diagonal; [33]: diagonal
+    __shared__ double Ts[32][33];
+    __shared__ double red[4];
+    const int tid = threadIdx.x;
+    const int g = tid >> 5, rl = tid & 31;
+    for (int e = tid; e < 32 * 33; e += 256) Ts[e / 33][e % 33] = 0.0;
+    {
+        const double* col = A + (long)c0 * lda;
+        double s2 = 0.0;
+        for (int i = c0 + 1 + tid; i < pp; i += 256) s2 += col[i] * col[i];
+        s2 = wave_sum(s2);
+        if ((tid & 63) == 0) red[tid >> 6] = s2;
+        if (tid == 0) fin[33] = col[c0];
+    }
+    __syncthreads();
+    if (tid == 0) fin[32] = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    for (int j = 0; j < 32; j++) {
+        const int r = c0 + j;
+        double* col = A + (long)r * lda;
+        const double alpha = fin[33], sig2 = fin[32];
+        double tau = 0.0, beta = alpha, scl = 0.0;
+        if (sig2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + sig2), alpha);
+            tau = (beta - alpha) / beta;
+            scl = 1.0 / (alpha - beta);
+        }
+        // (a) v and the reflected column
+        double* vj = V + (long)j * ldv;
+        for (int i = c0 + tid; i < pp; i += 256) {
+            double vi = 0.0;
+            if (i >= r) {
+                vi = i == r ? 1.0 : col[i] * scl;
+                col[i] = i == r ? beta : 0.0;
+            }
+            vj[i] = vi;
+        }
+        __syncthreads();
+        // (b) d_k = src_k . v over rows >= r (src_k = v_k for k < j, column
+        // c0 + k otherwise; d_j is not used)
+        const double* src[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int k = 4 * g + e;
+            src[e] = k < j ? V + (long)k * ldv : A + (long)(c0 + k) * lda;
+        }
+        const int i0 = c0 + rl + 32 * ((r - c0) >> 5);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int i = i0; i < pp; i += 32) {
+            const double vi = i >= r ? vj[i] : 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) acc[e] += src[e][i] * vi;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+#pragma unroll
+            for (int o = 16; o >= 1; o >>= 1) acc[e] += __shfl_xor(acc[e], o, 32);
+        }
+        if (rl == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) fin[4 * g + e] = acc[e];
+        }
+        __syncthreads();
+        // (c) H_j on the columns right of j; the next column's norm; T
+        double f[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) f[e] = 4 * g + e > j ? tau * fin[4 * g + e] : 0.0;
+        double s2 = 0.0;
+        const int kn = j + 1;
+        const bool own_next = kn < 32 && g == (kn >> 2);
+        // every lane has the same number of rows (pp - c0 is a multiple of
+        // 32); rows above r keep their values (v is zero there).  Blocks of
+        // four rows: all loads, then the updates, then all stores, since a
+        // load cannot move above a store the compiler cannot tell apart.
+        const int nrow = ((pp - c0) >> 5) - ((r - c0) >> 5);
+        auto rows = [&](int t, int nu) {
+            double x[4][4], w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (u >= nu) break;
+                const int i = i0 + 32 * (t + u);
+                w[u] = i >= r ? vj[i] : 0.0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) x[u][e] = A[(long)(c0 + 4 * g + e) * lda + i];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (u >= nu) break;
+                const int i = i0 + 32 * (t + u);
+                double y = 0.0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    x[u][e] -= f[e] * w[u];
+                    A[(long)(c0 + 4 * g + e) * lda + i] = x[u][e];
+                    y = 4 * g + e == kn ? x[u][e] : y;
+                }
+                s2 += i > r + 1 ? y * y : 0.0;
+                if (own_next && i == r + 1) fin[33] = y;
+            }
+        };
+        int t = 0;
+        for (; t + 4 <= nrow; t += 4) rows(t, 4);
+        for (; t < nrow; t++) rows(t, 1);
+        if (own_next) {
+#pragma unroll
+            for (int o = 16; o >= 1; o >>= 1) s2 += __shfl_xor(s2, o, 32);
+            if (rl == 0) fin[32] = s2;
+        }
+        if (tid < j) {
+            double t = 0.0;
+            for (int l = tid; l < j; l++) t += Ts[tid][l] * fin[l];
+            Ts[tid][j] = -tau * t;
+        } else if (tid == j) {
+            Ts[j][j] = tau;
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 32 * 32; e += 256) T[e] = Ts[e >> 5][e & 31];
+}
+hipError_t hqr_panel(double* A, long lda, int pp, int c0, double* V, long ldv, double* T, hipStream_t s)
+{
+    if (c0 < 0 || c0 + 32 > pp || lda < pp || ldv < pp) return hipErrorInvalidValue;
+    hqr_panel_k<<<1, 256, 0, s>>>(A, lda, pp, c0, V, ldv, T);
+    return hipGetLastError();
+}
+
+// m[i] = max over j != i (j < n) of |G_ij| / sqrt(G_ii G_jj), the largest
+// cosine between column i and another column of the matrix whose Gram matrix
+// G is (row-major, ld ldg); pairs with G_ii G_jj <= 1e-300 are skipped.
+// One workgroup per row.
+__global__ void __launch_bounds__(256) max_cos_k(const double* __restrict__ G, long ldg, int n,
+                                                  double* __restrict__ m)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    const double gi = G[(long)i * ldg + i];
+    double best = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        if (j == i) continue;
+        const double d = gi * G[(long)j * ldg + j];
+        if (d > 1e-300) best = fmax(best, fabs(G[(long)i * ldg + j]) / sqrt(d));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) best = fmax(best, __shfl_xor(best, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) m[i] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+hipError_t max_cos(const double* G, long ldg, int n, double* m, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    max_cos_k<<<n, 256, 0, s>>>(G, ldg, n, m);
+    return hipGetLastError();
+}
+
+// out[j*ldo + i] = in[perm[j]*ldi + i] (i < rows, j < ncols): a column gather
+__global__ void gather_cols_k(const double* __restrict__ in, long ldi, double* __restrict__ out, long ldo,
+                              const int* __restrict__ perm, int rows)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < rows) out[(long)j * ldo + i] = in[(long)perm[j] * ldi + i];
+}
+hipError_t gather_cols(const double* in, long ldi, double* out, long ldo, const int* perm, int rows, int ncols,
+                       hipStream_t s)
+{
+    if (rows <= 0 || ncols <= 0) return hipSuccess;
+    gather_cols_k<<<dim3((rows + 255) / 256, ncols), 256, 0, s>>>(in, ldi, out, ldo, perm, rows);
+    return hipGetLastError();
+}
+// X[j*ldx + perm[i]] = R[i*ldr + j] (i, j < n): R^T with its rows permuted
+__global__ void tr_scatter_k(const double* __restrict__ R, long ldr, double* __restrict__ X, long ldx,
+                             const int* __restrict__ perm, int n)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j < n) X[(long)j * ldx + perm[i]] = R[(long)i * ldr + j];
+}
+hipError_t tr_scatter(const double* R, long ldr, double* X, long ldx, const int* perm, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    tr_scatter_k<<<dim3((n + 255) / 256, n), 256, 0, s>>>(R, ldr, X, ldx, perm, n);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) col_norms_k(const double* __restrict__ X, long ldx, int p,
                                                    double* __restrict__ s)
 {

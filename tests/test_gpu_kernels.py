@@ -222,6 +222,47 @@ def test_svd_matches_numpy(ctx, m, n, rank):
     assert np.all(np.diff(s) <= 1e-15 * s_np[0])
 
 
+@pytest.mark.parametrize("m,n,decay,rank", [(512, 512, 25.0, 512), (300, 160, 10.0, 160), (160, 300, 1e9, 160),
+                                            (256, 256, 8.0, 200), (520, 200, 30.0, 200), (129, 700, 5.0, 129)])
+def test_svd_graded_spectrum(ctx, m, n, decay, rank):
+    """min(m, n) >= 128 takes the QR-preconditioned Jacobi.  Singular values
+    exp(-j/decay) down to ~1e-28 (a two-site split's spectrum decays like this),
+    exact zeros past the rank, tall and wide: the values must equal the
+    constructed ones and both factors stay orthonormal."""
+    rng = np.random.default_rng(60 + m + n)
+    k = min(m, n)
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, k)))
+    sig = np.exp(-np.arange(k) / decay)
+    sig[rank:] = 0.0
+    A = (Q1 * sig) @ Q2.T
+    U, s, Vt = ctx.svd(A)
+    assert np.abs(s - sig).max() < 1e-12
+    assert np.abs(U @ np.diag(s) @ Vt - A).max() < 1e-12
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
+    assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12
+    assert np.all(np.diff(s) <= 1e-15)
+
+
+@pytest.mark.parametrize("m,n", [(300, 200), (256, 512)])
+def test_svd_nearly_orthogonal_columns(ctx, m, n):
+    """Columns (rows, when wide) already nearly orthogonal, the shape of a
+    one-site gauge move's site tensor: the plain iteration runs on them.
+    Pinned against numpy's singular values and by orthonormality."""
+    rng = np.random.default_rng(70 + m)
+    k = min(m, n)
+    Q1, _ = np.linalg.qr(rng.standard_normal((max(m, n), k)))
+    sig = np.exp(-np.arange(k) / 20.0)
+    X = (Q1 * sig) @ (np.eye(k) + 1e-3 * rng.standard_normal((k, k)))
+    A = X if m >= n else X.T
+    U, s, Vt = ctx.svd(A)
+    s_np = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(s - s_np).max() < 1e-12
+    assert np.abs(U @ np.diag(s) @ Vt - A).max() < 1e-12
+    assert np.abs(U.T @ U - np.eye(k)).max() < 1e-12
+    assert np.abs(Vt @ Vt.T - np.eye(k)).max() < 1e-12
+
+
 def _qr_positive(A):
     """Householder QR with R's diagonal made positive: the unique QR of a full
     column rank A, which every correct QR must reproduce."""
