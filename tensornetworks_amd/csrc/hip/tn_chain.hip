@@ -778,7 +778,7 @@ int tn_ctx_destroy(tn_ctx* c)
                         &c->qr_T, &c->qr_linv, &c->qr_Q, &c->tune_C, &c->splitk, &c->b_L, &c->b_R,
                         &c->b_Llo, &c->b_Rlo, &c->b_x, &c->b_y, &c->b_M, &c->dav_W, &c->dav_vec,
                         &c->dav_wd, &c->mlp, &c->ident, &c->defl, &c->svd_R, &c->svd_Q, &c->svd_H,
-                        &c->svd_W};
+                        &c->svd_W, &c->svd_X2};
     for (auto* b : bufs) b->release();
     c->b_mix_heff.buf.release();
     c->b_mix_envl.buf.release();

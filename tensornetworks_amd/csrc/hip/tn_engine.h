@@ -87,7 +87,7 @@ struct tn_ctx {
     tn::DBuf T1, T2, xlo, part, krylov, lanczos_small, svd_X, svd_V, svd_U, svd_Vt, svd_s,
         svd_perm, svd_pairs, svd_flag, svd_A, spare, pack, qr_W, qr_G, qr_L, qr_T, qr_linv, qr_Q;
     // preconditioned SVD: R, Q_h, reflectors (then Q_h J), T factors + GEMM work space
-    tn::DBuf svd_R, svd_Q, svd_H, svd_W;
+    tn::DBuf svd_R, svd_Q, svd_H, svd_W, svd_X2;
     tn::Scalars sc;
     hipEvent_t ev[8] = {};
     // GEMM autotuning (off unless requested): best (tile code, k-split) per

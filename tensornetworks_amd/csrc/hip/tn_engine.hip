@@ -1097,20 +1097,29 @@ int svd_device(tn_ctx* c, const double* A, long rs, long cs, int r, int cc, doub
     const double tol = std::max(1e-15, 4.4e-16 * std::sqrt((double)pj));
     int hflag = 1;
     // one sweep of rotations on each pair's Gram matrix per visit: the pairs
-    // come back every outer sweep, and a launch has only nb/2 workgroups, so
-    // its latency (mostly the inner rounds) sets the cost.  Measured on the
+    // come back every outer sweep, and a round has only nb/2 pairs, so its
+    // latency (mostly the inner rounds) sets the cost.  Measured on the
     // 512 x 512 two-site splits at D=256: one inner sweep halves the SVD time
     // against diagonalising each pair (up to 20 inner sweeps), with more
     // outer sweeps and the same singular values.
     const int inner = 1;
+    // workgroups per pair, each rotating its own rows (about 256 in all, at
+    // least 32 rows each); the iterate alternates between X and X2
+    const int nchunk = std::max(1, std::min({16, 256 / std::max(1, nb / 2), std::max(1, pj / 32)}));
+    c->svd_X2.ensure((size_t)ldx * qp * 8);
+    double* Xa = X;
+    double* Xb = c->svd_X2.as<double>();
     for (int sweep = 0; sweep < 80 && hflag; sweep++) {
         TN_HIP(set_int(flag, 0, s));
-        for (int rd = 0; rd < nb - 1; rd++)
-            TN_HIP(jacobi_step(X, ldx, pj, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb,
-                               nb / 2, tol, flag, s, inner));
+        for (int rd = 0; rd < nb - 1; rd++) {
+            TN_HIP(jacobi_step(Xa, Xb, ldx, pj, V, qp, qp, c->svd_pairs.as<int>() + (size_t)rd * nb, nb / 2,
+                               nchunk, tol, flag, s, inner));
+            std::swap(Xa, Xb);
+        }
         TN_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         sync(c);
     }
+    X = Xa;
     // singular values = column norms; sort descending
     double* sd = c->svd_s.as<double>();
     TN_HIP(col_norms(X, ldx, pj, q, sd, s));

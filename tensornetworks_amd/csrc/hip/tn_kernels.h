@@ -179,10 +179,13 @@ hipError_t dcopy_scalar(const double* src, double* dst, hipStream_t s);
 // dst[0] = sqrt(src[0])
 hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s);
 
-// one-sided block Jacobi SVD pieces (fp64, column-major X with ldx, q%32==0);
-// each pair's 32 x 32 Gram matrix gets at most `inner` sweeps of rotations
-hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv,
-                       const int* pairs, int npairs, double tol, int* flag, hipStream_t s, int inner);
+// one-sided block Jacobi SVD round (fp64, column-major iterate with ldx,
+// q%32==0): the round's pairs rotate Xs into Xd (every column of the round's
+// pairs written, rotated or copied; Xs != Xd) and V in place, nchunk
+// workgroups per pair, each its own chunk of rows; each pair's 32 x 32 Gram
+// matrix gets at most `inner` sweeps of rotations
+hipError_t jacobi_step(const double* Xs, double* Xd, long ldx, int p, double* V, long ldv, int qv,
+                       const int* pairs, int npairs, int nchunk, double tol, int* flag, hipStream_t s, int inner);
 // Householder QR of the 32-column panel c0..c0+31 of the column-major A (pp
 // rows, ld lda), in place: R on and above the diagonal, zeros below; the
 // vectors in columns 0..31 of V (rows c0..pp-1, ld ldv) and the 32 x 32

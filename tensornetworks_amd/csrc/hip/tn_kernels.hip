@@ -1903,15 +1903,21 @@ hipError_t dsqrt_scalar(const double* src, double* dst, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
-// one-sided block Jacobi SVD: one workgroup per column-block pair (2 x 16
-// columns).  The 32x32 Gram matrix is diagonalised in LDS by cyclic parallel
-// Jacobi and the rotation applied to the 32 columns of X and of V.
+// one-sided block Jacobi SVD: gridDim.y workgroups per column-block pair (2 x
+// 16 columns).  Each forms the pair's 32x32 Gram matrix from the source
+// iterate Xs, diagonalises it in LDS by cyclic parallel Jacobi and applies the
+// rotation to its own chunk of rows: Xs -> Xd, and V in place.  The Gram
+// matrix and the rotation are computed the same way by every workgroup of the
+// pair (same data, same order: bit-identical), so the q/32 pairs of a round
+// spread over up to 16 times as many CUs; reading Xs and writing Xd keeps a
+// workgroup from seeing rows another has already rotated.  A pair that is
+// already orthogonal copies its rows to Xd.
 #define JB 16
 #define JW 32
 #define JR 128  // rows per Gram chunk
 
-__global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx, int p,
-                                                double* __restrict__ V, long ldv, int qv,
+__global__ void __launch_bounds__(256) jacobi_k(const double* __restrict__ Xs, double* __restrict__ Xd, long ldx,
+                                                int p, double* __restrict__ V, long ldv, int qv,
                                                 const int* __restrict__ pairs, double tol,
                                                 int* __restrict__ flag, int inner)
 {
@@ -1940,7 +1946,7 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
             const int idx = tid + 256 * u;
             const int row = idx % JR, c = idx / JR;
             const int gr = r0 + row;
-            ld[u] = (gr < p) ? X[(long)colidx(c) * ldx + gr] : 0.0;
+            ld[u] = (gr < p) ? Xs[(long)colidx(c) * ldx + gr] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < JR * JW / 256; u++) {
@@ -1981,7 +1987,18 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
         }
     }
     __syncthreads();
-    if (sh_off == 0) return;  // already orthogonal: nothing to rotate
+    // this workgroup's rows of the iterate and of V
+    const int nch = gridDim.y, ch = blockIdx.y;
+    const int xrows = (p + nch - 1) / nch, vrows = (qv + nch - 1) / nch;
+    const int xr0 = ch * xrows, xr1 = min(p, xr0 + xrows);
+    const int vr0 = ch * vrows, vr1 = min(qv, vr0 + vrows);
+    if (sh_off == 0) {  // already orthogonal: nothing to rotate
+        for (int e = tid; e < JW * (xr1 > xr0 ? xr1 - xr0 : 0); e += 256) {
+            const long off = (long)colidx(e / (xr1 - xr0)) * ldx + xr0 + e % (xr1 - xr0);
+            Xd[off] = Xs[off];
+        }
+        return;
+    }
     if (tid == 0) flag[0] = 1;
 
     // 2. cyclic parallel Jacobi on G (tournament ordering, 31 rounds per sweep)
@@ -2008,24 +2025,26 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
                 rq[tid] = Q;
             }
             __syncthreads();
-            // rows: G <- R^T G
-            for (int e = tid; e < (JW / 2) * JW; e += 256) {
-                const int pi = e / JW, col = e % JW;
-                const int P = rp[pi], Q = rq[pi];
-                const double c = rc[pi], s = rs[pi];
-                const double gp = G[P][col], gq = G[Q][col];
-                G[P][col] = c * gp - s * gq;
-                G[Q][col] = s * gp + c * gq;
+            // G <- R^T G R in one phase: the round's pairs are disjoint, so G
+            // splits into 16 x 16 independent 2 x 2 blocks (rows P1, Q1 of
+            // pair bi, columns P2, Q2 of pair bj), one per thread
+            {
+                const int bi = tid >> 4, bj = tid & 15;
+                const int P1 = rp[bi], Q1 = rq[bi], P2 = rp[bj], Q2 = rq[bj];
+                const double c1 = rc[bi], s1 = rs[bi], c2 = rc[bj], s2 = rs[bj];
+                const double g00 = G[P1][P2], g01 = G[P1][Q2], g10 = G[Q1][P2], g11 = G[Q1][Q2];
+                const double h00 = c1 * g00 - s1 * g10, h01 = c1 * g01 - s1 * g11;
+                const double h10 = s1 * g00 + c1 * g10, h11 = s1 * g01 + c1 * g11;
+                G[P1][P2] = c2 * h00 - s2 * h01;
+                G[P1][Q2] = s2 * h00 + c2 * h01;
+                G[Q1][P2] = c2 * h10 - s2 * h11;
+                G[Q1][Q2] = s2 * h10 + c2 * h11;
             }
-            __syncthreads();
-            // columns: G <- G R, J <- J R
+            // J <- J R
             for (int e = tid; e < (JW / 2) * JW; e += 256) {
                 const int pi = e / JW, row = e % JW;
                 const int P = rp[pi], Q = rq[pi];
                 const double c = rc[pi], s = rs[pi];
-                const double gp = G[row][P], gq = G[row][Q];
-                G[row][P] = c * gp - s * gq;
-                G[row][Q] = s * gp + c * gq;
                 const double jp = J[row][P], jq = J[row][Q];
                 J[row][P] = c * jp - s * jq;
                 J[row][Q] = s * jp + c * jq;
@@ -2038,10 +2057,10 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
 
     // 3. X[:, cols] <- X[:, cols] J ; V[:, cols] <- V[:, cols] J (J read in
     // 16-byte column pairs, the same for every lane)
-    for (int row = tid; row < p; row += 256) {
+    for (int row = xr0 + tid; row < xr1; row += 256) {
         double x[JW];
 #pragma unroll
-        for (int c = 0; c < JW; c++) x[c] = X[(long)colidx(c) * ldx + row];
+        for (int c = 0; c < JW; c++) x[c] = Xs[(long)colidx(c) * ldx + row];
 #pragma unroll 2
         for (int c2 = 0; c2 < JW; c2 += 2) {
             double s0 = 0.0, s1 = 0.0;
@@ -2051,11 +2070,11 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
                 s0 = fma(x[c], jj.x, s0);
                 s1 = fma(x[c], jj.y, s1);
             }
-            X[(long)colidx(c2) * ldx + row] = s0;
-            X[(long)colidx(c2 + 1) * ldx + row] = s1;
+            Xd[(long)colidx(c2) * ldx + row] = s0;
+            Xd[(long)colidx(c2 + 1) * ldx + row] = s1;
         }
     }
-    for (int row = tid; row < qv; row += 256) {
+    for (int row = vr0 + tid; row < vr1; row += 256) {
         double x[JW];
 #pragma unroll
         for (int c = 0; c < JW; c++) x[c] = V[(long)colidx(c) * ldv + row];
@@ -2074,11 +2093,12 @@ __global__ void __launch_bounds__(256) jacobi_k(double* __restrict__ X, long ldx
     }
 }
 
-hipError_t jacobi_step(double* X, long ldx, int p, double* V, long ldv, int qv, const int* pairs,
-                       int npairs, double tol, int* flag, hipStream_t s, int inner)
+hipError_t jacobi_step(const double* Xs, double* Xd, long ldx, int p, double* V, long ldv, int qv,
+                       const int* pairs, int npairs, int nchunk, double tol, int* flag, hipStream_t s, int inner)
 {
     if (npairs <= 0) return hipSuccess;
-    jacobi_k<<<npairs, 256, 0, s>>>(X, ldx, p, V, ldv, qv, pairs, tol, flag, inner);
+    if (nchunk < 1 || nchunk > 64 || Xs == Xd) return hipErrorInvalidValue;
+    jacobi_k<<<dim3(npairs, nchunk), 256, 0, s>>>(Xs, Xd, ldx, p, V, ldv, qv, pairs, tol, flag, inner);
     return hipGetLastError();
 }
 
