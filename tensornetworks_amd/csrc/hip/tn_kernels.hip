@@ -1934,11 +1934,13 @@ __global__ void __launch_bounds__(256) jacobi_k(const double* __restrict__ Xs, d
     auto colidx = [&](int j) { return j < JB ? pb * JB + j : qb * JB + (j - JB); };
 
     // 1. Gram matrix over row chunks of JR rows staged in LDS (all loads of a
-    // chunk in flight together: few load-then-barrier rounds per launch).
-    // Thread (bi, bj) forms the 2 x 2 block G[2bi..2bi+1][2bj..2bj+1] from
-    // two 16-byte LDS reads per row.
-    const int bi = tid >> 4, bj = tid & 15;
-    double g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0;
+    // chunk in flight together: few load-then-barrier rounds per launch), on
+    // the FP64 matrix cores: wave w forms the 16 x 16 block (w >> 1, w & 1)
+    // of G, one v_mfma_f64_16x16x4 per 4 rows (lane l reads row l/16 of the
+    // step and column l%16 of each of its two column blocks).
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wi = wv >> 1, wj = wv & 1;
+    v4d gacc = {0.0, 0.0, 0.0, 0.0};
     for (int r0 = 0; r0 < p; r0 += JR) {
         double ld[JR * JW / 256];
 #pragma unroll
@@ -1955,20 +1957,15 @@ __global__ void __launch_bounds__(256) jacobi_k(const double* __restrict__ Xs, d
         }
         __syncthreads();
 #pragma unroll 8
-        for (int rr = 0; rr < JR; rr++) {
-            const double2 a = *(const double2*)&tile[rr][2 * bi];
-            const double2 b = *(const double2*)&tile[rr][2 * bj];
-            g00 = fma(a.x, b.x, g00);
-            g01 = fma(a.x, b.y, g01);
-            g10 = fma(a.y, b.x, g10);
-            g11 = fma(a.y, b.y, g11);
+        for (int k0 = 0; k0 < JR; k0 += 4) {
+            const double a = tile[k0 + (lane >> 4)][16 * wi + (lane & 15)];
+            const double b = tile[k0 + (lane >> 4)][16 * wj + (lane & 15)];
+            gacc = mma(a, b, gacc);
         }
         __syncthreads();
     }
-    G[2 * bi][2 * bj] = g00;
-    G[2 * bi][2 * bj + 1] = g01;
-    G[2 * bi + 1][2 * bj] = g10;
-    G[2 * bi + 1][2 * bj + 1] = g11;
+#pragma unroll
+    for (int r = 0; r < 4; r++) G[16 * wi + (lane >> 4) + 4 * r][16 * wj + (lane & 15)] = gacc[r];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         const int e = tid + 256 * u;
