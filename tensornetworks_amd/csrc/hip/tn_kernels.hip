@@ -3011,32 +3011,63 @@ hipError_t bconvert(const ConvJob* jobs, int n, long nmax, hipStream_t s)
     return hipGetLastError();
 }
 
-// MPO mixing (FP32 in, TO out) for many problems: grid (Bn blocks, X, problem)
+// MPO mixing (FP32 in, TO out) for many problems: grid (Bn blocks, X, problem).
+// The workgroup's mixing table (one problem's) is staged in LDS first when it
+// fits: read from global memory, every entry would be a chain of dependent
+// loads (offset, value, selector, then the data) per output.
+#define BW_MAXO 64
+#define BW_MAXE 512
 template <typename TO>
 __global__ void __launch_bounds__(256) bwmix_k(const MixJob* __restrict__ jobs)
 {
+    __shared__ int s_optr[BW_MAXO + 1];
+    __shared__ long s_ooff[BW_MAXO];
+    __shared__ int s_osel[BW_MAXO];
+    __shared__ long s_eoff[BW_MAXE];
+    __shared__ float s_eval[BW_MAXE];
+    __shared__ int s_esel[BW_MAXE];
     const MixJob& j = jobs[blockIdx.z];
-    const int b = blockIdx.x * 256 + threadIdx.x;
     const long x = blockIdx.y;
-    if (b >= j.Bn || x >= j.X) return;
+    if (x >= j.X) return;  // the whole workgroup
+    const MixTable& t = j.t;
+    const int nout = t.nout, nnz = t.nnz;
+    const bool idt = t.esel != nullptr;
+    const bool staged = nout <= BW_MAXO && nnz <= BW_MAXE;
+    if (staged) {
+        for (int i = threadIdx.x; i <= nout; i += 256) s_optr[i] = t.optr[i];
+        for (int i = threadIdx.x; i < nout; i += 256) {
+            s_ooff[i] = t.ooff[i];
+            s_osel[i] = idt ? t.osel[i] : 0;
+        }
+        for (int i = threadIdx.x; i < nnz; i += 256) {
+            s_eoff[i] = t.eoff[i];
+            s_eval[i] = t.evalf[i];
+            s_esel[i] = idt ? t.esel[i] : 0;
+        }
+        __syncthreads();
+    }
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= j.Bn) return;
     const float* ip = (const float*)j.in + x * j.isx + b;
     TO* op = (TO*)j.out + x * j.osx + b;
-    if (j.t.esel) {
-        const float* ip2 = j.in2 + x * j.isx2 + b;
-        for (int o = 0; o < j.t.nout; o++) {
-            const int e0 = j.t.optr[o], e1 = j.t.optr[o + 1];
+    const float* ip2 = idt ? j.in2 + x * j.isx2 + b : nullptr;
+    if (staged) {
+        for (int o = 0; o < nout; o++) {
+            const int e0 = s_optr[o], e1 = s_optr[o + 1];
             float acc = 0.0f;
-            for (int e = e0; e < e1; e++) acc += j.t.evalf[e] * (j.t.esel[e] ? ip2[j.t.eoff[e]] : ip[j.t.eoff[e]]);
-            if (j.t.osel[o]) j.out2[x * j.osx2 + j.t.ooff[o] + b] = (double)acc;
-            else op[j.t.ooff[o]] = cvt_from_f<TO>(acc);
+#pragma unroll 4
+            for (int e = e0; e < e1; e++) acc += s_eval[e] * (s_esel[e] ? ip2[s_eoff[e]] : ip[s_eoff[e]]);
+            if (s_osel[o]) j.out2[x * j.osx2 + s_ooff[o] + b] = (double)acc;
+            else op[s_ooff[o]] = cvt_from_f<TO>(acc);
         }
         return;
     }
-    for (int o = 0; o < j.t.nout; o++) {
-        const int e0 = j.t.optr[o], e1 = j.t.optr[o + 1];
+    for (int o = 0; o < nout; o++) {
+        const int e0 = t.optr[o], e1 = t.optr[o + 1];
         float acc = 0.0f;
-        for (int e = e0; e < e1; e++) acc += j.t.evalf[e] * ip[j.t.eoff[e]];
-        op[j.t.ooff[o]] = cvt_from_f<TO>(acc);
+        for (int e = e0; e < e1; e++) acc += t.evalf[e] * (idt && t.esel[e] ? ip2[t.eoff[e]] : ip[t.eoff[e]]);
+        if (idt && t.osel[o]) j.out2[x * j.osx2 + t.ooff[o] + b] = (double)acc;
+        else op[t.ooff[o]] = cvt_from_f<TO>(acc);
     }
 }
 hipError_t bwmix(int tout, const MixJob* jobs, int n, int Xmax, int Bnmax, hipStream_t s)
