@@ -300,7 +300,7 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
         if (learned && !dr.converged) {
             if (st) st->n_precond_fallback++;
             LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, 0.0,
-                                      o.reorth == 1, unit_roundoff(TN_FP64));
+                                      o.reorth == 1, unit_roundoff(TN_FP64), true);
             if (st) st->n_full_reorth += r.n_full;
             *e0 = r.e0;
             *e1 = r.e1;
@@ -314,14 +314,14 @@ void eigensolve(tn_chain* ch, int i, const tn_sweep_opts& o, tn_sweep_stats* st,
     const bool refining = o.refine && unit_roundoff(o.refine_precision) < unit_roundoff(o.precision);
     const double rel = refining ? (o.precision == TN_BF16 ? 4e-3 : 4e-7) : 0.0;
     LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->A[i], o.eig_tol, o.max_krylov, o.max_restarts, rel,
-                              o.reorth == 1, unit_roundoff(o.precision));
+                              o.reorth == 1, unit_roundoff(o.precision), true);
     if (st) st->n_full_reorth += r.n_full;
     if (refining) {
         // polish the reduced-precision Ritz vector with a more accurate operator
         HeffCall hr = hc;
         hr.op = make_heff(ch, i, o.refine_precision, o.gemm_tile, wl, wr);
         LanczosResult r2 = lanczos(c, heff_cb, &hr, N, ch->A[i], o.refine_tol, o.max_krylov, o.max_restarts, 0.0,
-                                   o.reorth == 1, unit_roundoff(o.refine_precision));
+                                   o.reorth == 1, unit_roundoff(o.refine_precision), true);
         if (st) st->n_full_reorth += r2.n_full;
         r.e0 = r2.e0;
         r.e1 = r2.e1;
@@ -1923,7 +1923,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
         hc.D2 = D3;
         double tl = now_ms();
         LanczosResult r = lanczos(c, heff_cb, &hc, N, ch->theta.as<double>(), o.eig_tol, o.max_krylov, o.max_restarts,
-                                  0.0, o.reorth == 1, unit_roundoff(o.precision));
+                                  0.0, o.reorth == 1, unit_roundoff(o.precision), true);
         if (st) st->n_full_reorth += r.n_full;
         if (o.refine && o.refine_precision < o.precision) {
             HeffCall h64 = hc;
@@ -1931,7 +1931,7 @@ int tn_chain_sweep2(tn_chain* ch, int dir, int Dmax, double eps_sv, const tn_swe
             h64.op.Llo = lo_left(ch, o.refine_precision, i);
             h64.op.Rlo = lo_right(ch, o.refine_precision, j);
             LanczosResult r2 = lanczos(c, heff_cb, &h64, N, ch->theta.as<double>(), o.refine_tol, o.max_krylov,
-                                       o.max_restarts, 0.0, o.reorth == 1, unit_roundoff(o.refine_precision));
+                                       o.max_restarts, 0.0, o.reorth == 1, unit_roundoff(o.refine_precision), true);
             if (st) st->n_full_reorth += r2.n_full;
             r.e0 = r2.e0;
             r.e1 = r2.e1;

@@ -105,6 +105,10 @@ struct tn_ctx {
     void* noalloc = nullptr;
     // stats whose Heff event timings are recorded but not yet read
     tn_sweep_stats* heff_pending = nullptr;
+    // nonzero: the next heff_apply first scales its input in place by this
+    // factor (within its first pass over it) and resets it; set by the Lanczos
+    // driver right before the apply of a new basis vector
+    double x_prescale = 0.0;
     // seam state (tn_site_bind)
     int b_d = 0, b_D1 = 0, b_D2 = 0, b_Dw1 = 0, b_Dw2 = 0, b_prec = 0;
     int b_wl = -1, b_wr = -1;  // identity channels of the bound environments
@@ -205,9 +209,13 @@ typedef void (*ApplyFn)(tn_ctx* c, const void* op, const double* x, double* y);
 // estimated loss of orthogonality exceeds sqrt(eps_op), eps_op the relative
 // rounding of the operator (Simon's omega recurrence); false: the pass
 // against the whole basis every iteration
+//
+// prescale: fn applies the Heff through heff_apply, which honours
+// tn_ctx::x_prescale; the scaling v_{j+1} = w / beta_j is then left to the
+// next apply's first pass over its input instead of a separate kernel
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
                       int max_krylov, int max_restarts, double tol_rel = 0.0, bool partial = false,
-                      double eps_op = 0x1p-52);
+                      double eps_op = 0x1p-52, bool prescale = false);
 
 // --- Davidson with a diagonal / learned preconditioner (tn_precond.hip) ---
 struct DavidsonResult {

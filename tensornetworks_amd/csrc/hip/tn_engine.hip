@@ -340,6 +340,12 @@ size_t heff_scratch_bytes(const HeffOp& op, size_t* t1, size_t* t2, size_t* xlo)
 
 void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool timed, const HeffScratch* scr)
 {
+    // a pending Lanczos scaling of x (x is then the driver's own basis vector,
+    // scaled in place within the first pass over it), taken before anything
+    // can throw
+    const double pre = c->x_prescale;
+    c->x_prescale = 0.0;
+    double* const xs = pre != 0.0 ? const_cast<double*>(x) : nullptr;
     hipStream_t s = c->stream;
     const int p = op.p, D1p = op.D1p, D2p = op.D2p, Dw1 = op.Dw1, Dw2 = op.Dw2;
     const bool idl = op.wl_id >= 0, idr = op.wr_id >= 0;
@@ -406,6 +412,7 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
             T1 = c->T1.p;
             T2 = c->T2.p;
         }
+        if (xs) TN_HIP(scale_copy(pre, xs, xs, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
         g1.tin = T_F64; g1.tout = T_F64; g1.A = x; g1.B = op.R + roff; g1.C = T1;
         run(g1);
@@ -432,7 +439,8 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
             T2 = c->T2.p;
         }
         TN_REQUIRE(op.Lps > 0 && op.Rps > 0, TN_ERR_STATE, "heff_apply: split-bfloat16 operator without plane strides");
-        TN_HIP(split3(T_F64, x, XL, n, n, s));
+        if (xs) TN_HIP(split3_scale(xs, pre, XL, n, n, s));
+        else TN_HIP(split3(T_F64, x, XL, n, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
         if (fuse) {
             TN_HIP(x3_stage1_mix(p, nR, (const bf16*)XL, n, (const bf16*)op.Rlo + roff, op.Rps, x, (bf16*)T2, n2,
@@ -464,7 +472,8 @@ void heff_apply(tn_ctx* c, const HeffOp& op, const double* x, double* y, bool ti
             T1 = c->T1.p;
             T2 = c->T2.p;
         }
-        TN_HIP(convert(T_F64, x, tl, XL, n, s));
+        if (xs) TN_HIP(convert_scale(xs, pre, tl, XL, n, s));
+        else TN_HIP(convert(T_F64, x, tl, XL, n, s));
         if (timed) TN_HIP(hipEventRecord(c->ev[1], s));
         g1.tin = tl; g1.tout = T_F32; g1.A = XL; g1.B = (const char*)op.Rlo + roff * es; g1.C = T1;
         run(g1);
@@ -766,7 +775,7 @@ void tridiag_lowest(int k, const double* a, const double* b, double* t0, double*
 // sqrt(eps_op) keeps the tridiagonal matrix the projection of the operator to
 // O(eps_op |H|), the operator's own accuracy.
 LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, double tol,
-                      int max_krylov, int max_restarts, double tol_rel, bool partial, double eps_op)
+                      int max_krylov, int max_restarts, double tol_rel, bool partial, double eps_op, bool prescale)
 {
     hipStream_t s = c->stream;
     LanczosResult res;
@@ -786,6 +795,9 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
     const bool pro = partial && fused;
     const double eta = std::sqrt(eps_op);
     std::vector<double> om_old(kmax + 2, 0.0), om(kmax + 2, 0.0), om_new(kmax + 2, 0.0);
+    // the scaling of the next basis vector not launched yet (prescale)
+    double pending = 0.0;
+    c->x_prescale = 0.0;
 
     auto read = [&](double* dst, const double* src, int n) {
         if (n <= TN_PUBLISH_MAX) d2h_small(c, dst, src, n);
@@ -824,7 +836,16 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
         for (int j = 0; j < kmax; j++) {
             double* vj = V + (long)j * N;
             double* w = V + (long)(j + 1) * N;
-            fn(c, op, vj, w);
+            if (pending != 0.0) {  // v_j = w_{j-1} / beta_{j-1}, inside the apply
+                c->x_prescale = pending;
+                pending = 0.0;
+            }
+            try {
+                fn(c, op, vj, w);
+            } catch (...) {
+                c->x_prescale = 0.0;
+                throw;
+            }
             res.n_matvec++;
             // three-term step with the exact alpha = v_j.w (h[0] on the device),
             // then one classical Gram-Schmidt pass against every Krylov vector
@@ -903,7 +924,10 @@ LanczosResult lanczos(tn_ctx* c, ApplyFn fn, const void* op, long N, double* x, 
                 conv = true;
                 break;
             }
-            if (j + 1 < kmax) TN_HIP(scale_copy(1.0 / be[j], w, w, N, s));
+            if (j + 1 < kmax) {
+                if (prescale) pending = 1.0 / be[j];
+                else TN_HIP(scale_copy(1.0 / be[j], w, w, N, s));
+            }
         }
         // Ritz vector x = sum_i z_i v_i, normalised
         h2d(c, zd, z.data(), (size_t)k * 8);

@@ -129,6 +129,10 @@ hipError_t publish(const double* src, int n, double* host_dst, unsigned long lon
 hipError_t convert(int tin, const void* in, int tout, void* out, long n, hipStream_t s);
 // three bfloat16 planes of n values (T_F64 or T_F32 input): out[p*ps + i], p = 0..2
 hipError_t split3(int tin, const void* in, void* out, long n, long ps, hipStream_t s);
+// x <- a x in place (FP64) and the scaled values as three bfloat16 planes /
+// converted to tout (T_F32 or T_BF16)
+hipError_t split3_scale(double* x, double a, void* out, long n, long ps, hipStream_t s);
+hipError_t convert_scale(double* x, double a, int tout, void* out, long n, hipStream_t s);
 hipError_t fill_zero(void* p, long bytes, hipStream_t s);
 hipError_t fill_value(double* p, double v, long n, hipStream_t s);
 

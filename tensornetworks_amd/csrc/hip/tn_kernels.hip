@@ -1446,6 +1446,51 @@ hipError_t split3(int tin, const void* in, void* out, long n, long ps, hipStream
     return hipGetLastError();
 }
 
+// x <- a x in place, and the three bfloat16 planes of the scaled values
+// (the Lanczos scaling of the next basis vector fused into the apply's first
+// pass over it; a x rounds exactly as the separate scaling would)
+__global__ void split3_scale_k(double* __restrict__ x, double a, bf16* __restrict__ out, long n, long ps)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = a * x[i];
+        x[i] = v;
+        bf16 h0, h1, h2;
+        split3_val(v, h0, h1, h2);
+        out[i] = h0;
+        out[ps + i] = h1;
+        out[2 * ps + i] = h2;
+    }
+}
+hipError_t split3_scale(double* x, double a, void* out, long n, long ps, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (ps < n) return hipErrorInvalidValue;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    split3_scale_k<<<blocks, 256, 0, s>>>(x, a, (bf16*)out, n, ps);
+    return hipGetLastError();
+}
+// x <- a x in place, and the scaled values converted to TO
+template <typename TO>
+__global__ void convert_scale_k(double* __restrict__ x, double a, TO* __restrict__ out, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = a * x[i];
+        x[i] = v;
+        out[i] = cvt_from_d<TO>(v);
+    }
+}
+hipError_t convert_scale(double* x, double a, int tout, void* out, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (tout == T_F32) convert_scale_k<float><<<blocks, 256, 0, s>>>(x, a, (float*)out, n);
+    else if (tout == T_BF16) convert_scale_k<bf16><<<blocks, 256, 0, s>>>(x, a, (bf16*)out, n);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 __global__ void fill_zero_k(uint32_t* p, long n)
 {
     for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0u;
