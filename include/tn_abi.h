@@ -114,8 +114,12 @@ int tn_eig_lowest(tn_ctx* ctx, int64_t N, int nev, double tol, int maxit, double
                   double* evec, int* n_matvec);
 /* Thin SVD A = U diag(s) Vt of a column-major m x n matrix, k = min(m,n):
  * U (m x k), s (k, descending), Vt (k x n), column-major; U and Vt have
- * orthonormal columns/rows even for zero singular values.  Replaces
- * LapackSVDSolver<T>::Solve (NumericalMethods/LapackSVDSolver.C:42-90). */
+ * orthonormal columns/rows even for zero singular values.  FP64 one-sided
+ * block Jacobi on the device; for k >= 128 with columns far from orthogonal
+ * (largest cosine >= 0.1) the Jacobi runs on the triangular factor of a
+ * Householder QR of the norm-sorted columns (the environment variable
+ * TN_SVD_PLAIN=1 turns that off).  Replaces LapackSVDSolver<T>::Solve
+ * (NumericalMethods/LapackSVDSolver.C:42-90). */
 int tn_svd(tn_ctx* ctx, int m, int n, const double* A, double* U, double* s, double* Vt);
 /* Thin QR / LQ of a column-major m x n matrix on the device (shifted
  * CholeskyQR, FP64); replaces LapackQRSolver<T>::SolveThinQR / SolveThinLQ as
