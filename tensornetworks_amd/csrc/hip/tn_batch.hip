@@ -329,10 +329,12 @@ HeffLaunch heff_batch_stage(tn_batch* b, const std::vector<Slot>& S, const std::
     return h;
 }
 
-void heff_batch_launch(tn_batch* b, const HeffLaunch& h)
+// conv_done: the FP32 copies of x were written by the caller (fused with the
+// Lanczos scaling)
+void heff_batch_launch(tn_batch* b, const HeffLaunch& h, bool conv_done = false)
 {
     hipStream_t s = b->ctx->stream;
-    TN_HIP(bconvert(h.conv, h.nconv, h.nmax, s));
+    if (!conv_done) TN_HIP(bconvert(h.conv, h.nconv, h.nmax, s));
     gemm_group_launch(b, h.g1);
     TN_HIP(bwmix(T_F32, h.mix, h.nmix, h.Xmax, h.Bnmax, s));
     gemm_group_launch(b, h.g3);
@@ -436,7 +438,7 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             std::vector<Slot> sa;
             std::vector<const double*> xin;
             std::vector<double*> yout;
-            std::vector<ScaleJob> sc;
+            std::vector<ScaleConvJob> sc;
             long nmax = 1;
             for (int k : A) {
                 const long N = S[k].N;
@@ -446,14 +448,15 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
                 xin.push_back(vj);
                 yout.push_back(V(k) + (long)(j + 1) * N);
                 nmax = std::max(nmax, N);
-                // |w_{j-1}|^2 of iteration j-1
-                if (j > 0) sc.push_back(ScaleJob{0.0, vj, vj, N, h + nslot(j - 1)});
+                // v_j = w_{j-1} / |w_{j-1}| (|w_{j-1}|^2 of iteration j-1) and
+                // its FP32 copy for the Heff stages, in one pass
+                if (j > 0) sc.push_back(ScaleConvJob{0.0, vj, vj, b->xlo[S[k].q].as<float>(), N, h + nslot(j - 1)});
             }
             // every descriptor of the iteration in one transfer: the scaling,
             // the Heff stages, then the Lanczos step (alpha = v_j . w; three-term
             // step fused with the Gram-Schmidt dots; Gram-Schmidt update fused
             // with the norm, the kernels of tn::lanczos)
-            const ScaleJob* dsc = sc.empty() ? nullptr : stage(b, sc);
+            const ScaleConvJob* dsc = sc.empty() ? nullptr : stage(b, sc);
             HeffLaunch hl = heff_batch_stage(b, sa, xin, yout);
             std::vector<DotJob> d1;
             std::vector<StepJob> sj3;
@@ -476,9 +479,9 @@ void lanczos_batch(tn_batch* b, const std::vector<Slot>& S, double tol, int max_
             flush(b);
             const int na = (int)A.size();
             double* part = b->part.as<double>();
-            if (dsc) TN_HIP(bscale(dsc, (int)sc.size(), nmax, s));
+            if (dsc) TN_HIP(bscale_conv(dsc, (int)sc.size(), nmax, s));
             if (st) TN_HIP(hipEventRecord(c->ev[0], s));
-            heff_batch_launch(b, hl);
+            heff_batch_launch(b, hl, dsc != nullptr);
             if (st) TN_HIP(hipEventRecord(c->ev[1], s));
             TN_HIP(bmdot(dd1, na, 1, nmax, part, s));
             TN_HIP(bstep_dots(dsj, na, kl, nmax, part, s));

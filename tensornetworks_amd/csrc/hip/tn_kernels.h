@@ -283,6 +283,15 @@ struct ScaleJob {  // y = a x; with a_sq set, a = 1 / sqrt(*a_sq) read on the de
     const double* a_sq;
 };
 hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s);
+struct ScaleConvJob {  // y = a x and y32 = (float) y; with a_sq set, a = 1 / sqrt(*a_sq)
+    double a;
+    const double* x;
+    double* y;
+    float* y32;
+    long n;
+    const double* a_sq;
+};
+hipError_t bscale_conv(const ScaleConvJob* jobs, int n, long nmax, hipStream_t s);
 struct ConvJob {  // y = (float) x
     const double* x;
     float* y;

@@ -3041,6 +3041,30 @@ hipError_t bscale(const ScaleJob* jobs, int n, long nmax, hipStream_t s)
     return hipGetLastError();
 }
 
+// the scaling of the next Krylov vector and its FP32 copy in one pass
+__global__ void __launch_bounds__(256) bscale_conv_k(const ScaleConvJob* __restrict__ jobs)
+{
+    const ScaleConvJob& j = jobs[blockIdx.y];
+    const double a = j.a_sq ? 1.0 / sqrt(*j.a_sq) : j.a;
+    const double* x = j.x;  // y may be x (in place)
+    double* y = j.y;
+    float* __restrict__ y32 = j.y32;
+    const long n = j.n;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = a * x[i];
+        y[i] = v;
+        y32[i] = (float)v;
+    }
+}
+hipError_t bscale_conv(const ScaleConvJob* jobs, int n, long nmax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long blocks = (nmax + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    bscale_conv_k<<<dim3(blocks, n), 256, 0, s>>>(jobs);
+    return hipGetLastError();
+}
+
 // FP64 -> FP32 copy
 __global__ void __launch_bounds__(256) bconvert_k(const ConvJob* __restrict__ jobs)
 {
