@@ -224,7 +224,7 @@ def test_svd_matches_numpy(ctx, m, n, rank):
 
 @pytest.mark.parametrize("m,n,decay,rank", [(512, 512, 25.0, 512), (300, 160, 10.0, 160), (160, 300, 1e9, 160),
                                             (256, 256, 8.0, 200), (520, 200, 30.0, 200), (129, 700, 5.0, 129),
-                                            (1024, 1024, 40.0, 1024), (2048, 384, 20.0, 300)])
+                                            (1024, 1024, 40.0, 1024), (2048, 384, 20.0, 300), (256, 128, 8.0, 100)])
 def test_svd_graded_spectrum(ctx, m, n, decay, rank):
     """min(m, n) >= 128 takes the QR-preconditioned Jacobi.  Singular values
     exp(-j/decay) down to ~1e-28 (a two-site split's spectrum decays like this),
